@@ -1,0 +1,144 @@
+/* C-ABI of libshci_hip.so: the MI355X (gfx950) variational hot path of SHCI.
+ *
+ * The reference ("Arrow", QMC-Cornell/shci) computes this path on the CPU in
+ *   Hamiltonian<S>::update          src/solver/hamiltonian.h:87-598
+ *   SparseMatrix::mul               src/solver/sparse_matrix.cc:88-116
+ *   Davidson::diagonalize           src/solver/davidson.cc:37-225
+ *   RDM::get_1rdm / get_2rdm        src/chem/rdm.cc:12-131, 374-395
+ * Each entry point below names the reference interface it replaces.  All
+ * arguments are plain pointers and sizes; host pointers unless a flag says
+ * otherwise.  Every call returns SHCI_OK (0) or a negative status; the message
+ * is available from shci_last_error().  One context per process and device,
+ * calls serialised by the caller (like the reference's master thread).
+ *
+ * Determinants: one uint64 per spin (bit k = orbital k), n_orb <= 64.
+ * Integrals:    h1 dense n x n (symmetric, the value Integrals::get_1b returns);
+ *               eri packed by the reference key combine4(p,q,r,s) =
+ *               combine2(combine2(p,q), combine2(r,s)) (src/chem/integrals.cc:410-422),
+ *               one chemists' (pq|rs) per 8-fold class, length P(P+1)/2, P = n(n+1)/2.
+ */
+#ifndef SHCI_HIP_H_
+#define SHCI_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct shci_ctx shci_ctx;
+
+enum {
+  SHCI_OK = 0,
+  SHCI_ERR_HIP = -1,         /* a HIP runtime call failed */
+  SHCI_ERR_OOM = -2,         /* device allocation failed */
+  SHCI_ERR_UNSUPPORTED = -3, /* n_orb > 64, time_sym, ... */
+  SHCI_ERR_ARG = -4,         /* bad argument */
+  SHCI_ERR_STATE = -5        /* call order (e.g. SpMV before a build) */
+};
+
+/* Build / SpMV flags. */
+enum {
+  SHCI_STORE_FP64 = 1u << 0,   /* keep FP64 off-diagonal values (FP64 path) */
+  SHCI_STORE_FP32 = 1u << 1,   /* keep FP32 off-diagonal values (mixed path) */
+  SHCI_KEEP_ZEROS = 1u << 2,   /* keep |H_ij| < 1e-12 connections (exact RDM adjoint) */
+  SHCI_DEVICE_PTRS = 1u << 3,  /* vector/det arguments are device pointers */
+  SHCI_FP32 = 1u << 4          /* SpMV in FP32 storage / FP32 row accumulation */
+};
+
+/* Davidson modes. */
+enum {
+  SHCI_DAVIDSON_FP64 = 0,  /* reference algorithm, FP64 everywhere */
+  SHCI_DAVIDSON_MIXED = 1  /* FP32 CSR + vectors; FP64 diag, Gram-Schmidt, Rayleigh-Ritz, refinement */
+};
+
+/* Context lifetime.  No counterpart in the reference (it owns no device). */
+int shci_ctx_create(shci_ctx** ctx, int device);
+int shci_ctx_destroy(shci_ctx* ctx);
+const char* shci_last_error(const shci_ctx* ctx);
+
+/* Integrals as read by ChemSystem::get_hamiltonian_elem through
+ * Integrals::get_1b / get_2b (src/chem/integrals.cc:390-408). */
+int shci_set_integrals(shci_ctx* ctx, uint32_t n_orb, const double* h1, const double* eri_packed,
+                       double energy_core);
+
+/* Hamiltonian<S>::update (src/solver/hamiltonian.h:87-112).  dets up[i], dn[i]
+ * in the caller's order; the matrix is the full variational H over them
+ * (diagonal + every pair with excitation <= 2 and |H_ij| >= 1e-12,
+ * hamiltonian.h:440,466,506,527).  n_dets_prev follows the reference's
+ * append-only contract: n_dets == n_dets_prev returns at once (hamiltonian.h:91).
+ * flags: SHCI_STORE_FP64 and/or SHCI_STORE_FP32, SHCI_KEEP_ZEROS, SHCI_DEVICE_PTRS. */
+int shci_build_hamiltonian(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n_dets,
+                           uint64_t n_dets_prev, uint32_t flags);
+
+/* Rebuild from the determinants already resident on the device (same list as
+ * the last shci_build_hamiltonian); used to time device-resident builds. */
+int shci_rebuild_hamiltonian(shci_ctx* ctx, uint32_t flags);
+
+/* Hamiltonian<S>::update_existing_elems (hamiltonian.h:114-126,
+ * sparse_matrix.cc:191-201): recompute every stored value with the current
+ * integrals on the fixed connection pattern. */
+int shci_update_existing_elems(shci_ctx* ctx);
+
+/* SparseMatrix::count_n_elems (sparse_matrix.cc:73-86) and row count:
+ * n_stored_upper counts the diagonal + pairs i<j (the reference's storage);
+ * n_elems = 2 * n_stored_upper - n_dets (its printed "nonzero elems"). */
+int shci_get_counts(shci_ctx* ctx, uint64_t* n_dets, uint64_t* n_stored_upper, uint64_t* n_elems);
+
+/* SparseMatrix::get_diag() (sparse_matrix.h:16): FP64 diagonal, caller order. */
+int shci_get_diag(shci_ctx* ctx, double* diag);
+
+/* SparseMatrix::mul (sparse_matrix.cc:88-116): y = H x, caller order.
+ * FP64 by default; with SHCI_FP32 the FP32 store is used (values FP32, row
+ * accumulation FP32, diagonal FP64).  SHCI_DEVICE_PTRS: x, y on the device. */
+int shci_spmv(shci_ctx* ctx, const double* x, double* y, uint32_t flags);
+
+/* Block product Y = H X for k vectors (k <= 8), X, Y row-major k x n, caller
+ * order.  The reference declares a multi-vector mul (sparse_matrix.h:28-32)
+ * but never defines it; this is the block form used by the multi-root path. */
+int shci_spmm(shci_ctx* ctx, const double* X, double* Y, uint32_t k, uint32_t flags);
+
+/* Davidson::diagonalize (src/solver/davidson.cc:37-225) on the built H.
+ * init: n_states x n_dets (caller order); evals[n_states], evecs n_states x n.
+ * tol: eigenvalue-change tolerance (davidson.cc:203-210).  mode: SHCI_DAVIDSON_*.
+ * max_calls: the reference performs at most 2*5*n_states - n_states
+ * iterations per call and its caller re-invokes with the Ritz vectors
+ * (solver.h:430-530); max_calls > 1 repeats that warm restart here until
+ * converged.  traj_e/traj_t (optional, capacity traj_cap) receive the
+ * ground-state estimate and elapsed seconds after every Rayleigh-Ritz step;
+ * in MIXED mode the last entry is the FP64 re-evaluated energy. */
+int shci_davidson(shci_ctx* ctx, uint32_t n_states, const double* init, double tol, uint32_t mode,
+                  uint32_t max_calls, double* evals, double* evecs, int* converged, double* traj_e,
+                  double* traj_t, int traj_cap, int* n_traj);
+
+/* RDM::get_1rdm (rdm.cc:12-131) and RDM::get_2rdm(const SparseMatrix&)
+ * (rdm.cc:374-395) over the built connections, FP64 accumulation, states
+ * summed (rdm.cc:227,652).  coefs: n_states x n_dets (caller order).
+ * rdm1: n x n; rdm2: packed n^2(n^2+1)/2 in the reference layout
+ * (index a = p*n+s, b = q*n+r, a >= b, rdm.cc:644-657).  Either may be NULL.
+ * This is the adjoint of E(h, v): dE/dh = rdm1, dE/d(ps|qr) = D_pqrs / 2. */
+int shci_rdm(shci_ctx* ctx, uint32_t n_states, const double* coefs, double* rdm1, double* rdm2);
+
+/* SparseMatrix::get_row / get_connections (sparse_matrix.h:44-48): the stored
+ * upper triangle (diagonal + j > i) in caller indices.  row_ptr[n+1];
+ * cols/vals sized n_stored_upper (from shci_get_counts).  Any may be NULL
+ * except row_ptr. */
+int shci_export_upper(shci_ctx* ctx, int64_t* row_ptr, int64_t* cols, double* vals);
+
+/* Timing of the most recent build phases and a timed SpMV loop, measured with
+ * HIP events on the context's stream (milliseconds).  phases[8]: sort/group,
+ * singles, diagonal, count, scan+alloc, fill, compact, total. */
+int shci_get_build_timings(shci_ctx* ctx, float* phases);
+/* Average kernel time of `reps` SpMVs (device-resident vectors) and the
+ * algorithmic bytes one SpMV moves.  flags: SHCI_FP32 selects the FP32 store. */
+int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, double* bytes);
+
+/* Device memory in use by the context (bytes). */
+int shci_memory(shci_ctx* ctx, uint64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHCI_HIP_H_ */

@@ -1,0 +1,54 @@
+/* C API of libshci_host.so: the CPU side of the variational stage that stays
+ * on the host (integrals and heat-bath selection) plus the variational driver
+ * that calls the GPU path through include/shci_hip.h.
+ *
+ *   integrals           src/chem/integrals.cc:37-422 (FCIDUMP read, HF det, reorder)
+ *   heat-bath selection src/chem/chem_system.cc:74-402, src/solver/solver.h:435-495
+ *   variational loop    src/solver/solver.h:338-539 (run_all_variations / run_variation)
+ */
+#ifndef SHCI_HOST_H_
+#define SHCI_HOST_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* shci_host_last_error(void);
+
+/* Integrals::load (integrals.cc:17-35) without the binary cache.
+ * reorder: -1 reference default (!explicit occupations), 0 off, 1 on. */
+void* shci_host_system_from_fcidump(const char* path, unsigned n_up, unsigned n_dn, const unsigned* occs_up,
+                                    unsigned n_occs_up, const unsigned* occs_dn, unsigned n_occs_dn, int reorder);
+void* shci_host_system_from_arrays(unsigned n_orbs, unsigned n_up, unsigned n_dn, const double* h1_dense,
+                                   const double* eri_packed, double core, uint64_t hf_up, uint64_t hf_dn);
+void shci_host_system_free(void* sys);
+int shci_host_system_info(void* sys, unsigned* n_orbs, double* core, uint64_t* hf_up, uint64_t* hf_dn,
+                          size_t* eri_size, double* e_hf);
+int shci_host_system_integrals(void* sys, double* h1_dense, double* eri_packed, unsigned* orb_order);
+
+/* Variational determinant list (append-only, starts at the HF det). */
+int shci_host_reset_dets(void* sys);
+size_t shci_host_n_dets(void* sys);
+int shci_host_get_dets(void* sys, uint64_t* up, uint64_t* dn);
+int shci_host_set_dets(void* sys, const uint64_t* up, const uint64_t* dn, size_t n);
+
+/* One selection sweep of run_variation (solver.h:435-495); coefs n_states x n_dets.
+ * Returns the number of new determinants, or -1 on error. */
+long long shci_host_select(void* sys, const double* coefs, unsigned n_states, double eps_var);
+
+/* Solver::run_all_variations (solver.h:338-397) with the Hamiltonian build and
+ * Davidson on the GPU (include/shci_hip.h).  eps_vars / schedule as in the
+ * reference config; mode 0 = FP64 Davidson, 1 = mixed.  energies[n_states]
+ * receives the final variational energies. */
+int shci_host_run_variation(void* sys, const double* eps_vars, unsigned n_eps, const double* schedule,
+                            unsigned n_sched, double target_error, unsigned n_states, int mode, int device,
+                            int verbose, double* energies);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHCI_HOST_H_ */

@@ -1,0 +1,2189 @@
+// libshci_hip.so: the SHCI variational hot path written for MI355X (gfx950).
+//
+//   * Hamiltonian build (reference: src/solver/hamiltonian.h:87-598): packed
+//     uint64 alpha/beta strings, radix-sorted into (alpha, beta) order; alpha and
+//     beta groups; single-excitation partner lists of unique alpha strings from
+//     sorted "minus one electron" keys; one wavefront per row tests candidates
+//     with XOR/popcount, evaluates Slater-Condon elements in FP64 with the
+//     reference's exact operation order (bit-identical values, so the
+//     |H| >= 1e-12 cut keeps the same pattern), and writes a full-symmetric CSR
+//     (off-diagonal; FP64 diagonal kept apart) in two passes: count -> scan -> fill.
+//   * SpMV (reference: src/solver/sparse_matrix.cc:88-116): full-symmetric CSR,
+//     L lanes per row (L chosen from the mean row length), shuffle reductions, no
+//     atomics; FP64, FP32 (FP32 values + FP32 row accumulation, FP64 diagonal) and
+//     FP32-values/FP64-accumulation variants, plus a 4-vector block product.
+//   * Davidson (reference: src/solver/davidson.cc:37-225) with device vectors,
+//     device-resident Gram-Schmidt scalars and a host Jacobi Rayleigh-Ritz, in
+//     FP64 or mixed precision.
+//   * RDM adjoint (reference: src/chem/rdm.cc:12-131, 374-395, 435-657) over the
+//     same CSR with FP64 atomics.
+//
+// Everything runs on the context's single stream.
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "shci_hip.h"
+
+namespace {
+
+constexpr double kEps = 1.0e-12;  // src/util.h:50
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Device buffers and context
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+template <class T>
+inline T* P(const DevBuf& b) {
+  return static_cast<T*>(b.p);
+}
+
+struct SubspaceMat;
+
+}  // namespace
+
+struct shci_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  size_t mem_in_use = 0;
+
+  // Integrals.
+  uint32_t n_orb = 0;
+  double core = 0.0;
+  bool have_ints = false;
+  DevBuf h1, eri, tD, tX;
+  std::vector<double> h_h1, h_eri;
+
+  // Determinants in caller order and in internal (alpha, beta)-sorted order.
+  uint64_t n = 0;
+  DevBuf up_c, dn_c;
+  DevBuf up_s, dn_s, perm, iperm;
+  uint32_t n_up = 0, n_dn = 0;
+  uint32_t n_ugroups = 0, n_dgroups = 0;
+  DevBuf up_gid, up_gstart, dn_gid, dn_gstart, dnord_idx, dnord_up;
+  uint64_t n_singles = 0;
+  DevBuf sing_ptr, sing;
+
+  // Matrix.
+  bool built = false;
+  uint32_t build_flags = 0;
+  DevBuf diag, cnt, row_ptr, col, v64, v32, len;
+  uint64_t nnz = 0;  // stored off-diagonal entries (full symmetric)
+  int lanes = 16;
+
+  // Scratch.
+  DevBuf tmp, ka, kb, va, vb, scal, part;
+  DevBuf xv, yv, x32, y32;
+  DevBuf dav;  // Davidson vectors
+
+  hipEvent_t ev[9] = {};
+  float phase_ms[8] = {};
+};
+
+namespace {
+
+int fail(shci_ctx* c, int code, const std::string& msg) {
+  c->err = msg;
+  return code;
+}
+
+#define HIPC(...)                                                                         \
+  do {                                                                                    \
+    hipError_t _e = (__VA_ARGS__);                                                        \
+    if (_e != hipSuccess) {                                                               \
+      return fail(ctx, _e == hipErrorOutOfMemory ? SHCI_ERR_OOM : SHCI_ERR_HIP,           \
+                  std::string(#__VA_ARGS__) + ": " + hipGetErrorString(_e));              \
+    }                                                                                     \
+  } while (0)
+
+#define CHECK_RC(...)        \
+  do {                       \
+    int _rc = (__VA_ARGS__); \
+    if (_rc != SHCI_OK) return _rc; \
+  } while (0)
+
+int ensure(shci_ctx* ctx, DevBuf& b, size_t bytes) {
+  if (bytes <= b.bytes && b.p) return SHCI_OK;
+  if (b.p) {
+    HIPC(hipFree(b.p));
+    ctx->mem_in_use -= b.bytes;
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  if (bytes == 0) return SHCI_OK;
+  HIPC(hipMalloc(&b.p, bytes));
+  b.bytes = bytes;
+  ctx->mem_in_use += bytes;
+  return SHCI_OK;
+}
+
+void release(shci_ctx* ctx, DevBuf& b) {
+  if (b.p) {
+    (void)hipFree(b.p);
+    ctx->mem_in_use -= b.bytes;
+  }
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ int ctz64(uint64_t x) { return __builtin_ctzll(x); }
+
+// Workgroup id remapped so each XCD (8 on MI355X, assigned round-robin) sees a
+// contiguous range of rows: neighbouring rows share alpha groups in L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t G) {
+  const uint32_t X = 8, per = G / X, rem = G % X, x = b % X, idx = b / X;
+  return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
+}
+
+__device__ __forceinline__ uint64_t c2(uint64_t a, uint64_t b) {
+  return a > b ? (a * (a + 1)) / 2 + b : (b * (b + 1)) / 2 + a;
+}
+
+struct Ints {
+  const double* __restrict__ h1;   // n*n
+  const double* __restrict__ eri;  // combine4-packed
+  const double* __restrict__ tD;   // tD[(p*n+q)*n+k] = (pq|kk)
+  const double* __restrict__ tX;   // tX[(p*n+q)*n+k] = (pk|kq)
+  int n;
+  double core;
+};
+
+__device__ __forceinline__ double eri4(const Ints& I, int p, int q, int r, int s) {
+  return I.eri[c2(c2(p, q), c2(r, s))];
+}
+
+__device__ __forceinline__ int below(uint64_t s, int o) { return __popcll(s & ((1ull << o) - 1ull)); }
+
+// Single excitation p -> q in one spin, det_i on the left
+// (src/chem/chem_system.cc:544-577, same operation order).
+__device__ double single_elem(const Ints& I, uint64_t si, uint64_t sj, uint64_t oppo) {
+  const uint64_t lo = si & ~sj, ro = sj & ~si;
+  const int p = ctz64(lo), q = ctz64(ro);
+  const int cnt = below(si, p) + below(sj, q);
+  const double pf = (cnt & 1) ? -1.0 : 1.0;
+  const int n = I.n;
+  const double one = pf * I.h1[p * n + q];
+  const double* D = I.tD + (size_t)(p * n + q) * n;
+  const double* X = I.tX + (size_t)(p * n + q) * n;
+  double e = 0.0;
+  uint64_t m = si & ~(1ull << p);
+  uint64_t op = oppo;
+  while (m) {
+    const int k = ctz64(m);
+    m &= m - 1;
+    e -= X[k];
+    const double direct = D[k];
+    if ((op >> k) & 1ull) {
+      op &= ~(1ull << k);
+      e += 2 * direct;
+    } else {
+      e += direct;
+    }
+  }
+  while (op) {
+    const int k = ctz64(op);
+    op &= op - 1;
+    e += D[k];
+  }
+  e *= pf;
+  return one + e;
+}
+
+// Same-spin double (src/chem/chem_system.cc:581-598).
+__device__ double double_same(const Ints& I, uint64_t si, uint64_t sj) {
+  const uint64_t lo = si & ~sj, ro = sj & ~si;
+  const int i1 = ctz64(lo), i2 = ctz64(lo & (lo - 1));
+  const int j1 = ctz64(ro), j2 = ctz64(ro & (ro - 1));
+  const int cnt = below(si, i1) + below(si, i2) + below(sj, j1) + below(sj, j2);
+  double e = eri4(I, i1, j1, i2, j2) - eri4(I, i1, j2, i2, j1);
+  e *= (cnt & 1) ? -1.0 : 1.0;
+  return e;
+}
+
+// Opposite-spin double (src/chem/chem_system.cc:599-607).
+__device__ double double_mixed(const Ints& I, uint64_t ui, uint64_t uj, uint64_t di, uint64_t dj) {
+  const uint64_t lu = ui & ~uj, ru = uj & ~ui, ld = di & ~dj, rd = dj & ~di;
+  const int pu = ctz64(lu), qu = ctz64(ru), pd = ctz64(ld), qd = ctz64(rd);
+  const int cu = below(ui, pu) + below(uj, qu);
+  const int cd = below(di, pd) + below(dj, qd);
+  const int pf = ((cu & 1) ? -1 : 1) * ((cd & 1) ? -1 : 1);
+  double e = eri4(I, pu, qu, pd, qd);
+  e *= pf;
+  return e;
+}
+
+// Diagonal (src/chem/chem_system.cc:444-457 and 495-542, same order).
+__device__ double diag_elem(const Ints& I, uint64_t up, uint64_t dn) {
+  const int n = I.n;
+  double e1 = 0.0;
+  for (uint64_t m = up; m; m &= m - 1) {
+    const int k = ctz64(m);
+    e1 += I.h1[k * n + k];
+  }
+  if (up == dn) {
+    e1 *= 2;
+  } else {
+    for (uint64_t m = dn; m; m &= m - 1) {
+      const int k = ctz64(m);
+      e1 += I.h1[k * n + k];
+    }
+  }
+  double direct = 0.0, exchange = 0.0;
+  for (uint64_t a = up; a; a &= a - 1) {
+    const int i = ctz64(a);
+    const double* D = I.tD + (size_t)(i * n + i) * n;
+    const double* X = I.tX + (size_t)(i * n + i) * n;
+    for (uint64_t b = a & (a - 1); b; b &= b - 1) {
+      const int j = ctz64(b);
+      direct += D[j];
+      exchange -= X[j];
+    }
+  }
+  if (up == dn) {
+    direct *= 2;
+    exchange *= 2;
+  } else {
+    for (uint64_t a = dn; a; a &= a - 1) {
+      const int i = ctz64(a);
+      const double* D = I.tD + (size_t)(i * n + i) * n;
+      const double* X = I.tX + (size_t)(i * n + i) * n;
+      for (uint64_t b = a & (a - 1); b; b &= b - 1) {
+        const int j = ctz64(b);
+        direct += D[j];
+        exchange -= X[j];
+      }
+    }
+  }
+  for (uint64_t a = up; a; a &= a - 1) {
+    const int i = ctz64(a);
+    const double* D = I.tD + (size_t)(i * n + i) * n;
+    for (uint64_t b = dn; b; b &= b - 1) direct += D[ctz64(b)];
+  }
+  return e1 + (direct + exchange) + I.core;
+}
+
+// Any pair with excitation <= 2 (dispatch of chem_system.cc:409-442).
+__device__ double elem_any(const Ints& I, uint64_t ui, uint64_t di, uint64_t uj, uint64_t dj) {
+  const int cu = __popcll(ui ^ uj), cd = __popcll(di ^ dj);
+  if (cu == 0 && cd == 0) return diag_elem(I, ui, di);
+  if (cd == 0) return cu == 2 ? single_elem(I, ui, uj, di) : double_same(I, ui, uj);
+  if (cu == 0) return cd == 2 ? single_elem(I, di, dj, ui) : double_same(I, di, dj);
+  return double_mixed(I, ui, uj, di, dj);
+}
+
+// ---------------------------------------------------------------------------
+// Sort / grouping kernels
+// ---------------------------------------------------------------------------
+__global__ void k_iota(uint32_t* a, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = (uint32_t)i;
+}
+
+__global__ void k_gather_u64(const uint64_t* __restrict__ src, const uint32_t* __restrict__ idx,
+                             uint64_t* __restrict__ dst, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+__global__ void k_inverse_perm(const uint32_t* __restrict__ perm, uint32_t* __restrict__ iperm, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) iperm[perm[i]] = (uint32_t)i;
+}
+
+// flag[i] = 1 where a new run of equal keys starts.
+__global__ void k_head_flags(const uint64_t* __restrict__ keys, uint32_t* __restrict__ flag, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+// From inclusive scan of head flags: group id per element (scan-1) and group starts.
+__global__ void k_group_starts(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ incl,
+                               uint32_t* __restrict__ gid, uint32_t* __restrict__ gstart, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t g = incl[i] - 1;
+  gid[i] = g;
+  if (flag[i]) gstart[g] = (uint32_t)i;
+  if (i == n - 1) gstart[g + 1] = (uint32_t)n;
+}
+
+// dn-ordered arrays: dnord_up[k] = up_s[dnord_idx[k]], dn_gid[dnord_idx[k]] = group of k.
+__global__ void k_dn_order(const uint32_t* __restrict__ dnord_idx, const uint64_t* __restrict__ up_s,
+                           const uint32_t* __restrict__ kgid, uint64_t* __restrict__ dnord_up,
+                           uint32_t* __restrict__ dn_gid, uint64_t n) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = dnord_idx[k];
+  dnord_up[k] = up_s[i];
+  dn_gid[i] = kgid[k];
+}
+
+// Minus-one keys of unique alpha strings: key = alpha with one electron removed.
+__global__ void k_minus_one(const uint64_t* __restrict__ up_s, const uint32_t* __restrict__ gstart,
+                            uint32_t G, uint32_t n_e, uint64_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  uint64_t s = up_s[gstart[g]];
+  uint64_t m = s;
+  for (uint32_t e = 0; e < n_e; e++) {
+    const uint64_t bit = m & (~m + 1);
+    m &= m - 1;
+    keys[g * n_e + e] = s & ~bit;
+    vals[g * n_e + e] = (uint32_t)g;
+  }
+}
+
+// For sorted minus-one keys: every pair inside a run of equal keys is a pair of
+// single-excitation partners.  Count partners per alpha group.
+__global__ void k_singles_count(const uint32_t* __restrict__ run_start_of, const uint32_t* __restrict__ run_gstart,
+                                const uint32_t* __restrict__ vals, uint64_t m,
+                                unsigned long long* __restrict__ cnt) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  const uint32_t r = run_start_of[e];
+  const uint32_t len = run_gstart[r + 1] - run_gstart[r];
+  if (len > 1) atomicAdd(&cnt[vals[e]], (unsigned long long)(len - 1));
+}
+
+__global__ void k_singles_fill(const uint32_t* __restrict__ run_of, const uint32_t* __restrict__ run_gstart,
+                               const uint32_t* __restrict__ vals, uint64_t m,
+                               unsigned long long* __restrict__ cursor, uint64_t* __restrict__ pairs) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  const uint32_t r = run_of[e];
+  const uint32_t b = run_gstart[r], t = run_gstart[r + 1];
+  if (t - b < 2) return;
+  const uint32_t g = vals[e];
+  const unsigned long long pos = atomicAdd(&cursor[g], (unsigned long long)(t - b - 1));
+  uint64_t k = pos;
+  for (uint32_t f = b; f < t; f++) {
+    if (f == e) continue;
+    pairs[k++] = ((uint64_t)g << 32) | vals[f];
+  }
+}
+
+__global__ void k_low32(const uint64_t* __restrict__ pairs, uint32_t* __restrict__ out, uint64_t m) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) out[i] = (uint32_t)pairs[i];
+}
+
+__global__ void k_widen(const uint32_t* __restrict__ a, int64_t* __restrict__ b, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+// Input validation: every det within n_orb orbitals with the same electron
+// counts as det 0 (the search and the integral tables assume both).
+__global__ void k_validate(const uint64_t* __restrict__ up, const uint64_t* __restrict__ dn, uint64_t n,
+                           uint64_t mask, int nu, int nd, unsigned int* __restrict__ bad) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t u = up[i], d = dn[i];
+  if ((u & mask) || (d & mask) || __popcll(u) != nu || __popcll(d) != nd) atomicAdd(&bad[0], 1u);
+}
+
+// After the (alpha, beta) sort equal neighbours are duplicate determinants.
+__global__ void k_duplicates(const uint64_t* __restrict__ up, const uint64_t* __restrict__ dn, uint64_t n,
+                             unsigned int* __restrict__ bad) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 || i >= n) return;
+  if (up[i] == up[i - 1] && dn[i] == dn[i - 1]) atomicAdd(&bad[1], 1u);
+}
+
+__global__ void k_diag(Ints I, const uint64_t* __restrict__ up, const uint64_t* __restrict__ dn,
+                       double* __restrict__ diag, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) diag[i] = diag_elem(I, up[i], dn[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Row kernels: connection search (count / fill), one wavefront per row.
+// ---------------------------------------------------------------------------
+struct RowParams {
+  uint64_t n;
+  const uint64_t* __restrict__ up;
+  const uint64_t* __restrict__ dn;
+  const uint32_t* __restrict__ up_gid;
+  const uint32_t* __restrict__ up_gstart;
+  const uint32_t* __restrict__ dn_gid;
+  const uint32_t* __restrict__ dn_gstart;
+  const uint32_t* __restrict__ dnord_idx;
+  const uint64_t* __restrict__ dnord_up;
+  const uint64_t* __restrict__ sing_ptr;
+  const uint32_t* __restrict__ sing;
+  Ints I;
+  uint32_t* __restrict__ cnt;
+  const int64_t* __restrict__ row_ptr;
+  uint32_t* __restrict__ col;
+  double* __restrict__ v64;
+  float* __restrict__ v32;
+  uint32_t* __restrict__ len;
+  int keep_zeros;
+};
+
+template <bool FILL>
+struct Emitter {
+  uint64_t base;  // FILL: next output slot; COUNT: running count
+  __device__ __forceinline__ void hit_only(bool hit) { base += __popcll(__ballot(hit)); }
+  __device__ __forceinline__ void emit(const RowParams& Pm, bool hit, uint32_t j, double v) {
+    const bool keep = hit && (Pm.keep_zeros || fabs(v) >= kEps);
+    const uint64_t m = __ballot(keep);
+    if (keep) {
+      const uint64_t pos = base + mbcnt(m);
+      Pm.col[pos] = j;
+      if (Pm.v64) Pm.v64[pos] = v;
+      if (Pm.v32) Pm.v32[pos] = (float)v;
+    }
+    base += __popcll(m);
+  }
+};
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_rows(RowParams Pm) {
+  const uint32_t lane = lane_id();
+  const uint64_t r = (uint64_t)xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  if (r >= Pm.n) return;
+  const uint64_t up_i = Pm.up[r], dn_i = Pm.dn[r];
+  Emitter<FILL> em;
+  em.base = FILL ? (uint64_t)Pm.row_ptr[r] : 0;
+
+  // (A) same beta string: single/double alpha excitations (hamiltonian.h:430-442).
+  {
+    const uint32_t g = Pm.dn_gid[r];
+    const uint32_t b = Pm.dn_gstart[g], e = Pm.dn_gstart[g + 1];
+    for (uint32_t k0 = b; k0 < e; k0 += kWave) {
+      const uint32_t k = k0 + lane;
+      const bool valid = k < e;
+      const uint64_t u = valid ? Pm.dnord_up[k] : up_i;
+      const int c = __popcll(u ^ up_i);
+      const bool hit = valid && (c == 2 || c == 4);
+      if (!FILL) {
+        em.hit_only(hit);
+      } else {
+        double v = 0.0;
+        uint32_t j = 0;
+        if (hit) {
+          j = Pm.dnord_idx[k];
+          v = (c == 2) ? single_elem(Pm.I, up_i, u, dn_i) : double_same(Pm.I, up_i, u);
+        }
+        em.emit(Pm, hit, j, v);
+      }
+    }
+  }
+  // (B) same alpha string: single/double beta excitations (hamiltonian.h:456-468).
+  const uint32_t ga = Pm.up_gid[r];
+  {
+    const uint32_t b = Pm.up_gstart[ga], e = Pm.up_gstart[ga + 1];
+    for (uint32_t k0 = b; k0 < e; k0 += kWave) {
+      const uint32_t k = k0 + lane;
+      const bool valid = k < e;
+      const uint64_t d = valid ? Pm.dn[k] : dn_i;
+      const int c = __popcll(d ^ dn_i);
+      const bool hit = valid && (c == 2 || c == 4);
+      if (!FILL) {
+        em.hit_only(hit);
+      } else {
+        double v = 0.0;
+        if (hit) v = (c == 2) ? single_elem(Pm.I, dn_i, d, up_i) : double_same(Pm.I, dn_i, d);
+        em.emit(Pm, hit, k, v);
+      }
+    }
+  }
+  // (C) alpha single x beta single (hamiltonian.h:482-532): scan the alpha
+  // groups of every single-excitation partner of this alpha string, flattened
+  // across partners so all 64 lanes stay busy on short groups.
+  {
+    const uint64_t ps = Pm.sing_ptr[ga], pe = Pm.sing_ptr[ga + 1];
+    for (uint64_t pb = ps; pb < pe; pb += kWave) {
+      const uint64_t p = pb + lane;
+      uint32_t st = 0, ln = 0;
+      if (p < pe) {
+        const uint32_t g2 = Pm.sing[p];
+        st = Pm.up_gstart[g2];
+        ln = Pm.up_gstart[g2 + 1] - st;
+      }
+      uint32_t incl = ln;
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d, kWave);
+        if ((int)lane >= d) incl += y;
+      }
+      const uint32_t excl = incl - ln;
+      const uint32_t total = __shfl(incl, kWave - 1, kWave);
+      for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
+        const uint32_t t = t0 + lane;
+        const bool valid = t < total;
+        int lo = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+          const int cand = lo + step;
+          const uint32_t ec = __shfl(excl, cand < kWave ? cand : kWave - 1, kWave);
+          if (cand < kWave && ec <= t) lo = cand;
+        }
+        const uint32_t stl = __shfl(st, lo, kWave);
+        const uint32_t exl = __shfl(excl, lo, kWave);
+        const uint32_t k = valid ? stl + (t - exl) : 0;
+        const uint64_t d = valid ? Pm.dn[k] : dn_i;
+        const bool hit = valid && __popcll(d ^ dn_i) == 2;
+        if (!FILL) {
+          em.hit_only(hit);
+        } else {
+          double v = 0.0;
+          if (hit) v = double_mixed(Pm.I, up_i, Pm.up[k], dn_i, d);
+          em.emit(Pm, hit, k, v);
+        }
+      }
+    }
+  }
+  if (lane == 0) {
+    if (FILL) {
+      Pm.len[r] = (uint32_t)(em.base - (uint64_t)Pm.row_ptr[r]);
+    } else {
+      Pm.cnt[r] = (uint32_t)em.base;
+    }
+  }
+}
+
+// Move rows into an exact-size CSR when the |H| cut dropped entries.
+__global__ __launch_bounds__(256) void k_compact(uint64_t n, const int64_t* __restrict__ old_rp,
+                                                 const int64_t* __restrict__ new_rp, const uint32_t* __restrict__ col_in,
+                                                 const double* __restrict__ v64_in, const float* __restrict__ v32_in,
+                                                 uint32_t* __restrict__ col_out, double* __restrict__ v64_out,
+                                                 float* __restrict__ v32_out) {
+  const uint64_t r = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= n) return;
+  const int64_t b = old_rp[r], nb = new_rp[r], m = new_rp[r + 1] - nb;
+  for (int64_t k = lane_id(); k < m; k += kWave) {
+    col_out[nb + k] = col_in[b + k];
+    if (v64_out) v64_out[nb + k] = v64_in[b + k];
+    if (v32_out) v32_out[nb + k] = v32_in[b + k];
+  }
+}
+
+// Recompute values on a fixed pattern (update_existing_elems).
+__global__ __launch_bounds__(256) void k_update_values(uint64_t n, Ints I, const uint64_t* __restrict__ up,
+                                                       const uint64_t* __restrict__ dn, const int64_t* __restrict__ rp,
+                                                       const uint32_t* __restrict__ col, double* __restrict__ v64,
+                                                       float* __restrict__ v32, double* __restrict__ diag) {
+  const uint64_t r = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= n) return;
+  const uint64_t ui = up[r], di = dn[r];
+  if (lane_id() == 0) diag[r] = diag_elem(I, ui, di);
+  for (int64_t k = rp[r] + lane_id(); k < rp[r + 1]; k += kWave) {
+    const uint32_t j = col[k];
+    const double v = elem_any(I, ui, di, up[j], dn[j]);
+    if (v64) v64[k] = v;
+    if (v32) v32[k] = (float)v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SpMV kernels: L lanes per row, 256 threads per block.
+//   y_i = (d_i - shift) x_i + sum_j A_ij x_j
+// VT: stored value type; XT: vector type; AT: row accumulator type.
+// ---------------------------------------------------------------------------
+template <int L, typename VT, typename XT, typename AT>
+__global__ __launch_bounds__(256) void k_spmv(uint64_t n, const int64_t* __restrict__ rp,
+                                              const uint32_t* __restrict__ col, const VT* __restrict__ val,
+                                              const double* __restrict__ diag, double shift,
+                                              const XT* __restrict__ x, XT* __restrict__ y) {
+  const uint32_t sub = threadIdx.x & (L - 1);
+  const uint64_t row = (uint64_t)xcd_remap(blockIdx.x, gridDim.x) * (256 / L) + threadIdx.x / L;
+  AT acc0 = 0, acc1 = 0;
+  if (row < n) {
+    const int64_t b = rp[row], e = rp[row + 1];
+    int64_t k = b + sub;
+    for (; k + L < e; k += 2 * L) {
+      const uint32_t c0 = __builtin_nontemporal_load(&col[k]);
+      const uint32_t c1 = __builtin_nontemporal_load(&col[k + L]);
+      const VT a0 = __builtin_nontemporal_load(&val[k]);
+      const VT a1 = __builtin_nontemporal_load(&val[k + L]);
+      acc0 += (AT)a0 * (AT)x[c0];
+      acc1 += (AT)a1 * (AT)x[c1];
+    }
+    if (k < e) {
+      const uint32_t c0 = __builtin_nontemporal_load(&col[k]);
+      const VT a0 = __builtin_nontemporal_load(&val[k]);
+      acc0 += (AT)a0 * (AT)x[c0];
+    }
+  }
+  AT acc = acc0 + acc1;
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, L);
+  if (row < n && sub == 0) {
+    const double dx = (diag[row] - shift) * (double)x[row];
+    y[row] = (XT)(dx + (double)acc);
+  }
+}
+
+// Block product for K vectors stored interleaved (x[i*K + v]).
+template <int L, int K>
+__global__ __launch_bounds__(256) void k_spmm_f32(uint64_t n, const int64_t* __restrict__ rp,
+                                                  const uint32_t* __restrict__ col, const float* __restrict__ val,
+                                                  const double* __restrict__ diag, double shift,
+                                                  const float* __restrict__ x, float* __restrict__ y) {
+  const uint32_t sub = threadIdx.x & (L - 1);
+  const uint64_t row = (uint64_t)xcd_remap(blockIdx.x, gridDim.x) * (256 / L) + threadIdx.x / L;
+  float acc[K];
+#pragma unroll
+  for (int v = 0; v < K; v++) acc[v] = 0.f;
+  if (row < n) {
+    const int64_t b = rp[row], e = rp[row + 1];
+    for (int64_t k = b + sub; k < e; k += L) {
+      const uint32_t c = __builtin_nontemporal_load(&col[k]);
+      const float a = __builtin_nontemporal_load(&val[k]);
+      const float* xc = x + (size_t)c * K;
+#pragma unroll
+      for (int v = 0; v < K; v++) acc[v] += a * xc[v];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < K; v++) {
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) acc[v] += __shfl_xor(acc[v], o, L);
+  }
+  if (row < n && sub == 0) {
+    const double d = diag[row] - shift;
+#pragma unroll
+    for (int v = 0; v < K; v++) y[(size_t)row * K + v] = (float)(d * (double)x[(size_t)row * K + v] + (double)acc[v]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Vector kernels (Davidson).
+// ---------------------------------------------------------------------------
+constexpr int kDotBlocks = 1024;
+constexpr int kMaxDots = 24;
+
+struct DotArgs {
+  const void* b[kMaxDots];
+};
+
+// Partial dot products of a against m vectors: part[j*kDotBlocks + block].
+template <typename T>
+__global__ __launch_bounds__(256) void k_dots_partial(const T* __restrict__ a, DotArgs B, int m, uint64_t n,
+                                                      double* __restrict__ part) {
+  __shared__ double red[256];
+  for (int j = 0; j < m; j++) {
+    const T* __restrict__ bj = static_cast<const T*>(B.b[j]);
+    double s = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+      s += (double)a[i] * (double)bj[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[j * kDotBlocks + blockIdx.x] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dots_final(const double* __restrict__ part, int m, int nblk,
+                                                    double* __restrict__ out) {
+  __shared__ double red[256];
+  for (int j = 0; j < m; j++) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += 256) s += part[j * kDotBlocks + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = red[0];
+    __syncthreads();
+  }
+}
+
+// v -= s[0] * w  (s a device scalar; arithmetic in FP64).
+template <typename T>
+__global__ void k_axpy_dev(T* __restrict__ v, const T* __restrict__ w, const double* __restrict__ s, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (T)((double)v[i] - s[0] * (double)w[i]);
+}
+
+// v *= 1/norm with norm a host value.
+template <typename T>
+__global__ void k_scale(T* __restrict__ v, double inv, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (T)((double)v[i] * inv);
+}
+
+// v = x / norm (copy with scaling), possibly converting precision.
+template <typename TI, typename TO>
+__global__ void k_copy_scale(const TI* __restrict__ x, TO* __restrict__ v, double inv, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (TO)((double)x[i] * inv);
+}
+
+// Diagonal preconditioner (davidson.cc:135-143): lam and diag in FP64.
+template <typename T>
+__global__ void k_precond(T* __restrict__ v, const T* __restrict__ Hw, const T* __restrict__ w,
+                          const double* __restrict__ diag, double lam, double shift, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double d = lam - (diag[i] - shift);
+  v[i] = fabs(d) < 1.0e-8 ? (T)0 : (T)(((double)Hw[i] - lam * (double)w[i]) / d);
+}
+
+struct RitzArgs {
+  const void* v[kMaxDots];
+  const void* hv[kMaxDots];
+  double c[kMaxDots];
+};
+
+// w = sum_i c_i v_i, Hw = sum_i c_i Hv_i (davidson.cc:184-194), FP64 accumulation.
+template <typename T, typename TO>
+__global__ void k_ritz(RitzArgs R, int m, uint64_t n, TO* __restrict__ w, TO* __restrict__ hw) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double a = 0.0, b = 0.0;
+  for (int i = 0; i < m; i++) {
+    a += (double)static_cast<const T*>(R.v[i])[j] * R.c[i];
+    if (hw) b += (double)static_cast<const T*>(R.hv[i])[j] * R.c[i];
+  }
+  w[j] = (TO)a;
+  if (hw) hw[j] = (TO)b;
+}
+
+// caller-order <-> internal-order permutations.
+template <typename TI, typename TO>
+__global__ void k_to_internal(const TI* __restrict__ xc, const uint32_t* __restrict__ perm, TO* __restrict__ xi,
+                              uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) xi[i] = (TO)xc[perm[i]];
+}
+template <typename TI, typename TO>
+__global__ void k_to_caller(const TI* __restrict__ xi, const uint32_t* __restrict__ perm, TO* __restrict__ xc,
+                            uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) xc[perm[i]] = (TO)xi[i];
+}
+// interleave k vectors (row-major k x n, caller order) into internal x[i*K+v].
+__global__ void k_interleave(const double* __restrict__ X, const uint32_t* __restrict__ perm, float* __restrict__ xi,
+                             uint32_t K, uint32_t Kp, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t c = perm[i];
+  for (uint32_t v = 0; v < Kp; v++) xi[i * Kp + v] = v < K ? (float)X[(size_t)v * n + c] : 0.f;
+}
+__global__ void k_deinterleave(const float* __restrict__ yi, const uint32_t* __restrict__ perm, double* __restrict__ Y,
+                               uint32_t K, uint32_t Kp, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t c = perm[i];
+  for (uint32_t v = 0; v < K; v++) Y[(size_t)v * n + c] = (double)yi[i * Kp + v];
+}
+
+// ---------------------------------------------------------------------------
+// RDM adjoint kernel (src/chem/rdm.cc).  Each unordered connected pair (and
+// each diagonal) is visited once: row r, entries with col > r.
+// ---------------------------------------------------------------------------
+struct RdmParams {
+  uint64_t n;
+  int n_orb;
+  uint32_t n_states;
+  const uint64_t* __restrict__ up;
+  const uint64_t* __restrict__ dn;
+  const int64_t* __restrict__ rp;
+  const uint32_t* __restrict__ col;
+  const double* __restrict__ c;  // n_states x n, internal order
+  double* __restrict__ rdm1;
+  double* __restrict__ rdm2;
+};
+
+__device__ __forceinline__ double cc(const RdmParams& R, uint64_t i, uint64_t j) {
+  double v = 0.0;
+  for (uint32_t s = 0; s < R.n_states; s++) v += R.c[s * R.n + i] * R.c[s * R.n + j];
+  return v;
+}
+
+// write_in_2rdm (rdm.cc:644-657): packed a = p*n+s, b = q*n+r, stored when a >= b.
+__device__ __forceinline__ void w2(const RdmParams& R, int p, int q, int r, int s, double f, double cij) {
+  const uint64_t n = (uint64_t)R.n_orb;
+  const uint64_t a = (uint64_t)p * n + s, b = (uint64_t)q * n + r;
+  if (a >= b) atomicAdd(&R.rdm2[(a * (a + 1)) / 2 + b], f * cij);
+}
+
+// permfac_ccaa (rdm.cc:243-269): sign of c+_p c+_q c_r c_s |h>.
+__device__ __forceinline__ double permfac_ccaa(uint64_t h, int p, int q, int r, int s) {
+  int c = below(h, s);
+  h &= ~(1ull << s);
+  c += below(h, r);
+  h &= ~(1ull << r);
+  c += below(h, q);
+  h |= 1ull << q;
+  c += below(h, p);
+  return (c & 1) ? -1.0 : 1.0;
+}
+
+__device__ __forceinline__ double pf_of(uint64_t l, uint64_t rr) {
+  // diff(l, rr).permutation_factor (half_det.cc:201-287)
+  int c = 0;
+  for (uint64_t m = l & ~rr; m; m &= m - 1) c += below(l, ctz64(m));
+  for (uint64_t m = rr & ~l; m; m &= m - 1) c += below(rr, ctz64(m));
+  return (c & 1) ? -1.0 : 1.0;
+}
+
+// rdm.cc:435-642 with connected = (uc, dc) at j and this = (ut, dt) at i.
+__device__ void rdm_pair(const RdmParams& R, uint64_t ut, uint64_t dt, uint64_t i, uint64_t uc, uint64_t dc,
+                         uint64_t j) {
+  const double cij = cc(R, i, j);
+  const int n = R.n_orb;
+  if (ut == uc) {
+    if (dt == dc) {
+      // 1-RDM diagonal: c_i^2 for each occupied orbital of either spin.
+      for (uint64_t m = ut; m; m &= m - 1) {
+        const int p = ctz64(m);
+        atomicAdd(&R.rdm1[p * n + p], cij);
+      }
+      for (uint64_t m = dt; m; m &= m - 1) {
+        const int p = ctz64(m);
+        atomicAdd(&R.rdm1[p * n + p], cij);
+      }
+      for (uint64_t a = ut; a; a &= a - 1)
+        for (uint64_t b = a & (a - 1); b; b &= b - 1) {
+          const int s = ctz64(a), r = ctz64(b);
+          w2(R, s, r, r, s, 1.0, cij);
+          w2(R, s, r, s, r, -1.0, cij);
+          w2(R, r, s, s, r, 1.0, cij);
+          w2(R, r, s, r, s, -1.0, cij);
+        }
+      for (uint64_t a = dt; a; a &= a - 1)
+        for (uint64_t b = a & (a - 1); b; b &= b - 1) {
+          const int s = ctz64(a), r = ctz64(b);
+          w2(R, s, r, r, s, 1.0, cij);
+          w2(R, s, r, s, r, -1.0, cij);
+          w2(R, r, s, s, r, 1.0, cij);
+          w2(R, r, s, r, s, -1.0, cij);
+        }
+      for (uint64_t a = ut; a; a &= a - 1)
+        for (uint64_t b = dt; b; b &= b - 1) {
+          const int s = ctz64(a), r = ctz64(b);
+          w2(R, s, r, r, s, 1.0, cij);
+          w2(R, r, s, s, r, 1.0, cij);
+        }
+      return;
+    }
+    const int nd = __popcll(dc ^ dt) / 2;
+    if (nd == 1) {
+      const int b1 = ctz64(dt & ~dc), b2 = ctz64(dc & ~dt);  // from (this), to (connected)
+      const double pf = pf_of(dc, dt);
+      // 1-RDM, both orientations (rdm.cc:105-124).
+      atomicAdd(&R.rdm1[b1 * n + b2], pf * cij);
+      atomicAdd(&R.rdm1[b2 * n + b1], pf * cij);
+      for (uint64_t m = dt; m; m &= m - 1) {
+        const int p = ctz64(m);
+        if (p == b1 || p == b2) continue;
+        double sf = permfac_ccaa(dt, p, b2, b1, p);
+        w2(R, p, b2, b1, p, sf, cij);
+        w2(R, b2, p, b1, p, -sf, cij);
+        w2(R, p, b2, p, b1, -sf, cij);
+        w2(R, b2, p, p, b1, sf, cij);
+        sf = permfac_ccaa(dc, p, b1, b2, p);
+        w2(R, p, b1, b2, p, sf, cij);
+        w2(R, b1, p, b2, p, -sf, cij);
+        w2(R, p, b1, p, b2, -sf, cij);
+        w2(R, b1, p, p, b2, sf, cij);
+      }
+      const double sf1 = pf, sf2 = pf_of(dt, dc);
+      for (uint64_t m = ut; m; m &= m - 1) {
+        const int p = ctz64(m);
+        w2(R, p, b2, b1, p, sf1, cij);
+        w2(R, b2, p, p, b1, sf1, cij);
+        w2(R, p, b1, b2, p, sf2, cij);
+        w2(R, b1, p, p, b2, sf2, cij);
+      }
+    } else if (nd == 2) {
+      const uint64_t ro = dt & ~dc, lo = dc & ~dt;
+      const int b1 = ctz64(ro), b2 = ctz64(ro & (ro - 1)), b3 = ctz64(lo), b4 = ctz64(lo & (lo - 1));
+      double sf = pf_of(dc, dt);
+      w2(R, b3, b4, b2, b1, sf, cij);
+      w2(R, b3, b4, b1, b2, -sf, cij);
+      w2(R, b4, b3, b2, b1, -sf, cij);
+      w2(R, b4, b3, b1, b2, sf, cij);
+      sf = pf_of(dt, dc);
+      w2(R, b1, b2, b4, b3, sf, cij);
+      w2(R, b1, b2, b3, b4, -sf, cij);
+      w2(R, b2, b1, b4, b3, -sf, cij);
+      w2(R, b2, b1, b3, b4, sf, cij);
+    }
+    return;
+  }
+  const int na = __popcll(uc ^ ut) / 2;
+  if (na == 1) {
+    const int a1 = ctz64(ut & ~uc), a2 = ctz64(uc & ~ut);
+    if (dt == dc) {
+      const double pf = pf_of(uc, ut);
+      atomicAdd(&R.rdm1[a1 * n + a2], pf * cij);
+      atomicAdd(&R.rdm1[a2 * n + a1], pf * cij);
+      for (uint64_t m = ut; m; m &= m - 1) {
+        const int p = ctz64(m);
+        if (p == a1 || p == a2) continue;
+        double sf = permfac_ccaa(ut, p, a2, a1, p);
+        w2(R, p, a2, a1, p, sf, cij);
+        w2(R, p, a2, p, a1, -sf, cij);
+        w2(R, a2, p, a1, p, -sf, cij);
+        w2(R, a2, p, p, a1, sf, cij);
+        sf = permfac_ccaa(uc, p, a1, a2, p);
+        w2(R, p, a1, a2, p, sf, cij);
+        w2(R, p, a1, p, a2, -sf, cij);
+        w2(R, a1, p, a2, p, -sf, cij);
+        w2(R, a1, p, p, a2, sf, cij);
+      }
+      const double sf1 = pf, sf2 = pf_of(ut, uc);
+      for (uint64_t m = dt; m; m &= m - 1) {
+        const int p = ctz64(m);
+        w2(R, a2, p, p, a1, sf1, cij);
+        w2(R, p, a2, a1, p, sf1, cij);
+        w2(R, a1, p, p, a2, sf2, cij);
+        w2(R, p, a1, a2, p, sf2, cij);
+      }
+    } else if (__popcll(dc ^ dt) == 2) {
+      const int b1 = ctz64(dt & ~dc), b2 = ctz64(dc & ~dt);
+      double sf = pf_of(uc, ut) * pf_of(dc, dt);
+      w2(R, a2, b2, b1, a1, sf, cij);
+      w2(R, b2, a2, a1, b1, sf, cij);
+      sf = pf_of(ut, uc) * pf_of(dt, dc);
+      w2(R, a1, b1, b2, a2, sf, cij);
+      w2(R, b1, a1, a2, b2, sf, cij);
+    }
+  } else if (na == 2 && dt == dc) {
+    const uint64_t ro = ut & ~uc, lo = uc & ~ut;
+    const int a1 = ctz64(ro), a2 = ctz64(ro & (ro - 1)), a3 = ctz64(lo), a4 = ctz64(lo & (lo - 1));
+    double sf = pf_of(uc, ut);
+    w2(R, a3, a4, a2, a1, sf, cij);
+    w2(R, a3, a4, a1, a2, -sf, cij);
+    w2(R, a4, a3, a2, a1, -sf, cij);
+    w2(R, a4, a3, a1, a2, sf, cij);
+    sf = pf_of(ut, uc);
+    w2(R, a1, a2, a4, a3, sf, cij);
+    w2(R, a1, a2, a3, a4, -sf, cij);
+    w2(R, a2, a1, a4, a3, -sf, cij);
+    w2(R, a2, a1, a3, a4, sf, cij);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rdm(RdmParams R) {
+  const uint64_t r = (uint64_t)xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  if (r >= R.n) return;
+  const uint64_t ut = R.up[r], dt = R.dn[r];
+  if (lane_id() == 0) rdm_pair(R, ut, dt, r, ut, dt, r);
+  for (int64_t k = R.rp[r] + lane_id(); k < R.rp[r + 1]; k += kWave) {
+    const uint32_t j = R.col[k];
+    if (j <= r) continue;
+    rdm_pair(R, ut, dt, r, R.up[j], R.dn[j], j);
+  }
+}
+
+// Export helpers: count, per caller row, the stored upper entries (diag + j>i).
+__global__ __launch_bounds__(256) void k_export_count(uint64_t n, const int64_t* __restrict__ rp,
+                                                      const uint32_t* __restrict__ col,
+                                                      const uint32_t* __restrict__ perm,
+                                                      unsigned long long* __restrict__ cnt_caller) {
+  const uint64_t r = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= n) return;
+  const uint32_t ci = perm[r];
+  uint32_t c = 0;
+  for (int64_t k = rp[r] + lane_id(); k < rp[r + 1]; k += kWave) c += perm[col[k]] > ci ? 1u : 0u;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, kWave);
+  if (lane_id() == 0) cnt_caller[ci] = c + 1;
+}
+
+__global__ __launch_bounds__(256) void k_export_fill(uint64_t n, const int64_t* __restrict__ rp,
+                                                     const uint32_t* __restrict__ col, const double* __restrict__ v64,
+                                                     const double* __restrict__ diag, const uint32_t* __restrict__ perm,
+                                                     const unsigned long long* __restrict__ out_rp,
+                                                     int64_t* __restrict__ out_col, double* __restrict__ out_val) {
+  const uint64_t r = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= n) return;
+  const uint32_t ci = perm[r];
+  uint64_t base = out_rp[ci];
+  if (lane_id() == 0) {
+    out_col[base] = ci;
+    out_val[base] = diag[r];
+  }
+  base += 1;
+  for (int64_t k0 = rp[r]; k0 < rp[r + 1]; k0 += kWave) {
+    const int64_t k = k0 + lane_id();
+    const bool valid = k < rp[r + 1];
+    const uint32_t cj = valid ? perm[col[k]] : 0;
+    const bool keep = valid && cj > ci;
+    const uint64_t m = __ballot(keep);
+    if (keep) {
+      const uint64_t pos = base + mbcnt(m);
+      out_col[pos] = cj;
+      out_val[pos] = v64 ? v64[k] : 0.0;
+    }
+    base += __popcll(m);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side helpers
+// ---------------------------------------------------------------------------
+int radix_sort_pairs(shci_ctx* ctx, const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout,
+                     uint64_t n, int end_bit) {
+  size_t tb = 0;
+  HIPC(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kin, kout, vin, vout, (int)n, 0, end_bit, ctx->stream));
+  CHECK_RC(ensure(ctx, ctx->tmp, tb));
+  HIPC(hipcub::DeviceRadixSort::SortPairs(ctx->tmp.p, tb, kin, kout, vin, vout, (int)n, 0, end_bit, ctx->stream));
+  return SHCI_OK;
+}
+
+int sort_keys_u64(shci_ctx* ctx, const uint64_t* kin, uint64_t* kout, uint64_t n, int end_bit) {
+  size_t tb = 0;
+  HIPC(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, kin, kout, (int)n, 0, end_bit, ctx->stream));
+  CHECK_RC(ensure(ctx, ctx->tmp, tb));
+  HIPC(hipcub::DeviceRadixSort::SortKeys(ctx->tmp.p, tb, kin, kout, (int)n, 0, end_bit, ctx->stream));
+  return SHCI_OK;
+}
+
+template <typename T>
+int incl_scan(shci_ctx* ctx, const T* in, T* out, uint64_t n) {
+  size_t tb = 0;
+  HIPC(hipcub::DeviceScan::InclusiveSum(nullptr, tb, in, out, (int)n, ctx->stream));
+  CHECK_RC(ensure(ctx, ctx->tmp, tb));
+  HIPC(hipcub::DeviceScan::InclusiveSum(ctx->tmp.p, tb, in, out, (int)n, ctx->stream));
+  return SHCI_OK;
+}
+
+template <typename TI, typename TO>
+int excl_scan(shci_ctx* ctx, const TI* in, TO* out, uint64_t n) {
+  size_t tb = 0;
+  HIPC(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, (int)n, ctx->stream));
+  CHECK_RC(ensure(ctx, ctx->tmp, tb));
+  HIPC(hipcub::DeviceScan::ExclusiveSum(ctx->tmp.p, tb, in, out, (int)n, ctx->stream));
+  return SHCI_OK;
+}
+
+int bits_for(uint64_t v) {
+  int b = 1;
+  while (b < 64 && (v >> b) != 0) b++;
+  return b;
+}
+
+Ints make_ints(shci_ctx* ctx) {
+  Ints I;
+  I.h1 = P<double>(ctx->h1);
+  I.eri = P<double>(ctx->eri);
+  I.tD = P<double>(ctx->tD);
+  I.tX = P<double>(ctx->tX);
+  I.n = (int)ctx->n_orb;
+  I.core = ctx->core;
+  return I;
+}
+
+// Group the determinants: (alpha, beta) sort, alpha groups, beta order + groups.
+int build_groups(shci_ctx* ctx) {
+  const uint64_t n = ctx->n;
+  const int nb = (int)ctx->n_orb;
+  const hipStream_t st = ctx->stream;
+  const uint32_t g256 = div_up(n, 256);
+  CHECK_RC(ensure(ctx, ctx->ka, n * 8));
+  CHECK_RC(ensure(ctx, ctx->kb, n * 8));
+  CHECK_RC(ensure(ctx, ctx->va, n * 4));
+  CHECK_RC(ensure(ctx, ctx->vb, n * 4));
+  CHECK_RC(ensure(ctx, ctx->perm, n * 4));
+  CHECK_RC(ensure(ctx, ctx->iperm, n * 4));
+  CHECK_RC(ensure(ctx, ctx->up_s, n * 8));
+  CHECK_RC(ensure(ctx, ctx->dn_s, n * 8));
+  // LSD: stable sort by beta, then by alpha -> (alpha, beta) order.
+  k_iota<<<g256, 256, 0, st>>>(P<uint32_t>(ctx->va), n);
+  CHECK_RC(radix_sort_pairs(ctx, P<uint64_t>(ctx->dn_c), P<uint64_t>(ctx->kb), P<uint32_t>(ctx->va),
+                            P<uint32_t>(ctx->vb), n, nb));
+  k_gather_u64<<<g256, 256, 0, st>>>(P<uint64_t>(ctx->up_c), P<uint32_t>(ctx->vb), P<uint64_t>(ctx->ka), n);
+  CHECK_RC(radix_sort_pairs(ctx, P<uint64_t>(ctx->ka), P<uint64_t>(ctx->up_s), P<uint32_t>(ctx->vb),
+                            P<uint32_t>(ctx->perm), n, nb));
+  k_gather_u64<<<g256, 256, 0, st>>>(P<uint64_t>(ctx->dn_c), P<uint32_t>(ctx->perm), P<uint64_t>(ctx->dn_s), n);
+  k_inverse_perm<<<g256, 256, 0, st>>>(P<uint32_t>(ctx->perm), P<uint32_t>(ctx->iperm), n);
+  {
+    CHECK_RC(ensure(ctx, ctx->scal, 64 * 8));
+    unsigned int* bad = reinterpret_cast<unsigned int*>(ctx->scal.p);
+    unsigned int hb[2] = {0, 0};
+    HIPC(hipMemsetAsync(bad, 0, 8, st));
+    k_duplicates<<<g256, 256, 0, st>>>(P<uint64_t>(ctx->up_s), P<uint64_t>(ctx->dn_s), n, bad);
+    HIPC(hipMemcpyAsync(hb, bad, 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    if (hb[1]) return fail(ctx, SHCI_ERR_ARG, std::to_string(hb[1]) + " duplicate determinants");
+  }
+  // Alpha groups.
+  CHECK_RC(ensure(ctx, ctx->up_gid, n * 4));
+  CHECK_RC(ensure(ctx, ctx->up_gstart, (n + 1) * 4));
+  k_head_flags<<<g256, 256, 0, st>>>(P<uint64_t>(ctx->up_s), P<uint32_t>(ctx->va), n);
+  CHECK_RC(incl_scan(ctx, P<uint32_t>(ctx->va), P<uint32_t>(ctx->vb), n));
+  k_group_starts<<<g256, 256, 0, st>>>(P<uint32_t>(ctx->va), P<uint32_t>(ctx->vb), P<uint32_t>(ctx->up_gid),
+                                       P<uint32_t>(ctx->up_gstart), n);
+  HIPC(hipMemcpyAsync(&ctx->n_ugroups, P<uint32_t>(ctx->vb) + (n - 1), 4, hipMemcpyDeviceToHost, st));
+  // Beta order (stable sort of internal indices by beta) and beta groups.
+  CHECK_RC(ensure(ctx, ctx->dnord_idx, n * 4));
+  CHECK_RC(ensure(ctx, ctx->dnord_up, n * 8));
+  CHECK_RC(ensure(ctx, ctx->dn_gid, n * 4));
+  CHECK_RC(ensure(ctx, ctx->dn_gstart, (n + 1) * 4));
+  k_iota<<<g256, 256, 0, st>>>(P<uint32_t>(ctx->va), n);
+  CHECK_RC(radix_sort_pairs(ctx, P<uint64_t>(ctx->dn_s), P<uint64_t>(ctx->kb), P<uint32_t>(ctx->va),
+                            P<uint32_t>(ctx->dnord_idx), n, nb));
+  k_head_flags<<<g256, 256, 0, st>>>(P<uint64_t>(ctx->kb), P<uint32_t>(ctx->va), n);
+  CHECK_RC(incl_scan(ctx, P<uint32_t>(ctx->va), P<uint32_t>(ctx->vb), n));
+  // group id in beta order -> reuse ka as a u32 scratch for kgid
+  uint32_t* kgid = reinterpret_cast<uint32_t*>(P<uint64_t>(ctx->ka));
+  k_group_starts<<<g256, 256, 0, st>>>(P<uint32_t>(ctx->va), P<uint32_t>(ctx->vb), kgid, P<uint32_t>(ctx->dn_gstart), n);
+  HIPC(hipMemcpyAsync(&ctx->n_dgroups, P<uint32_t>(ctx->vb) + (n - 1), 4, hipMemcpyDeviceToHost, st));
+  k_dn_order<<<g256, 256, 0, st>>>(P<uint32_t>(ctx->dnord_idx), P<uint64_t>(ctx->up_s), kgid,
+                                   P<uint64_t>(ctx->dnord_up), P<uint32_t>(ctx->dn_gid), n);
+  HIPC(hipStreamSynchronize(st));
+  return SHCI_OK;
+}
+
+// Single-excitation partner lists of the unique alpha strings.
+int build_singles(shci_ctx* ctx) {
+  const uint32_t G = ctx->n_ugroups;
+  const uint32_t ne = ctx->n_up;
+  const hipStream_t st = ctx->stream;
+  CHECK_RC(ensure(ctx, ctx->sing_ptr, ((uint64_t)G + 1) * 8));
+  if (ne == 0 || G == 0) {
+    HIPC(hipMemsetAsync(ctx->sing_ptr.p, 0, ((uint64_t)G + 1) * 8, st));
+    ctx->n_singles = 0;
+    return SHCI_OK;
+  }
+  const uint64_t m = (uint64_t)G * ne;
+  DevBuf keys, keys_s, vals, vals_s, flag, incl, rid, rstart, cntb;
+  auto cleanup = [&]() {
+    release(ctx, keys);
+    release(ctx, keys_s);
+    release(ctx, vals);
+    release(ctx, vals_s);
+    release(ctx, flag);
+    release(ctx, incl);
+    release(ctx, rid);
+    release(ctx, rstart);
+    release(ctx, cntb);
+  };
+  int rc = SHCI_OK;
+  do {
+    if ((rc = ensure(ctx, keys, m * 8)) || (rc = ensure(ctx, keys_s, m * 8)) || (rc = ensure(ctx, vals, m * 4)) ||
+        (rc = ensure(ctx, vals_s, m * 4)) || (rc = ensure(ctx, flag, m * 4)) || (rc = ensure(ctx, incl, m * 4)) ||
+        (rc = ensure(ctx, rid, m * 4)) || (rc = ensure(ctx, rstart, (m + 1) * 4)) ||
+        (rc = ensure(ctx, cntb, ((uint64_t)G + 1) * 8)))
+      break;
+    k_minus_one<<<div_up(G, 256), 256, 0, st>>>(P<uint64_t>(ctx->up_s), P<uint32_t>(ctx->up_gstart), G, ne,
+                                                P<uint64_t>(keys), P<uint32_t>(vals));
+    if ((rc = radix_sort_pairs(ctx, P<uint64_t>(keys), P<uint64_t>(keys_s), P<uint32_t>(vals), P<uint32_t>(vals_s), m,
+                               (int)ctx->n_orb)))
+      break;
+    k_head_flags<<<div_up(m, 256), 256, 0, st>>>(P<uint64_t>(keys_s), P<uint32_t>(flag), m);
+    if ((rc = incl_scan(ctx, P<uint32_t>(flag), P<uint32_t>(incl), m))) break;
+    k_group_starts<<<div_up(m, 256), 256, 0, st>>>(P<uint32_t>(flag), P<uint32_t>(incl), P<uint32_t>(rid),
+                                                   P<uint32_t>(rstart), m);
+    if (hipMemsetAsync(cntb.p, 0, ((uint64_t)G + 1) * 8, st) != hipSuccess) {
+      rc = fail(ctx, SHCI_ERR_HIP, "memset");
+      break;
+    }
+    k_singles_count<<<div_up(m, 256), 256, 0, st>>>(P<uint32_t>(rid), P<uint32_t>(rstart), P<uint32_t>(vals_s), m,
+                                                    P<unsigned long long>(cntb));
+    if ((rc = excl_scan(ctx, P<unsigned long long>(cntb), P<unsigned long long>(ctx->sing_ptr), (uint64_t)G + 1))) break;
+    unsigned long long total = 0;
+    if (hipMemcpyAsync(&total, P<unsigned long long>(ctx->sing_ptr) + G, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(cntb.p, ctx->sing_ptr.p, ((uint64_t)G + 1) * 8, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      rc = fail(ctx, SHCI_ERR_HIP, "singles total");
+      break;
+    }
+    ctx->n_singles = total;
+    if ((rc = ensure(ctx, ctx->sing, std::max<uint64_t>(total, 1) * 4))) break;
+    if (total > 0) {
+      DevBuf pairs, pairs_s;
+      if ((rc = ensure(ctx, pairs, total * 8)) || (rc = ensure(ctx, pairs_s, total * 8))) {
+        release(ctx, pairs);
+        release(ctx, pairs_s);
+        break;
+      }
+      // cursor = start offsets (copy of the exclusive scan)
+      k_singles_fill<<<div_up(m, 256), 256, 0, st>>>(P<uint32_t>(rid), P<uint32_t>(rstart), P<uint32_t>(vals_s), m,
+                                                     P<unsigned long long>(cntb), P<uint64_t>(pairs));
+      rc = sort_keys_u64(ctx, P<uint64_t>(pairs), P<uint64_t>(pairs_s), total, 32 + bits_for(G));
+      if (rc == SHCI_OK) k_low32<<<div_up(total, 256), 256, 0, st>>>(P<uint64_t>(pairs_s), P<uint32_t>(ctx->sing), total);
+      if (rc == SHCI_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(ctx, SHCI_ERR_HIP, "singles sync");
+      release(ctx, pairs);
+      release(ctx, pairs_s);
+    }
+  } while (false);
+  if (rc == SHCI_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(ctx, SHCI_ERR_HIP, "singles sync");
+  cleanup();
+  return rc;
+}
+
+int choose_lanes(double mean) {
+  if (mean >= 96) return 64;
+  if (mean >= 48) return 32;
+  if (mean >= 20) return 16;
+  return 8;
+}
+
+RowParams row_params(shci_ctx* ctx) {
+  RowParams Pm;
+  Pm.n = ctx->n;
+  Pm.up = P<uint64_t>(ctx->up_s);
+  Pm.dn = P<uint64_t>(ctx->dn_s);
+  Pm.up_gid = P<uint32_t>(ctx->up_gid);
+  Pm.up_gstart = P<uint32_t>(ctx->up_gstart);
+  Pm.dn_gid = P<uint32_t>(ctx->dn_gid);
+  Pm.dn_gstart = P<uint32_t>(ctx->dn_gstart);
+  Pm.dnord_idx = P<uint32_t>(ctx->dnord_idx);
+  Pm.dnord_up = P<uint64_t>(ctx->dnord_up);
+  Pm.sing_ptr = P<uint64_t>(ctx->sing_ptr);
+  Pm.sing = P<uint32_t>(ctx->sing);
+  Pm.I = make_ints(ctx);
+  Pm.cnt = P<uint32_t>(ctx->cnt);
+  Pm.row_ptr = P<int64_t>(ctx->row_ptr);
+  Pm.col = P<uint32_t>(ctx->col);
+  Pm.v64 = P<double>(ctx->v64);
+  Pm.v32 = P<float>(ctx->v32);
+  Pm.len = P<uint32_t>(ctx->len);
+  Pm.keep_zeros = (ctx->build_flags & SHCI_KEEP_ZEROS) ? 1 : 0;
+  return Pm;
+}
+
+int do_build(shci_ctx* ctx, uint32_t flags) {
+  if (!ctx->have_ints) return fail(ctx, SHCI_ERR_STATE, "integrals not set");
+  if (!(flags & (SHCI_STORE_FP64 | SHCI_STORE_FP32))) flags |= SHCI_STORE_FP64;
+  ctx->build_flags = flags;
+  ctx->built = false;
+  const uint64_t n = ctx->n;
+  const hipStream_t st = ctx->stream;
+  HIPC(hipEventRecord(ctx->ev[0], st));
+  CHECK_RC(build_groups(ctx));
+  HIPC(hipEventRecord(ctx->ev[1], st));
+  CHECK_RC(build_singles(ctx));
+  HIPC(hipEventRecord(ctx->ev[2], st));
+  CHECK_RC(ensure(ctx, ctx->diag, n * 8));
+  k_diag<<<div_up(n, 256), 256, 0, st>>>(make_ints(ctx), P<uint64_t>(ctx->up_s), P<uint64_t>(ctx->dn_s),
+                                         P<double>(ctx->diag), n);
+  HIPC(hipEventRecord(ctx->ev[3], st));
+  // Pass 1: count candidates per row.
+  CHECK_RC(ensure(ctx, ctx->cnt, n * 4));
+  CHECK_RC(ensure(ctx, ctx->len, n * 4));
+  CHECK_RC(ensure(ctx, ctx->row_ptr, (n + 1) * 8));
+  {
+    RowParams Pm = row_params(ctx);
+    k_rows<false><<<div_up(n, 4), 256, 0, st>>>(Pm);
+    HIPC(hipGetLastError());
+  }
+  HIPC(hipEventRecord(ctx->ev[4], st));
+  // Scan -> row pointers; allocate the CSR.
+  {
+    DevBuf cnt64;
+    CHECK_RC(ensure(ctx, cnt64, (n + 1) * 8));
+    // widen counts to int64 with a trailing zero, then exclusive-scan
+    HIPC(hipMemsetAsync(cnt64.p, 0, (n + 1) * 8, st));
+    k_widen<<<div_up(n, 256), 256, 0, st>>>(P<uint32_t>(ctx->cnt), P<int64_t>(cnt64), n);
+    int rc = excl_scan(ctx, P<int64_t>(cnt64), P<int64_t>(ctx->row_ptr), n + 1);
+    release(ctx, cnt64);
+    CHECK_RC(rc);
+  }
+  int64_t total = 0;
+  HIPC(hipMemcpyAsync(&total, P<int64_t>(ctx->row_ptr) + n, 8, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  const uint64_t cap = std::max<int64_t>(total, 1);
+  release(ctx, ctx->col);
+  release(ctx, ctx->v64);
+  release(ctx, ctx->v32);
+  CHECK_RC(ensure(ctx, ctx->col, cap * 4));
+  if (flags & SHCI_STORE_FP64) CHECK_RC(ensure(ctx, ctx->v64, cap * 8));
+  if (flags & SHCI_STORE_FP32) CHECK_RC(ensure(ctx, ctx->v32, cap * 4));
+  HIPC(hipEventRecord(ctx->ev[5], st));
+  // Pass 2: fill.
+  {
+    RowParams Pm = row_params(ctx);
+    k_rows<true><<<div_up(n, 4), 256, 0, st>>>(Pm);
+    HIPC(hipGetLastError());
+  }
+  HIPC(hipEventRecord(ctx->ev[6], st));
+  // Compact if the |H| cut dropped anything.
+  {
+    DevBuf len64, new_rp;
+    CHECK_RC(ensure(ctx, len64, (n + 1) * 8));
+    HIPC(hipMemsetAsync(len64.p, 0, (n + 1) * 8, st));
+    k_widen<<<div_up(n, 256), 256, 0, st>>>(P<uint32_t>(ctx->len), P<int64_t>(len64), n);
+    int rc = ensure(ctx, new_rp, (n + 1) * 8);
+    if (rc == SHCI_OK) rc = excl_scan(ctx, P<int64_t>(len64), P<int64_t>(new_rp), n + 1);
+    int64_t kept = 0;
+    if (rc == SHCI_OK) {
+      if (hipMemcpyAsync(&kept, P<int64_t>(new_rp) + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        rc = fail(ctx, SHCI_ERR_HIP, "compact count");
+    }
+    if (rc == SHCI_OK && kept != total) {
+      DevBuf col2, v642, v322;
+      const uint64_t cap2 = std::max<int64_t>(kept, 1);
+      rc = ensure(ctx, col2, cap2 * 4);
+      if (rc == SHCI_OK && ctx->v64.p) rc = ensure(ctx, v642, cap2 * 8);
+      if (rc == SHCI_OK && ctx->v32.p) rc = ensure(ctx, v322, cap2 * 4);
+      if (rc == SHCI_OK) {
+        k_compact<<<div_up(n, 4), 256, 0, st>>>(n, P<int64_t>(ctx->row_ptr), P<int64_t>(new_rp), P<uint32_t>(ctx->col),
+                                                P<double>(ctx->v64), P<float>(ctx->v32), P<uint32_t>(col2),
+                                                P<double>(v642), P<float>(v322));
+        if (hipMemcpyAsync(ctx->row_ptr.p, new_rp.p, (n + 1) * 8, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+          rc = fail(ctx, SHCI_ERR_HIP, "compact");
+        std::swap(ctx->col, col2);
+        std::swap(ctx->v64, v642);
+        std::swap(ctx->v32, v322);
+      }
+      release(ctx, col2);
+      release(ctx, v642);
+      release(ctx, v322);
+      total = kept;
+    }
+    release(ctx, len64);
+    release(ctx, new_rp);
+    CHECK_RC(rc);
+  }
+  HIPC(hipEventRecord(ctx->ev[7], st));
+  HIPC(hipEventSynchronize(ctx->ev[7]));
+  ctx->nnz = (uint64_t)total;
+  ctx->lanes = choose_lanes(n ? (double)total / (double)n : 0.0);
+  for (int k = 0; k < 7; k++) HIPC(hipEventElapsedTime(&ctx->phase_ms[k], ctx->ev[k], ctx->ev[k + 1]));
+  HIPC(hipEventElapsedTime(&ctx->phase_ms[7], ctx->ev[0], ctx->ev[7]));
+  ctx->built = true;
+  return SHCI_OK;
+}
+
+// SpMV launch on internal-order device vectors.
+template <typename VT, typename XT, typename AT>
+int launch_spmv(shci_ctx* ctx, const VT* val, double shift, const XT* x, XT* y) {
+  const uint64_t n = ctx->n;
+  const int L = ctx->lanes;
+  const uint32_t grid = div_up(n, 256 / L);
+  const hipStream_t st = ctx->stream;
+  const int64_t* rp = P<int64_t>(ctx->row_ptr);
+  const uint32_t* col = P<uint32_t>(ctx->col);
+  const double* d = P<double>(ctx->diag);
+  switch (L) {
+    case 64: k_spmv<64, VT, XT, AT><<<grid, 256, 0, st>>>(n, rp, col, val, d, shift, x, y); break;
+    case 32: k_spmv<32, VT, XT, AT><<<grid, 256, 0, st>>>(n, rp, col, val, d, shift, x, y); break;
+    case 16: k_spmv<16, VT, XT, AT><<<grid, 256, 0, st>>>(n, rp, col, val, d, shift, x, y); break;
+    default: k_spmv<8, VT, XT, AT><<<grid, 256, 0, st>>>(n, rp, col, val, d, shift, x, y); break;
+  }
+  HIPC(hipGetLastError());
+  return SHCI_OK;
+}
+
+int launch_spmm4(shci_ctx* ctx, double shift, const float* x, float* y) {
+  const uint64_t n = ctx->n;
+  const int L = ctx->lanes;
+  const uint32_t grid = div_up(n, 256 / L);
+  const hipStream_t st = ctx->stream;
+  const int64_t* rp = P<int64_t>(ctx->row_ptr);
+  const uint32_t* col = P<uint32_t>(ctx->col);
+  const float* v = P<float>(ctx->v32);
+  const double* d = P<double>(ctx->diag);
+  switch (L) {
+    case 64: k_spmm_f32<64, 4><<<grid, 256, 0, st>>>(n, rp, col, v, d, shift, x, y); break;
+    case 32: k_spmm_f32<32, 4><<<grid, 256, 0, st>>>(n, rp, col, v, d, shift, x, y); break;
+    case 16: k_spmm_f32<16, 4><<<grid, 256, 0, st>>>(n, rp, col, v, d, shift, x, y); break;
+    default: k_spmm_f32<8, 4><<<grid, 256, 0, st>>>(n, rp, col, v, d, shift, x, y); break;
+  }
+  HIPC(hipGetLastError());
+  return SHCI_OK;
+}
+
+// FP64 results of up to kMaxDots dot products of a with b[0..m) (device vectors).
+template <typename T>
+int dots(shci_ctx* ctx, const T* a, const T* const* b, int m, double* host_out) {
+  CHECK_RC(ensure(ctx, ctx->part, (size_t)kMaxDots * kDotBlocks * 8));
+  CHECK_RC(ensure(ctx, ctx->scal, 64 * 8));
+  DotArgs B;
+  for (int j = 0; j < m; j++) B.b[j] = b[j];
+  const uint32_t nblk = std::min<uint32_t>(kDotBlocks, std::max<uint32_t>(1, div_up(ctx->n, 256)));
+  k_dots_partial<T><<<nblk, 256, 0, ctx->stream>>>(a, B, m, ctx->n, P<double>(ctx->part));
+  k_dots_final<<<1, 256, 0, ctx->stream>>>(P<double>(ctx->part), m, (int)nblk, P<double>(ctx->scal));
+  if (host_out) {
+    HIPC(hipMemcpyAsync(host_out, ctx->scal.p, m * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+  }
+  return SHCI_OK;
+}
+
+// Dense symmetric eigen-decomposition (cyclic Jacobi) of the lower triangle of
+// an n x n row-major matrix; ascending eigenvalues, eigenvectors as columns.
+void jacobi_eigh(int n, const std::vector<double>& a_in, std::vector<double>& w, std::vector<double>& v) {
+  std::vector<double> a = a_in;
+  v.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) v[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) off += a[(size_t)p * n + q] * a[(size_t)p * n + q];
+    if (off < 1e-300) break;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) {
+        const double apq = a[(size_t)p * n + q];
+        if (apq == 0.0) continue;
+        const double theta = (a[(size_t)q * n + q] - a[(size_t)p * n + p]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; k++) {
+          const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+          a[(size_t)k * n + p] = c * akp - s * akq;
+          a[(size_t)k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+          a[(size_t)p * n + k] = c * apk - s * aqk;
+          a[(size_t)q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          const double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
+          v[(size_t)k * n + p] = c * vkp - s * vkq;
+          v[(size_t)k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  std::vector<int> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::sort(ord.begin(), ord.end(), [&](int x, int y) { return a[(size_t)x * n + x] < a[(size_t)y * n + y]; });
+  w.resize(n);
+  std::vector<double> vs((size_t)n * n);
+  for (int j = 0; j < n; j++) {
+    w[j] = a[(size_t)ord[j] * n + ord[j]];
+    for (int i = 0; i < n; i++) vs[(size_t)i * n + j] = v[(size_t)i * n + ord[j]];
+  }
+  v.swap(vs);
+}
+
+// One reference Davidson call (davidson.cc:37-225) on internal-order device
+// vectors of type T.  For T = float the matrix is applied shifted by `shift`
+// (FP32 store, FP32 row accumulation, FP64 diagonal) and eigenvalues are
+// reported in the shifted frame.
+template <typename T>
+struct Davidson {
+  shci_ctx* ctx;
+  uint32_t ns;    // n_states
+  size_t K;       // ns * n_store
+  uint64_t n;
+  double shift;
+  std::vector<T*> v, hv, w, hw;
+  T* tmp;
+  std::vector<double> lowest, lowest_prev;
+  bool converged = false;
+  std::chrono::steady_clock::time_point t0;
+  double* traj_e;
+  double* traj_t;
+  int traj_cap;
+  int* n_traj;
+  double report_shift = 0.0;
+
+  int spmv(const T* x, T* y);
+
+  void record(double e) {
+    if (traj_e && *n_traj < traj_cap) {
+      traj_e[*n_traj] = e;
+      if (traj_t) traj_t[*n_traj] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      (*n_traj)++;
+    }
+  }
+
+  // init: ns device vectors (internal order, already normalised/orthogonal).
+  int run(double tol) {
+    const uint32_t g = div_up(n, 256);
+    const hipStream_t st = ctx->stream;
+    const size_t n_store = K / ns;
+    std::vector<double> hk(K * K, 0.0);
+    auto H = [&](size_t i, size_t j) -> double& { return hk[i * K + j]; };
+    lowest.assign(ns, 0.0);
+    size_t n_converged = 0;
+    converged = false;
+    for (uint32_t s = 0; s < ns; s++) {
+      CHECK_RC(spmv(v[s], hv[s]));
+      const T* b[1] = {hv[s]};
+      double e;
+      CHECK_RC(dots<T>(ctx, v[s], b, 1, &e));
+      lowest[s] = e;
+      H(s, s) = e;
+      HIPC(hipMemcpyAsync(w[s], v[s], n * sizeof(T), hipMemcpyDeviceToDevice, st));
+      HIPC(hipMemcpyAsync(hw[s], hv[s], n * sizeof(T), hipMemcpyDeviceToDevice, st));
+    }
+    record(lowest[0] + report_shift);
+    lowest_prev = lowest;
+    size_t i_state_precond = 0;
+    for (size_t it = ns; it < n_store * ns * 2; it++) {
+      const size_t ic = it % K;
+      if (it >= K) {
+        if (ic < ns - 1) continue;
+        if (ic == ns - 1) {
+          for (uint32_t s = 0; s < ns; s++) {
+            HIPC(hipMemcpyAsync(v[s], w[s], n * sizeof(T), hipMemcpyDeviceToDevice, st));
+            HIPC(hipMemcpyAsync(hv[s], hw[s], n * sizeof(T), hipMemcpyDeviceToDevice, st));
+          }
+          for (uint32_t s = 0; s < ns; s++) {
+            std::vector<const T*> b;
+            for (uint32_t k = s; k < ns; k++) b.push_back(hv[k]);
+            std::vector<double> d(b.size());
+            CHECK_RC(dots<T>(ctx, v[s], b.data(), (int)b.size(), d.data()));
+            lowest[s] = d[0];
+            H(s, s) = d[0];
+            for (uint32_t k = s + 1; k < ns; k++) H(s, k) = H(k, s) = d[k - s];
+          }
+          continue;
+        }
+      }
+      const double lam = lowest[i_state_precond];
+      k_precond<T><<<g, 256, 0, st>>>(v[ic], hw[i_state_precond], w[i_state_precond], P<double>(ctx->diag), lam,
+                                      shift, n);
+      // Modified Gram-Schmidt with device-resident scalars (davidson.cc:146-152).
+      for (size_t i = 0; i < ic; i++) {
+        const T* b[1] = {v[i]};
+        CHECK_RC(dots<T>(ctx, v[ic], b, 1, nullptr));
+        k_axpy_dev<T><<<g, 256, 0, st>>>(v[ic], v[i], P<double>(ctx->scal), n);
+      }
+      double nrm2;
+      {
+        const T* b[1] = {v[ic]};
+        CHECK_RC(dots<T>(ctx, v[ic], b, 1, &nrm2));
+      }
+      const double norm = std::sqrt(nrm2);
+      if (norm < 1e-12) {
+        converged = true;
+        break;
+      }
+      k_scale<T><<<g, 256, 0, st>>>(v[ic], 1.0 / norm, n);
+      CHECK_RC(spmv(v[ic], hv[ic]));
+      {
+        std::vector<const T*> b(v.begin(), v.begin() + ic + 1);
+        std::vector<double> d(ic + 1);
+        CHECK_RC(dots<T>(ctx, hv[ic], b.data(), (int)ic + 1, d.data()));
+        for (size_t i = 0; i <= ic; i++) H(ic, i) = d[i];
+      }
+      if (i_state_precond + 1 == ns) {
+        const int ks = (int)ic + 1;
+        std::vector<double> sub((size_t)ks * ks);
+        for (int i = 0; i < ks; i++)
+          for (int j = 0; j < ks; j++) sub[(size_t)i * ks + j] = i >= j ? H(i, j) : H(j, i);
+        std::vector<double> ev, evec;
+        jacobi_eigh(ks, sub, ev, evec);
+        for (uint32_t s = 0; s < ns; s++) {
+          lowest[s] = ev[s];
+          const double factor = evec[s] < 0 ? -1.0 : 1.0;
+          RitzArgs R;
+          for (int i = 0; i < ks; i++) {
+            R.v[i] = v[i];
+            R.hv[i] = hv[i];
+            R.c[i] = evec[(size_t)i * ks + s] * factor;
+          }
+          k_ritz<T, T><<<g, 256, 0, st>>>(R, ks, n, tmp, hw[s]);
+          HIPC(hipMemcpyAsync(w[s], tmp, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+        }
+        HIPC(hipStreamSynchronize(st));
+        record(lowest[0] + report_shift);
+        for (uint32_t s = (uint32_t)n_converged; s < ns; s++) {
+          if (std::fabs(lowest[s] - lowest_prev[s]) > tol) break;
+          n_converged++;
+        }
+        if (n_converged == ns) converged = true;
+        if (!converged) lowest_prev = lowest;
+        if (converged) break;
+      }
+      i_state_precond++;
+      if (i_state_precond >= ns) i_state_precond = n_converged;
+    }
+    return SHCI_OK;
+  }
+};
+
+template <>
+int Davidson<double>::spmv(const double* x, double* y) {
+  return launch_spmv<double, double, double>(ctx, P<double>(ctx->v64), 0.0, x, y);
+}
+template <>
+int Davidson<float>::spmv(const float* x, float* y) {
+  return launch_spmv<float, float, float>(ctx, P<float>(ctx->v32), shift, x, y);
+}
+
+// Normalise + orthogonalise the initial vectors (davidson.cc:73-88) into v[0..ns).
+template <typename T>
+int init_vectors(shci_ctx* ctx, Davidson<T>& D, const double* init_dev_internal) {
+  const uint64_t n = ctx->n;
+  const uint32_t g = div_up(n, 256);
+  for (uint32_t s = 0; s < D.ns; s++) {
+    const double* x = init_dev_internal + (size_t)s * n;
+    double nrm2;
+    const double* b[1] = {x};
+    CHECK_RC(dots<double>(ctx, x, b, 1, &nrm2));
+    k_copy_scale<double, T><<<g, 256, 0, ctx->stream>>>(x, D.v[s], 1.0 / std::sqrt(nrm2), n);
+    if (s > 0) {
+      for (uint32_t k = 0; k < s; k++) {
+        const T* bb[1] = {D.v[k]};
+        CHECK_RC(dots<T>(ctx, D.v[s], bb, 1, nullptr));
+        k_axpy_dev<T><<<g, 256, 0, ctx->stream>>>(D.v[s], D.v[k], P<double>(ctx->scal), n);
+      }
+      const T* bb[1] = {D.v[s]};
+      CHECK_RC(dots<T>(ctx, D.v[s], bb, 1, &nrm2));
+      k_scale<T><<<g, 256, 0, ctx->stream>>>(D.v[s], 1.0 / std::sqrt(nrm2), n);
+    }
+  }
+  return SHCI_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int shci_ctx_create(shci_ctx** out, int device) {
+  if (!out) return SHCI_ERR_ARG;
+  auto* ctx = new shci_ctx();
+  ctx->device = device;
+  *out = ctx;
+  HIPC(hipSetDevice(device));
+  HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  for (auto& e : ctx->ev) HIPC(hipEventCreate(&e));
+  return SHCI_OK;
+}
+
+int shci_ctx_destroy(shci_ctx* ctx) {
+  if (!ctx) return SHCI_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  DevBuf* bufs[] = {&ctx->h1, &ctx->eri, &ctx->tD, &ctx->tX, &ctx->up_c, &ctx->dn_c, &ctx->up_s, &ctx->dn_s,
+                    &ctx->perm, &ctx->iperm, &ctx->up_gid, &ctx->up_gstart, &ctx->dn_gid, &ctx->dn_gstart,
+                    &ctx->dnord_idx, &ctx->dnord_up, &ctx->sing_ptr, &ctx->sing, &ctx->diag, &ctx->cnt,
+                    &ctx->row_ptr, &ctx->col, &ctx->v64, &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb,
+                    &ctx->va, &ctx->vb, &ctx->scal, &ctx->part, &ctx->xv, &ctx->yv, &ctx->x32, &ctx->y32,
+                    &ctx->dav};
+  for (DevBuf* b : bufs) release(ctx, *b);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SHCI_OK;
+}
+
+const char* shci_last_error(const shci_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int shci_set_integrals(shci_ctx* ctx, uint32_t n_orb, const double* h1, const double* eri_packed, double core) {
+  if (!ctx || !h1 || !eri_packed) return SHCI_ERR_ARG;
+  if (n_orb == 0 || n_orb > 64) return fail(ctx, SHCI_ERR_UNSUPPORTED, "n_orb must be in [1, 64]");
+  HIPC(hipSetDevice(ctx->device));
+  const uint64_t n = n_orb;
+  const uint64_t np = n * (n + 1) / 2, ne = np * (np + 1) / 2;
+  std::vector<double> D(n * n * n), X(n * n * n);
+  auto c2h = [](uint64_t a, uint64_t b) { return a > b ? (a * (a + 1)) / 2 + b : (b * (b + 1)) / 2 + a; };
+  for (uint64_t p = 0; p < n; p++)
+    for (uint64_t q = 0; q < n; q++)
+      for (uint64_t k = 0; k < n; k++) {
+        D[(p * n + q) * n + k] = eri_packed[c2h(c2h(p, q), c2h(k, k))];
+        X[(p * n + q) * n + k] = eri_packed[c2h(c2h(p, k), c2h(k, q))];
+      }
+  // h1 symmetric lookup semantics: the value for key combine2(p,q) is h1[max][min].
+  std::vector<double> hs(n * n);
+  for (uint64_t p = 0; p < n; p++)
+    for (uint64_t q = 0; q < n; q++) hs[p * n + q] = p >= q ? h1[p * n + q] : h1[q * n + p];
+  CHECK_RC(ensure(ctx, ctx->h1, n * n * 8));
+  CHECK_RC(ensure(ctx, ctx->eri, ne * 8));
+  CHECK_RC(ensure(ctx, ctx->tD, n * n * n * 8));
+  CHECK_RC(ensure(ctx, ctx->tX, n * n * n * 8));
+  HIPC(hipMemcpy(ctx->h1.p, hs.data(), n * n * 8, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(ctx->eri.p, eri_packed, ne * 8, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(ctx->tD.p, D.data(), n * n * n * 8, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(ctx->tX.p, X.data(), n * n * n * 8, hipMemcpyHostToDevice));
+  ctx->n_orb = n_orb;
+  ctx->core = core;
+  ctx->have_ints = true;
+  return SHCI_OK;
+}
+
+int shci_build_hamiltonian(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n_dets,
+                           uint64_t n_dets_prev, uint32_t flags) {
+  if (!ctx) return SHCI_ERR_ARG;
+  if (!up || !dn || n_dets == 0) return fail(ctx, SHCI_ERR_ARG, "empty determinant list");
+  if (n_dets >= (1ull << 32)) return fail(ctx, SHCI_ERR_UNSUPPORTED, "n_dets >= 2^32");
+  HIPC(hipSetDevice(ctx->device));
+  if (ctx->built && n_dets == n_dets_prev && n_dets == ctx->n) return SHCI_OK;  // hamiltonian.h:91
+  std::vector<uint64_t> hu, hd;
+  const uint64_t* su = up;
+  const uint64_t* sd = dn;
+  if (flags & SHCI_DEVICE_PTRS) {
+    hu.resize(1);
+    hd.resize(1);
+    HIPC(hipMemcpy(hu.data(), up, 8, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(hd.data(), dn, 8, hipMemcpyDeviceToHost));
+    su = hu.data();
+    sd = hd.data();
+  }
+  const uint32_t nu = __builtin_popcountll(su[0]), nd = __builtin_popcountll(sd[0]);
+  if (!ctx->have_ints) return fail(ctx, SHCI_ERR_STATE, "integrals not set");
+  CHECK_RC(ensure(ctx, ctx->up_c, n_dets * 8));
+  CHECK_RC(ensure(ctx, ctx->dn_c, n_dets * 8));
+  const hipMemcpyKind kind = (flags & SHCI_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  HIPC(hipMemcpyAsync(ctx->up_c.p, up, n_dets * 8, kind, ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->dn_c.p, dn, n_dets * 8, kind, ctx->stream));
+  {
+    const uint64_t mask = ctx->n_orb >= 64 ? 0ull : ~((1ull << ctx->n_orb) - 1ull);
+    CHECK_RC(ensure(ctx, ctx->scal, 64 * 8));
+    unsigned int* bad = reinterpret_cast<unsigned int*>(ctx->scal.p);
+    unsigned int hb[2] = {0, 0};
+    HIPC(hipMemsetAsync(bad, 0, 8, ctx->stream));
+    k_validate<<<div_up(n_dets, 256), 256, 0, ctx->stream>>>(P<uint64_t>(ctx->up_c), P<uint64_t>(ctx->dn_c), n_dets,
+                                                             mask, (int)nu, (int)nd, bad);
+    HIPC(hipMemcpyAsync(hb, bad, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    if (hb[0])
+      return fail(ctx, SHCI_ERR_ARG,
+                  std::to_string(hb[0]) + " determinants outside n_orb or with electron counts differing from det 0");
+  }
+  ctx->n = n_dets;
+  ctx->n_up = nu;
+  ctx->n_dn = nd;
+  return do_build(ctx, flags & ~SHCI_DEVICE_PTRS);
+}
+
+int shci_rebuild_hamiltonian(shci_ctx* ctx, uint32_t flags) {
+  if (!ctx) return SHCI_ERR_ARG;
+  if (ctx->n == 0) return fail(ctx, SHCI_ERR_STATE, "no determinants resident");
+  HIPC(hipSetDevice(ctx->device));
+  return do_build(ctx, flags & ~SHCI_DEVICE_PTRS);
+}
+
+int shci_update_existing_elems(shci_ctx* ctx) {
+  if (!ctx) return SHCI_ERR_ARG;
+  if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
+  HIPC(hipSetDevice(ctx->device));
+  k_update_values<<<div_up(ctx->n, 4), 256, 0, ctx->stream>>>(
+      ctx->n, make_ints(ctx), P<uint64_t>(ctx->up_s), P<uint64_t>(ctx->dn_s), P<int64_t>(ctx->row_ptr),
+      P<uint32_t>(ctx->col), P<double>(ctx->v64), P<float>(ctx->v32), P<double>(ctx->diag));
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return SHCI_OK;
+}
+
+int shci_get_counts(shci_ctx* ctx, uint64_t* n_dets, uint64_t* n_stored_upper, uint64_t* n_elems) {
+  if (!ctx) return SHCI_ERR_ARG;
+  if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
+  const uint64_t up = ctx->nnz / 2 + ctx->n;
+  if (n_dets) *n_dets = ctx->n;
+  if (n_stored_upper) *n_stored_upper = up;
+  if (n_elems) *n_elems = 2 * up - ctx->n;
+  return SHCI_OK;
+}
+
+int shci_get_diag(shci_ctx* ctx, double* diag) {
+  if (!ctx || !diag) return SHCI_ERR_ARG;
+  if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
+  HIPC(hipSetDevice(ctx->device));
+  CHECK_RC(ensure(ctx, ctx->xv, ctx->n * 8));
+  k_to_caller<double, double><<<div_up(ctx->n, 256), 256, 0, ctx->stream>>>(P<double>(ctx->diag), P<uint32_t>(ctx->perm),
+                                                                            P<double>(ctx->xv), ctx->n);
+  HIPC(hipMemcpyAsync(diag, ctx->xv.p, ctx->n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return SHCI_OK;
+}
+
+int shci_spmv(shci_ctx* ctx, const double* x, double* y, uint32_t flags) {
+  if (!ctx || !x || !y) return SHCI_ERR_ARG;
+  if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
+  HIPC(hipSetDevice(ctx->device));
+  const uint64_t n = ctx->n;
+  const uint32_t g = div_up(n, 256);
+  const hipStream_t st = ctx->stream;
+  const bool fp32 = flags & SHCI_FP32;
+  if (fp32 && !ctx->v32.p) return fail(ctx, SHCI_ERR_STATE, "FP32 store not built");
+  if (!fp32 && !ctx->v64.p) return fail(ctx, SHCI_ERR_STATE, "FP64 store not built");
+  CHECK_RC(ensure(ctx, ctx->xv, n * 8));
+  CHECK_RC(ensure(ctx, ctx->yv, n * 8));
+  const double* xd = x;
+  if (!(flags & SHCI_DEVICE_PTRS)) {
+    CHECK_RC(ensure(ctx, ctx->x32, n * 8));
+    HIPC(hipMemcpyAsync(ctx->x32.p, x, n * 8, hipMemcpyHostToDevice, st));
+    xd = P<double>(ctx->x32);
+  }
+  if (fp32) {
+    CHECK_RC(ensure(ctx, ctx->y32, n * 8));
+    float* xf = reinterpret_cast<float*>(P<double>(ctx->xv));
+    float* yf = reinterpret_cast<float*>(P<double>(ctx->yv));
+    k_to_internal<double, float><<<g, 256, 0, st>>>(xd, P<uint32_t>(ctx->perm), xf, n);
+    CHECK_RC(launch_spmv<float, float, float>(ctx, P<float>(ctx->v32), 0.0, xf, yf));
+    double* yo = (flags & SHCI_DEVICE_PTRS) ? y : P<double>(ctx->y32);
+    k_to_caller<float, double><<<g, 256, 0, st>>>(yf, P<uint32_t>(ctx->perm), yo, n);
+  } else {
+    k_to_internal<double, double><<<g, 256, 0, st>>>(xd, P<uint32_t>(ctx->perm), P<double>(ctx->xv), n);
+    CHECK_RC(launch_spmv<double, double, double>(ctx, P<double>(ctx->v64), 0.0, P<double>(ctx->xv),
+                                                 P<double>(ctx->yv)));
+    CHECK_RC(ensure(ctx, ctx->y32, n * 8));
+    double* yo = (flags & SHCI_DEVICE_PTRS) ? y : P<double>(ctx->y32);
+    k_to_caller<double, double><<<g, 256, 0, st>>>(P<double>(ctx->yv), P<uint32_t>(ctx->perm), yo, n);
+  }
+  if (!(flags & SHCI_DEVICE_PTRS)) HIPC(hipMemcpyAsync(y, ctx->y32.p, n * 8, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  return SHCI_OK;
+}
+
+int shci_spmm(shci_ctx* ctx, const double* X, double* Y, uint32_t k, uint32_t flags) {
+  if (!ctx || !X || !Y || k == 0 || k > 8) return SHCI_ERR_ARG;
+  if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
+  HIPC(hipSetDevice(ctx->device));
+  const uint64_t n = ctx->n;
+  const hipStream_t st = ctx->stream;
+  if (!(flags & SHCI_FP32) || !ctx->v32.p) {
+    // FP64 block product as k SpMVs.
+    for (uint32_t v = 0; v < k; v++) CHECK_RC(shci_spmv(ctx, X + (size_t)v * n, Y + (size_t)v * n, flags));
+    return SHCI_OK;
+  }
+  const uint32_t Kp = 4;
+  for (uint32_t v0 = 0; v0 < k; v0 += Kp) {
+    const uint32_t kk = std::min(Kp, k - v0);
+    DevBuf Xd, Yd, xi, yi;
+    auto cleanup = [&]() {
+      release(ctx, Xd);
+      release(ctx, Yd);
+      release(ctx, xi);
+      release(ctx, yi);
+    };
+    int rc = SHCI_OK;
+    const double* Xs = X + (size_t)v0 * n;
+    double* Ys = Y + (size_t)v0 * n;
+    if (!(flags & SHCI_DEVICE_PTRS)) {
+      if ((rc = ensure(ctx, Xd, (size_t)kk * n * 8)) || (rc = ensure(ctx, Yd, (size_t)kk * n * 8))) {
+        cleanup();
+        return rc;
+      }
+      if (hipMemcpyAsync(Xd.p, Xs, (size_t)kk * n * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
+        cleanup();
+        return fail(ctx, SHCI_ERR_HIP, "spmm copy");
+      }
+      Xs = P<double>(Xd);
+    }
+    if ((rc = ensure(ctx, xi, (size_t)Kp * n * 4)) || (rc = ensure(ctx, yi, (size_t)Kp * n * 4))) {
+      cleanup();
+      return rc;
+    }
+    k_interleave<<<div_up(n, 256), 256, 0, st>>>(Xs, P<uint32_t>(ctx->perm), P<float>(xi), kk, Kp, n);
+    rc = launch_spmm4(ctx, 0.0, P<float>(xi), P<float>(yi));
+    double* Yo = (flags & SHCI_DEVICE_PTRS) ? Ys : P<double>(Yd);
+    if (rc == SHCI_OK) k_deinterleave<<<div_up(n, 256), 256, 0, st>>>(P<float>(yi), P<uint32_t>(ctx->perm), Yo, kk, Kp, n);
+    if (rc == SHCI_OK && !(flags & SHCI_DEVICE_PTRS) &&
+        hipMemcpyAsync(Ys, Yd.p, (size_t)kk * n * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
+      rc = fail(ctx, SHCI_ERR_HIP, "spmm copy back");
+    if (rc == SHCI_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(ctx, SHCI_ERR_HIP, "spmm sync");
+    cleanup();
+    CHECK_RC(rc);
+  }
+  return SHCI_OK;
+}
+
+int shci_davidson(shci_ctx* ctx, uint32_t n_states, const double* init, double tol, uint32_t mode,
+                  uint32_t max_calls, double* evals, double* evecs, int* converged, double* traj_e, double* traj_t,
+                  int traj_cap, int* n_traj) {
+  if (!ctx || !init || !evals || n_states == 0 || n_states > 4) return ctx ? fail(ctx, SHCI_ERR_ARG, "bad davidson args") : SHCI_ERR_ARG;
+  if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
+  const bool mixed = mode == SHCI_DAVIDSON_MIXED;
+  if (mixed && !ctx->v32.p) return fail(ctx, SHCI_ERR_STATE, "mixed Davidson needs the FP32 store");
+  if (!mixed && !ctx->v64.p) return fail(ctx, SHCI_ERR_STATE, "FP64 Davidson needs the FP64 store");
+  HIPC(hipSetDevice(ctx->device));
+  const uint64_t n = ctx->n;
+  const uint32_t g = div_up(n, 256);
+  const hipStream_t st = ctx->stream;
+  int nt_local = 0;
+  if (!n_traj) n_traj = &nt_local;
+  *n_traj = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  if (max_calls == 0) max_calls = 1;
+  if (n == 1) {  // davidson.cc:49-55
+    double d;
+    HIPC(hipMemcpy(&d, ctx->diag.p, 8, hipMemcpyDeviceToHost));
+    evals[0] = d;
+    if (evecs) evecs[0] = 1.0;
+    if (converged) *converged = 1;
+    return SHCI_OK;
+  }
+  const uint32_t ns = (uint32_t)std::min<uint64_t>(n, n_states);
+  const size_t n_store = std::min<uint64_t>(n, 5);
+  const size_t K = ns * n_store;
+  const size_t es = mixed ? 4 : 8;
+  // Layout: v[K], hv[K], w[ns], hw[ns], tmp, init64[ns], work64[2].
+  const size_t n_vec = 2 * K + 2 * ns + 1;
+  CHECK_RC(ensure(ctx, ctx->dav, n_vec * n * es + (ns + 2) * n * 8));
+  char* base = static_cast<char*>(ctx->dav.p);
+  double* init_i = reinterpret_cast<double*>(base + n_vec * n * es);
+  double* w64 = init_i + ns * n;
+  double* y64 = w64 + n;
+  // init -> internal order (FP64)
+  {
+    DevBuf hbuf;
+    CHECK_RC(ensure(ctx, hbuf, (size_t)ns * n * 8));
+    int rc = SHCI_OK;
+    if (hipMemcpyAsync(hbuf.p, init, (size_t)ns * n * 8, hipMemcpyHostToDevice, st) != hipSuccess)
+      rc = fail(ctx, SHCI_ERR_HIP, "init copy");
+    for (uint32_t s = 0; s < ns && rc == SHCI_OK; s++)
+      k_to_internal<double, double><<<g, 256, 0, st>>>(P<double>(hbuf) + (size_t)s * n, P<uint32_t>(ctx->perm),
+                                                       init_i + (size_t)s * n, n);
+    if (rc == SHCI_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(ctx, SHCI_ERR_HIP, "init sync");
+    release(ctx, hbuf);
+    CHECK_RC(rc);
+  }
+  std::vector<double> result(ns, 0.0);
+  int conv = 0;
+  auto run_mode = [&](auto tag) -> int {
+    using T = decltype(tag);
+    Davidson<T> D;
+    D.ctx = ctx;
+    D.ns = ns;
+    D.K = K;
+    D.n = n;
+    D.t0 = t0;
+    D.traj_e = traj_e;
+    D.traj_t = traj_t;
+    D.traj_cap = traj_cap;
+    D.n_traj = n_traj;
+    T* vb = reinterpret_cast<T*>(base);
+    for (size_t i = 0; i < K; i++) D.v.push_back(vb + i * n);
+    for (size_t i = 0; i < K; i++) D.hv.push_back(vb + (K + i) * n);
+    for (size_t i = 0; i < ns; i++) D.w.push_back(vb + (2 * K + i) * n);
+    for (size_t i = 0; i < ns; i++) D.hw.push_back(vb + (2 * K + ns + i) * n);
+    D.tmp = vb + (2 * K + 2 * ns) * n;
+    D.shift = 0.0;
+    if (mixed) {
+      // Shift = FP64 Rayleigh quotient of the initial ground-state vector
+      // (FP32 values, FP64 accumulation and diagonal): keeps FP32 values of
+      // H - shift small so FP32 row sums lose no absolute accuracy.
+      CHECK_RC(launch_spmv<float, double, double>(ctx, P<float>(ctx->v32), 0.0, init_i, y64));
+      const double* b1[1] = {y64};
+      const double* b2[1] = {init_i};
+      double num, den;
+      CHECK_RC(dots<double>(ctx, init_i, b1, 1, &num));
+      CHECK_RC(dots<double>(ctx, init_i, b2, 1, &den));
+      D.shift = num / den;
+      D.report_shift = D.shift;
+    }
+    CHECK_RC(init_vectors<T>(ctx, D, init_i));
+    for (uint32_t call = 0; call < max_calls; call++) {
+      CHECK_RC(D.run(tol));
+      if (D.converged) break;
+      if (call + 1 < max_calls) {
+        // Warm restart from the Ritz vectors (the reference caller's loop, solver.h:430-530).
+        for (uint32_t s = 0; s < ns; s++)
+          k_copy_scale<T, double><<<g, 256, 0, st>>>(D.w[s], init_i + (size_t)s * n, 1.0, n);
+        CHECK_RC(init_vectors<T>(ctx, D, init_i));
+      }
+    }
+    conv = D.converged ? 1 : 0;
+    for (uint32_t s = 0; s < ns; s++) result[s] = D.lowest[s] + D.shift;
+    // Eigenvectors (caller order) and, in mixed mode, the FP64 re-evaluated energy.
+    for (uint32_t s = 0; s < ns; s++) {
+      k_copy_scale<T, double><<<g, 256, 0, st>>>(D.w[s], w64, 1.0, n);
+      if (mixed) {
+        CHECK_RC(launch_spmv<float, double, double>(ctx, P<float>(ctx->v32), 0.0, w64, y64));
+        const double* b1[1] = {y64};
+        const double* b2[1] = {w64};
+        double num, den;
+        CHECK_RC(dots<double>(ctx, w64, b1, 1, &num));
+        CHECK_RC(dots<double>(ctx, w64, b2, 1, &den));
+        result[s] = num / den;
+        if (s == 0) D.record(result[0]);
+      }
+      if (evecs) {
+        k_to_caller<double, double><<<g, 256, 0, st>>>(w64, P<uint32_t>(ctx->perm), y64, n);
+        HIPC(hipMemcpyAsync(evecs + (size_t)s * n, y64, n * 8, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+      }
+    }
+    return SHCI_OK;
+  };
+  if (mixed) {
+    CHECK_RC(run_mode(float{}));
+  } else {
+    CHECK_RC(run_mode(double{}));
+  }
+  for (uint32_t s = 0; s < ns; s++) evals[s] = result[s];
+  if (evecs)
+    for (uint32_t s = ns; s < n_states; s++) std::memcpy(evecs + (size_t)s * n, init + (size_t)s * n, n * 8);
+  if (converged) *converged = conv;
+  return SHCI_OK;
+}
+
+int shci_rdm(shci_ctx* ctx, uint32_t n_states, const double* coefs, double* rdm1, double* rdm2) {
+  if (!ctx || !coefs || n_states == 0) return SHCI_ERR_ARG;
+  if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
+  HIPC(hipSetDevice(ctx->device));
+  const uint64_t n = ctx->n;
+  const uint64_t no = ctx->n_orb;
+  const uint64_t n2 = no * no * (no * no + 1) / 2;
+  const hipStream_t st = ctx->stream;
+  DevBuf cb, ci, r1, r2;
+  auto cleanup = [&]() {
+    release(ctx, cb);
+    release(ctx, ci);
+    release(ctx, r1);
+    release(ctx, r2);
+  };
+  int rc = SHCI_OK;
+  if ((rc = ensure(ctx, cb, (size_t)n_states * n * 8)) || (rc = ensure(ctx, ci, (size_t)n_states * n * 8)) ||
+      (rc = ensure(ctx, r1, no * no * 8)) || (rc = ensure(ctx, r2, n2 * 8))) {
+    cleanup();
+    return rc;
+  }
+  if (hipMemcpyAsync(cb.p, coefs, (size_t)n_states * n * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemsetAsync(r1.p, 0, no * no * 8, st) != hipSuccess || hipMemsetAsync(r2.p, 0, n2 * 8, st) != hipSuccess) {
+    cleanup();
+    return fail(ctx, SHCI_ERR_HIP, "rdm setup");
+  }
+  for (uint32_t s = 0; s < n_states; s++)
+    k_to_internal<double, double><<<div_up(n, 256), 256, 0, st>>>(P<double>(cb) + (size_t)s * n, P<uint32_t>(ctx->perm),
+                                                                  P<double>(ci) + (size_t)s * n, n);
+  RdmParams R;
+  R.n = n;
+  R.n_orb = (int)no;
+  R.n_states = n_states;
+  R.up = P<uint64_t>(ctx->up_s);
+  R.dn = P<uint64_t>(ctx->dn_s);
+  R.rp = P<int64_t>(ctx->row_ptr);
+  R.col = P<uint32_t>(ctx->col);
+  R.c = P<double>(ci);
+  R.rdm1 = P<double>(r1);
+  R.rdm2 = P<double>(r2);
+  k_rdm<<<div_up(n, 4), 256, 0, st>>>(R);
+  if (hipGetLastError() != hipSuccess) rc = fail(ctx, SHCI_ERR_HIP, "rdm launch");
+  if (rc == SHCI_OK && rdm1 && hipMemcpyAsync(rdm1, r1.p, no * no * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
+    rc = fail(ctx, SHCI_ERR_HIP, "rdm1 copy");
+  if (rc == SHCI_OK && rdm2 && hipMemcpyAsync(rdm2, r2.p, n2 * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
+    rc = fail(ctx, SHCI_ERR_HIP, "rdm2 copy");
+  if (rc == SHCI_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(ctx, SHCI_ERR_HIP, "rdm sync");
+  cleanup();
+  return rc;
+}
+
+int shci_export_upper(shci_ctx* ctx, int64_t* row_ptr, int64_t* cols, double* vals) {
+  if (!ctx || !row_ptr) return SHCI_ERR_ARG;
+  if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
+  HIPC(hipSetDevice(ctx->device));
+  const uint64_t n = ctx->n;
+  const uint64_t m = ctx->nnz / 2 + n;
+  const hipStream_t st = ctx->stream;
+  DevBuf cnt, orp, oc, ov;
+  auto cleanup = [&]() {
+    release(ctx, cnt);
+    release(ctx, orp);
+    release(ctx, oc);
+    release(ctx, ov);
+  };
+  int rc = SHCI_OK;
+  if ((rc = ensure(ctx, cnt, (n + 1) * 8)) || (rc = ensure(ctx, orp, (n + 1) * 8))) {
+    cleanup();
+    return rc;
+  }
+  if (hipMemsetAsync(cnt.p, 0, (n + 1) * 8, st) != hipSuccess) {
+    cleanup();
+    return fail(ctx, SHCI_ERR_HIP, "export memset");
+  }
+  k_export_count<<<div_up(n, 4), 256, 0, st>>>(n, P<int64_t>(ctx->row_ptr), P<uint32_t>(ctx->col),
+                                                P<uint32_t>(ctx->perm), P<unsigned long long>(cnt));
+  rc = excl_scan(ctx, P<unsigned long long>(cnt), P<unsigned long long>(orp), n + 1);
+  unsigned long long total = 0;
+  if (rc == SHCI_OK && (hipMemcpyAsync(&total, P<unsigned long long>(orp) + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                        hipStreamSynchronize(st) != hipSuccess))
+    rc = fail(ctx, SHCI_ERR_HIP, "export count");
+  if (rc == SHCI_OK && total != m) rc = fail(ctx, SHCI_ERR_STATE, "stored pattern is not symmetric");
+  if (rc == SHCI_OK) rc = ensure(ctx, oc, m * 8);
+  if (rc == SHCI_OK) rc = ensure(ctx, ov, m * 8);
+  if (rc == SHCI_OK) {
+    k_export_fill<<<div_up(n, 4), 256, 0, st>>>(n, P<int64_t>(ctx->row_ptr), P<uint32_t>(ctx->col), P<double>(ctx->v64),
+                                               P<double>(ctx->diag), P<uint32_t>(ctx->perm),
+                                               P<unsigned long long>(orp), P<int64_t>(oc), P<double>(ov));
+    if (hipMemcpyAsync(row_ptr, orp.p, (n + 1) * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        (cols && hipMemcpyAsync(cols, oc.p, m * 8, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+        (vals && hipMemcpyAsync(vals, ov.p, m * 8, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+        hipStreamSynchronize(st) != hipSuccess)
+      rc = fail(ctx, SHCI_ERR_HIP, "export copy");
+  }
+  cleanup();
+  return rc;
+}
+
+int shci_get_build_timings(shci_ctx* ctx, float* phases) {
+  if (!ctx || !phases) return SHCI_ERR_ARG;
+  std::memcpy(phases, ctx->phase_ms, sizeof(ctx->phase_ms));
+  return SHCI_OK;
+}
+
+int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, double* bytes) {
+  if (!ctx || reps <= 0) return SHCI_ERR_ARG;
+  if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
+  HIPC(hipSetDevice(ctx->device));
+  const uint64_t n = ctx->n;
+  const bool fp32 = flags & SHCI_FP32;
+  if (fp32 && !ctx->v32.p) return fail(ctx, SHCI_ERR_STATE, "FP32 store not built");
+  if (!fp32 && !ctx->v64.p) return fail(ctx, SHCI_ERR_STATE, "FP64 store not built");
+  CHECK_RC(ensure(ctx, ctx->xv, n * 8));
+  CHECK_RC(ensure(ctx, ctx->yv, n * 8));
+  const hipStream_t st = ctx->stream;
+  // x = 1/sqrt(n) everywhere (values do not change the traffic).
+  if (fp32) {
+    std::vector<float> h(n, (float)(1.0 / std::sqrt((double)n)));
+    HIPC(hipMemcpyAsync(ctx->xv.p, h.data(), n * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipStreamSynchronize(st));
+  } else {
+    std::vector<double> h(n, 1.0 / std::sqrt((double)n));
+    HIPC(hipMemcpyAsync(ctx->xv.p, h.data(), n * 8, hipMemcpyHostToDevice, st));
+    HIPC(hipStreamSynchronize(st));
+  }
+  auto once = [&]() -> int {
+    if (fp32)
+      return launch_spmv<float, float, float>(ctx, P<float>(ctx->v32), 0.0, reinterpret_cast<float*>(ctx->xv.p),
+                                              reinterpret_cast<float*>(ctx->yv.p));
+    return launch_spmv<double, double, double>(ctx, P<double>(ctx->v64), 0.0, P<double>(ctx->xv), P<double>(ctx->yv));
+  };
+  CHECK_RC(once());  // warm-up
+  HIPC(hipEventRecord(ctx->ev[0], st));
+  for (int r = 0; r < reps; r++) CHECK_RC(once());
+  HIPC(hipEventRecord(ctx->ev[1], st));
+  HIPC(hipEventSynchronize(ctx->ev[1]));
+  float ms = 0.f;
+  HIPC(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  if (ms_avg) *ms_avg = ms / reps;
+  if (bytes) {
+    const double F = (double)ctx->nnz;  // off-diagonal full-symmetric entries
+    const double vs = fp32 ? 4.0 : 8.0;
+    // values + columns, row pointers, FP64 diagonal, x once, y once.
+    *bytes = F * (vs + 4.0) + (double)(n + 1) * 8.0 + (double)n * 8.0 + 2.0 * (double)n * vs;
+  }
+  return SHCI_OK;
+}
+
+int shci_memory(shci_ctx* ctx, uint64_t* bytes) {
+  if (!ctx || !bytes) return SHCI_ERR_ARG;
+  *bytes = ctx->mem_in_use;
+  return SHCI_OK;
+}
+
+}  // extern "C"

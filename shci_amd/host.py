@@ -1,0 +1,97 @@
+"""ctypes bindings of libshci_host.so: integrals (FCIDUMP semantics of
+src/chem/integrals.cc) and the CPU heat-bath selection
+(src/chem/chem_system.cc:74-402, src/solver/solver.h:435-495), which stays on
+the CPU and feeds determinant lists to the GPU path."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native
+
+_u64p = C.POINTER(C.c_uint64)
+_dp = C.POINTER(C.c_double)
+
+
+def _lib():
+    return _native.host_lib()
+
+
+class HostSystem:
+    """Integrals + the variational determinant list + selection state."""
+
+    def __init__(self, handle, n_up, n_dn, n_states=1):
+        lib = _lib()
+        if not handle:
+            raise RuntimeError("shci_host: " + lib.shci_host_last_error().decode())
+        self._h = handle
+        self.n_up, self.n_dn, self.n_states = n_up, n_dn, n_states
+        n = C.c_uint()
+        core = C.c_double()
+        up = C.c_uint64()
+        dn = C.c_uint64()
+        sz = C.c_size_t()
+        ehf = C.c_double()
+        lib.shci_host_system_info(self._h, C.byref(n), C.byref(core), C.byref(up), C.byref(dn), C.byref(sz),
+                                  C.byref(ehf))
+        self.n_orbs, self.core, self.hf_up, self.hf_dn = n.value, core.value, up.value, dn.value
+        self.eri_size, self.energy_hf = sz.value, ehf.value
+        lib.shci_host_reset_dets(self._h)
+
+    @classmethod
+    def from_arrays(cls, n, n_up, n_dn, h1, eri, core=0.0, n_states=1, hf=None):
+        h1 = np.ascontiguousarray(h1, dtype=np.float64)
+        eri = np.ascontiguousarray(eri, dtype=np.float64)
+        hf_up, hf_dn = hf if hf is not None else ((1 << n_up) - 1, (1 << n_dn) - 1)
+        h = _lib().shci_host_system_from_arrays(n, n_up, n_dn, h1.ctypes.data_as(_dp), eri.ctypes.data_as(_dp),
+                                                core, hf_up, hf_dn)
+        return cls(h, n_up, n_dn, n_states)
+
+    @classmethod
+    def from_fcidump(cls, path, n_up, n_dn, occs_up=(), occs_dn=(), reorder=-1, n_states=1):
+        ou = (C.c_uint * max(1, len(occs_up)))(*occs_up)
+        od = (C.c_uint * max(1, len(occs_dn)))(*occs_dn)
+        h = _lib().shci_host_system_from_fcidump(str(path).encode(), n_up, n_dn, ou, len(occs_up), od,
+                                                 len(occs_dn), reorder)
+        return cls(h, n_up, n_dn, n_states)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib().shci_host_system_free(self._h)
+            self._h = None
+
+    def integrals(self):
+        n = self.n_orbs
+        h1 = np.zeros((n, n))
+        eri = np.zeros(self.eri_size)
+        order = (C.c_uint * n)()
+        _lib().shci_host_system_integrals(self._h, h1.ctypes.data_as(_dp), eri.ctypes.data_as(_dp), order)
+        return h1, eri, np.array(order[:], dtype=np.int64)
+
+    @property
+    def n_dets(self) -> int:
+        return int(_lib().shci_host_n_dets(self._h))
+
+    def dets(self):
+        n = self.n_dets
+        up = np.zeros(n, dtype=np.uint64)
+        dn = np.zeros(n, dtype=np.uint64)
+        _lib().shci_host_get_dets(self._h, up.ctypes.data_as(_u64p), dn.ctypes.data_as(_u64p))
+        return up, dn
+
+    def set_dets(self, up, dn):
+        up = np.ascontiguousarray(up, dtype=np.uint64)
+        dn = np.ascontiguousarray(dn, dtype=np.uint64)
+        _lib().shci_host_set_dets(self._h, up.ctypes.data_as(_u64p), dn.ctypes.data_as(_u64p), len(up))
+
+    def reset(self):
+        _lib().shci_host_reset_dets(self._h)
+
+    def select(self, coefs, eps_var) -> int:
+        """One selection sweep; appends new dets, returns how many were added."""
+        coefs = np.atleast_2d(np.ascontiguousarray(coefs, dtype=np.float64))
+        r = _lib().shci_host_select(self._h, coefs.ctypes.data_as(_dp), coefs.shape[0], eps_var)
+        if r < 0:
+            raise RuntimeError("shci_host: " + _lib().shci_host_last_error().decode())
+        return int(r)
