@@ -81,6 +81,11 @@ int shci_rebuild_hamiltonian(shci_ctx* ctx, uint32_t flags);
  * integrals on the fixed connection pattern. */
 int shci_update_existing_elems(shci_ctx* ctx);
 
+/* Hamiltonian<S>::clear / SparseMatrix::clear (hamiltonian.h:128-155,
+ * sparse_matrix.cc:153-160): release the determinants, the matrix and all
+ * work buffers (the integrals stay). */
+int shci_clear(shci_ctx* ctx);
+
 /* SparseMatrix::count_n_elems (sparse_matrix.cc:73-86) and row count:
  * n_stored_upper counts the diagonal + pairs i<j (the reference's storage);
  * n_elems = 2 * n_stored_upper - n_dets (its printed "nonzero elems"). */

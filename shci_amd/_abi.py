@@ -28,6 +28,7 @@ EXPORTS = {
     "shci_build_hamiltonian": (C.c_int, [_vp, _vp, _vp, C.c_uint64, C.c_uint64, C.c_uint32]),
     "shci_rebuild_hamiltonian": (C.c_int, [_vp, C.c_uint32]),
     "shci_update_existing_elems": (C.c_int, [_vp]),
+    "shci_clear": (C.c_int, [_vp]),
     "shci_get_counts": (C.c_int, [_vp, _u64p, _u64p, _u64p]),
     "shci_get_diag": (C.c_int, [_vp, _dp]),
     "shci_spmv": (C.c_int, [_vp, _vp, _vp, C.c_uint32]),

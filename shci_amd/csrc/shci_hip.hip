@@ -76,14 +76,20 @@ struct shci_ctx {
   uint32_t n_up = 0, n_dn = 0;
   uint32_t n_ugroups = 0, n_dgroups = 0;
   DevBuf up_gid, up_gstart, dn_gid, dn_gstart, dnord_idx, dnord_up;
-  uint64_t n_singles = 0;
-  DevBuf sing_ptr, sing;
+  // "Minus one electron" tables of the unique alpha / beta strings and the
+  // mixed-double segments ((alpha-1, beta-1) cores) of the determinants.
+  uint32_t r_alpha = 0, r_beta = 0;  // distinct minus-one strings
+  DevBuf m1a_id, m1b_id;             // [group * n_e + k] -> minus-one string id
+  uint64_t n_dm1 = 0, n_dm1_runs = 0;
+  DevBuf dm1_vals, dm1_rstart, dm1_seg;  // sorted (det * K + slot); run starts; slot -> run
+  DevBuf flag;                           // device status words (fill overflow)
 
   // Matrix.
   bool built = false;
   uint32_t build_flags = 0;
-  DevBuf diag, cnt, row_ptr, col, v64, v32, len;
-  uint64_t nnz = 0;  // stored off-diagonal entries (full symmetric)
+  DevBuf diag, cnt, row_ptr, row_end, col, v64, v32, len;
+  uint64_t nnz = 0;        // stored off-diagonal entries (full symmetric)
+  uint64_t n_dropped = 0;  // candidates removed by the |H| >= 1e-12 cut
   int lanes = 16;
 
   // Scratch.
@@ -333,6 +339,15 @@ __global__ void k_group_starts(const uint32_t* __restrict__ flag, const uint32_t
   if (i == n - 1) gstart[g + 1] = (uint32_t)n;
 }
 
+// Run starts only (no per-element id): rstart[run] = first position, rstart[R] = n.
+__global__ void k_run_starts(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ incl,
+                             uint32_t* __restrict__ rstart, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flag[i]) rstart[incl[i] - 1] = (uint32_t)i;
+  if (i == n - 1) rstart[incl[i]] = (uint32_t)n;
+}
+
 // dn-ordered arrays: dnord_up[k] = up_s[dnord_idx[k]], dn_gid[dnord_idx[k]] = group of k.
 __global__ void k_dn_order(const uint32_t* __restrict__ dnord_idx, const uint64_t* __restrict__ up_s,
                            const uint32_t* __restrict__ kgid, uint64_t* __restrict__ dnord_up,
@@ -344,54 +359,56 @@ __global__ void k_dn_order(const uint32_t* __restrict__ dnord_idx, const uint64_
   dn_gid[i] = kgid[k];
 }
 
-// Minus-one keys of unique alpha strings: key = alpha with one electron removed.
-__global__ void k_minus_one(const uint64_t* __restrict__ up_s, const uint32_t* __restrict__ gstart,
-                            uint32_t G, uint32_t n_e, uint64_t* __restrict__ keys,
+// Unique strings of a grouped, sorted key array: out[g] = keys[gstart[g]].
+__global__ void k_unique_strings(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ gstart, uint32_t G,
+                                 uint64_t* __restrict__ out) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < G) out[g] = keys[gstart[g]];
+}
+
+// Minus-one keys of unique spin strings: key = string with its k-th electron
+// removed, val = g * n_e + k (the analogue of the reference's abm1 map,
+// src/solver/hamiltonian.h:256-305).
+__global__ void k_minus_one(const uint64_t* __restrict__ uniq, uint32_t G, uint32_t n_e, uint64_t* __restrict__ keys,
                             uint32_t* __restrict__ vals) {
   const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= G) return;
-  uint64_t s = up_s[gstart[g]];
+  const uint64_t s = uniq[g];
   uint64_t m = s;
   for (uint32_t e = 0; e < n_e; e++) {
     const uint64_t bit = m & (~m + 1);
     m &= m - 1;
     keys[g * n_e + e] = s & ~bit;
-    vals[g * n_e + e] = (uint32_t)g;
+    vals[g * n_e + e] = (uint32_t)(g * n_e + e);
   }
 }
 
-// For sorted minus-one keys: every pair inside a run of equal keys is a pair of
-// single-excitation partners.  Count partners per alpha group.
-__global__ void k_singles_count(const uint32_t* __restrict__ run_start_of, const uint32_t* __restrict__ run_gstart,
-                                const uint32_t* __restrict__ vals, uint64_t m,
-                                unsigned long long* __restrict__ cnt) {
-  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m) return;
-  const uint32_t r = run_start_of[e];
-  const uint32_t len = run_gstart[r + 1] - run_gstart[r];
-  if (len > 1) atomicAdd(&cnt[vals[e]], (unsigned long long)(len - 1));
+// id[vals[p]] = run id of sorted position p (incl = inclusive scan of head flags).
+__global__ void k_scatter_run_id(const uint32_t* __restrict__ vals, const uint32_t* __restrict__ incl, uint64_t m,
+                                 uint32_t* __restrict__ id) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < m) id[vals[p]] = incl[p] - 1;
 }
 
-__global__ void k_singles_fill(const uint32_t* __restrict__ run_of, const uint32_t* __restrict__ run_gstart,
-                               const uint32_t* __restrict__ vals, uint64_t m,
-                               unsigned long long* __restrict__ cursor, uint64_t* __restrict__ pairs) {
-  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m) return;
-  const uint32_t r = run_of[e];
-  const uint32_t b = run_gstart[r], t = run_gstart[r + 1];
-  if (t - b < 2) return;
-  const uint32_t g = vals[e];
-  const unsigned long long pos = atomicAdd(&cursor[g], (unsigned long long)(t - b - 1));
-  uint64_t k = pos;
-  for (uint32_t f = b; f < t; f++) {
-    if (f == e) continue;
-    pairs[k++] = ((uint64_t)g << 32) | vals[f];
-  }
-}
-
-__global__ void k_low32(const uint64_t* __restrict__ pairs, uint32_t* __restrict__ out, uint64_t m) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m) out[i] = (uint32_t)pairs[i];
+// Mixed-double keys: for det i (alpha group g, beta group h) and electrons
+// k1 < n_up, k2 < n_dn, key = (alpha-minus-one id << bb) | beta-minus-one id,
+// val = i * K + k1 * n_dn + k2.  Two dets share a key iff they differ by one
+// alpha and one beta excitation (or are equal in one spin): every
+// alpha-single x beta-single connection (hamiltonian.h:482-532) shares exactly
+// one key.
+__global__ void k_dm1_keys(const uint32_t* __restrict__ up_gid, const uint32_t* __restrict__ dn_gid,
+                           const uint32_t* __restrict__ m1a, const uint32_t* __restrict__ m1b, uint64_t n,
+                           uint32_t nu, uint32_t nd, int bb, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t K = nu * nd;
+  if (t >= n * K) return;
+  const uint64_t i = t / K;
+  const uint32_t slot = (uint32_t)(t - i * K);
+  const uint32_t k1 = slot / nd, k2 = slot - k1 * nd;
+  const uint64_t a = m1a[(uint64_t)up_gid[i] * nu + k1];
+  const uint64_t b = m1b[(uint64_t)dn_gid[i] * nd + k2];
+  keys[t] = (a << bb) | b;
+  vals[t] = (uint32_t)t;
 }
 
 __global__ void k_widen(const uint32_t* __restrict__ a, int64_t* __restrict__ b, uint64_t n) {
@@ -436,30 +453,39 @@ struct RowParams {
   const uint32_t* __restrict__ dn_gstart;
   const uint32_t* __restrict__ dnord_idx;
   const uint64_t* __restrict__ dnord_up;
-  const uint64_t* __restrict__ sing_ptr;
-  const uint32_t* __restrict__ sing;
+  const uint32_t* __restrict__ dm1_vals;    // sorted det * K + slot
+  const uint32_t* __restrict__ dm1_rstart;  // segment starts
+  const uint32_t* __restrict__ dm1_seg;     // det * K + slot -> segment
+  uint32_t K;                               // n_up * n_dn
   Ints I;
   uint32_t* __restrict__ cnt;
   const int64_t* __restrict__ row_ptr;
+  int64_t* __restrict__ row_end;
   uint32_t* __restrict__ col;
   double* __restrict__ v64;
   float* __restrict__ v32;
   uint32_t* __restrict__ len;
+  unsigned int* __restrict__ overflow;
   int keep_zeros;
 };
 
 template <bool FILL>
 struct Emitter {
   uint64_t base;  // FILL: next output slot; COUNT: running count
+  uint64_t end;   // FILL: end of this row's slots (writes beyond are dropped and flagged)
   __device__ __forceinline__ void hit_only(bool hit) { base += __popcll(__ballot(hit)); }
   __device__ __forceinline__ void emit(const RowParams& Pm, bool hit, uint32_t j, double v) {
     const bool keep = hit && (Pm.keep_zeros || fabs(v) >= kEps);
     const uint64_t m = __ballot(keep);
     if (keep) {
       const uint64_t pos = base + mbcnt(m);
-      Pm.col[pos] = j;
-      if (Pm.v64) Pm.v64[pos] = v;
-      if (Pm.v32) Pm.v32[pos] = (float)v;
+      if (pos < end) {
+        Pm.col[pos] = j;
+        if (Pm.v64) Pm.v64[pos] = v;
+        if (Pm.v32) Pm.v32[pos] = (float)v;
+      } else {
+        atomicOr(Pm.overflow, 1u);
+      }
     }
     base += __popcll(m);
   }
@@ -473,6 +499,7 @@ __global__ __launch_bounds__(256) void k_rows(RowParams Pm) {
   const uint64_t up_i = Pm.up[r], dn_i = Pm.dn[r];
   Emitter<FILL> em;
   em.base = FILL ? (uint64_t)Pm.row_ptr[r] : 0;
+  em.end = FILL ? (uint64_t)Pm.row_ptr[r + 1] : 0;
 
   // (A) same beta string: single/double alpha excitations (hamiltonian.h:430-442).
   {
@@ -516,18 +543,21 @@ __global__ __launch_bounds__(256) void k_rows(RowParams Pm) {
       }
     }
   }
-  // (C) alpha single x beta single (hamiltonian.h:482-532): scan the alpha
-  // groups of every single-excitation partner of this alpha string, flattened
-  // across partners so all 64 lanes stay busy on short groups.
+  // (C) alpha single x beta single (hamiltonian.h:482-532): the members of
+  // this det's K = n_up * n_dn mixed-double segments (dets sharing its
+  // (alpha - 1 electron, beta - 1 electron) core), flattened across segments
+  // so all 64 lanes stay busy.  A member differing in both spins is a mixed
+  // double; members equal in one spin are singles found by (A)/(B).
   {
-    const uint64_t ps = Pm.sing_ptr[ga], pe = Pm.sing_ptr[ga + 1];
-    for (uint64_t pb = ps; pb < pe; pb += kWave) {
-      const uint64_t p = pb + lane;
+    const uint32_t K = Pm.K;
+    const uint64_t sb = r * (uint64_t)K;
+    for (uint32_t s0 = 0; s0 < K; s0 += kWave) {
+      const uint32_t sl = s0 + lane;
       uint32_t st = 0, ln = 0;
-      if (p < pe) {
-        const uint32_t g2 = Pm.sing[p];
-        st = Pm.up_gstart[g2];
-        ln = Pm.up_gstart[g2 + 1] - st;
+      if (sl < K) {
+        const uint32_t seg = Pm.dm1_seg[sb + sl];
+        st = Pm.dm1_rstart[seg];
+        ln = Pm.dm1_rstart[seg + 1] - st;
       }
       uint32_t incl = ln;
 #pragma unroll
@@ -549,15 +579,20 @@ __global__ __launch_bounds__(256) void k_rows(RowParams Pm) {
         }
         const uint32_t stl = __shfl(st, lo, kWave);
         const uint32_t exl = __shfl(excl, lo, kWave);
-        const uint32_t k = valid ? stl + (t - exl) : 0;
-        const uint64_t d = valid ? Pm.dn[k] : dn_i;
-        const bool hit = valid && __popcll(d ^ dn_i) == 2;
+        uint32_t j = 0;
+        uint64_t uj = up_i, dj = dn_i;
+        if (valid) {
+          j = Pm.dm1_vals[stl + (t - exl)] / K;
+          uj = Pm.up[j];
+          dj = Pm.dn[j];
+        }
+        const bool hit = valid && uj != up_i && dj != dn_i;
         if (!FILL) {
           em.hit_only(hit);
         } else {
           double v = 0.0;
-          if (hit) v = double_mixed(Pm.I, up_i, Pm.up[k], dn_i, d);
-          em.emit(Pm, hit, k, v);
+          if (hit) v = double_mixed(Pm.I, up_i, uj, dn_i, dj);
+          em.emit(Pm, hit, j, v);
         }
       }
     }
@@ -565,38 +600,37 @@ __global__ __launch_bounds__(256) void k_rows(RowParams Pm) {
   if (lane == 0) {
     if (FILL) {
       Pm.len[r] = (uint32_t)(em.base - (uint64_t)Pm.row_ptr[r]);
+      Pm.row_end[r] = (int64_t)em.base;
     } else {
       Pm.cnt[r] = (uint32_t)em.base;
     }
   }
 }
 
-// Move rows into an exact-size CSR when the |H| cut dropped entries.
-__global__ __launch_bounds__(256) void k_compact(uint64_t n, const int64_t* __restrict__ old_rp,
-                                                 const int64_t* __restrict__ new_rp, const uint32_t* __restrict__ col_in,
-                                                 const double* __restrict__ v64_in, const float* __restrict__ v32_in,
-                                                 uint32_t* __restrict__ col_out, double* __restrict__ v64_out,
-                                                 float* __restrict__ v32_out) {
-  const uint64_t r = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (r >= n) return;
-  const int64_t b = old_rp[r], nb = new_rp[r], m = new_rp[r + 1] - nb;
-  for (int64_t k = lane_id(); k < m; k += kWave) {
-    col_out[nb + k] = col_in[b + k];
-    if (v64_out) v64_out[nb + k] = v64_in[b + k];
-    if (v32_out) v32_out[nb + k] = v32_in[b + k];
+// Sum of per-row lengths (the stored off-diagonal count).
+__global__ void k_sum_u32(const uint32_t* __restrict__ a, uint64_t n, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[256];
+  unsigned long long v = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) v += a[i];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) atomicAdd(out, red[0]);
 }
 
 // Recompute values on a fixed pattern (update_existing_elems).
 __global__ __launch_bounds__(256) void k_update_values(uint64_t n, Ints I, const uint64_t* __restrict__ up,
                                                        const uint64_t* __restrict__ dn, const int64_t* __restrict__ rp,
-                                                       const uint32_t* __restrict__ col, double* __restrict__ v64,
+                                                       const int64_t* __restrict__ re, const uint32_t* __restrict__ col, double* __restrict__ v64,
                                                        float* __restrict__ v32, double* __restrict__ diag) {
   const uint64_t r = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (r >= n) return;
   const uint64_t ui = up[r], di = dn[r];
   if (lane_id() == 0) diag[r] = diag_elem(I, ui, di);
-  for (int64_t k = rp[r] + lane_id(); k < rp[r + 1]; k += kWave) {
+  for (int64_t k = rp[r] + lane_id(); k < re[r]; k += kWave) {
     const uint32_t j = col[k];
     const double v = elem_any(I, ui, di, up[j], dn[j]);
     if (v64) v64[k] = v;
@@ -611,14 +645,14 @@ __global__ __launch_bounds__(256) void k_update_values(uint64_t n, Ints I, const
 // ---------------------------------------------------------------------------
 template <int L, typename VT, typename XT, typename AT>
 __global__ __launch_bounds__(256) void k_spmv(uint64_t n, const int64_t* __restrict__ rp,
-                                              const uint32_t* __restrict__ col, const VT* __restrict__ val,
-                                              const double* __restrict__ diag, double shift,
-                                              const XT* __restrict__ x, XT* __restrict__ y) {
+                                              const int64_t* __restrict__ re, const uint32_t* __restrict__ col,
+                                              const VT* __restrict__ val, const double* __restrict__ diag,
+                                              double shift, const XT* __restrict__ x, XT* __restrict__ y) {
   const uint32_t sub = threadIdx.x & (L - 1);
   const uint64_t row = (uint64_t)xcd_remap(blockIdx.x, gridDim.x) * (256 / L) + threadIdx.x / L;
   AT acc0 = 0, acc1 = 0;
   if (row < n) {
-    const int64_t b = rp[row], e = rp[row + 1];
+    const int64_t b = rp[row], e = re[row];
     int64_t k = b + sub;
     for (; k + L < e; k += 2 * L) {
       const uint32_t c0 = __builtin_nontemporal_load(&col[k]);
@@ -646,16 +680,16 @@ __global__ __launch_bounds__(256) void k_spmv(uint64_t n, const int64_t* __restr
 // Block product for K vectors stored interleaved (x[i*K + v]).
 template <int L, int K>
 __global__ __launch_bounds__(256) void k_spmm_f32(uint64_t n, const int64_t* __restrict__ rp,
-                                                  const uint32_t* __restrict__ col, const float* __restrict__ val,
-                                                  const double* __restrict__ diag, double shift,
-                                                  const float* __restrict__ x, float* __restrict__ y) {
+                                                  const int64_t* __restrict__ re, const uint32_t* __restrict__ col,
+                                                  const float* __restrict__ val, const double* __restrict__ diag,
+                                                  double shift, const float* __restrict__ x, float* __restrict__ y) {
   const uint32_t sub = threadIdx.x & (L - 1);
   const uint64_t row = (uint64_t)xcd_remap(blockIdx.x, gridDim.x) * (256 / L) + threadIdx.x / L;
   float acc[K];
 #pragma unroll
   for (int v = 0; v < K; v++) acc[v] = 0.f;
   if (row < n) {
-    const int64_t b = rp[row], e = rp[row + 1];
+    const int64_t b = rp[row], e = re[row];
     for (int64_t k = b + sub; k < e; k += L) {
       const uint32_t c = __builtin_nontemporal_load(&col[k]);
       const float a = __builtin_nontemporal_load(&val[k]);
@@ -815,6 +849,7 @@ struct RdmParams {
   const uint64_t* __restrict__ up;
   const uint64_t* __restrict__ dn;
   const int64_t* __restrict__ rp;
+  const int64_t* __restrict__ re;
   const uint32_t* __restrict__ col;
   const double* __restrict__ c;  // n_states x n, internal order
   double* __restrict__ rdm1;
@@ -998,7 +1033,7 @@ __global__ __launch_bounds__(256) void k_rdm(RdmParams R) {
   if (r >= R.n) return;
   const uint64_t ut = R.up[r], dt = R.dn[r];
   if (lane_id() == 0) rdm_pair(R, ut, dt, r, ut, dt, r);
-  for (int64_t k = R.rp[r] + lane_id(); k < R.rp[r + 1]; k += kWave) {
+  for (int64_t k = R.rp[r] + lane_id(); k < R.re[r]; k += kWave) {
     const uint32_t j = R.col[k];
     if (j <= r) continue;
     rdm_pair(R, ut, dt, r, R.up[j], R.dn[j], j);
@@ -1007,20 +1042,20 @@ __global__ __launch_bounds__(256) void k_rdm(RdmParams R) {
 
 // Export helpers: count, per caller row, the stored upper entries (diag + j>i).
 __global__ __launch_bounds__(256) void k_export_count(uint64_t n, const int64_t* __restrict__ rp,
-                                                      const uint32_t* __restrict__ col,
+                                                      const int64_t* __restrict__ re, const uint32_t* __restrict__ col,
                                                       const uint32_t* __restrict__ perm,
                                                       unsigned long long* __restrict__ cnt_caller) {
   const uint64_t r = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (r >= n) return;
   const uint32_t ci = perm[r];
   uint32_t c = 0;
-  for (int64_t k = rp[r] + lane_id(); k < rp[r + 1]; k += kWave) c += perm[col[k]] > ci ? 1u : 0u;
+  for (int64_t k = rp[r] + lane_id(); k < re[r]; k += kWave) c += perm[col[k]] > ci ? 1u : 0u;
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, kWave);
   if (lane_id() == 0) cnt_caller[ci] = c + 1;
 }
 
 __global__ __launch_bounds__(256) void k_export_fill(uint64_t n, const int64_t* __restrict__ rp,
-                                                     const uint32_t* __restrict__ col, const double* __restrict__ v64,
+                                                     const int64_t* __restrict__ re, const uint32_t* __restrict__ col, const double* __restrict__ v64,
                                                      const double* __restrict__ diag, const uint32_t* __restrict__ perm,
                                                      const unsigned long long* __restrict__ out_rp,
                                                      int64_t* __restrict__ out_col, double* __restrict__ out_val) {
@@ -1033,9 +1068,9 @@ __global__ __launch_bounds__(256) void k_export_fill(uint64_t n, const int64_t* 
     out_val[base] = diag[r];
   }
   base += 1;
-  for (int64_t k0 = rp[r]; k0 < rp[r + 1]; k0 += kWave) {
+  for (int64_t k0 = rp[r]; k0 < re[r]; k0 += kWave) {
     const int64_t k = k0 + lane_id();
-    const bool valid = k < rp[r + 1];
+    const bool valid = k < re[r];
     const uint32_t cj = valid ? perm[col[k]] : 0;
     const bool keep = valid && cj > ci;
     const uint64_t m = __ballot(keep);
@@ -1164,19 +1199,18 @@ int build_groups(shci_ctx* ctx) {
   return SHCI_OK;
 }
 
-// Single-excitation partner lists of the unique alpha strings.
-int build_singles(shci_ctx* ctx) {
-  const uint32_t G = ctx->n_ugroups;
-  const uint32_t ne = ctx->n_up;
+// Minus-one table of G unique strings (n_e electrons each): id[g*n_e+k] =
+// index of the distinct string "g without its k-th electron"; returns the
+// number of distinct minus-one strings in *R.
+int build_m1(shci_ctx* ctx, const uint64_t* uniq, uint32_t G, uint32_t ne, DevBuf& id, uint32_t* R) {
   const hipStream_t st = ctx->stream;
-  CHECK_RC(ensure(ctx, ctx->sing_ptr, ((uint64_t)G + 1) * 8));
-  if (ne == 0 || G == 0) {
-    HIPC(hipMemsetAsync(ctx->sing_ptr.p, 0, ((uint64_t)G + 1) * 8, st));
-    ctx->n_singles = 0;
+  const uint64_t m = (uint64_t)G * ne;
+  CHECK_RC(ensure(ctx, id, std::max<uint64_t>(m, 1) * 4));
+  if (m == 0) {
+    *R = 0;
     return SHCI_OK;
   }
-  const uint64_t m = (uint64_t)G * ne;
-  DevBuf keys, keys_s, vals, vals_s, flag, incl, rid, rstart, cntb;
+  DevBuf keys, keys_s, vals, vals_s, flag, incl;
   auto cleanup = [&]() {
     release(ctx, keys);
     release(ctx, keys_s);
@@ -1184,60 +1218,104 @@ int build_singles(shci_ctx* ctx) {
     release(ctx, vals_s);
     release(ctx, flag);
     release(ctx, incl);
-    release(ctx, rid);
-    release(ctx, rstart);
-    release(ctx, cntb);
   };
   int rc = SHCI_OK;
-  do {
-    if ((rc = ensure(ctx, keys, m * 8)) || (rc = ensure(ctx, keys_s, m * 8)) || (rc = ensure(ctx, vals, m * 4)) ||
-        (rc = ensure(ctx, vals_s, m * 4)) || (rc = ensure(ctx, flag, m * 4)) || (rc = ensure(ctx, incl, m * 4)) ||
-        (rc = ensure(ctx, rid, m * 4)) || (rc = ensure(ctx, rstart, (m + 1) * 4)) ||
-        (rc = ensure(ctx, cntb, ((uint64_t)G + 1) * 8)))
-      break;
-    k_minus_one<<<div_up(G, 256), 256, 0, st>>>(P<uint64_t>(ctx->up_s), P<uint32_t>(ctx->up_gstart), G, ne,
-                                                P<uint64_t>(keys), P<uint32_t>(vals));
-    if ((rc = radix_sort_pairs(ctx, P<uint64_t>(keys), P<uint64_t>(keys_s), P<uint32_t>(vals), P<uint32_t>(vals_s), m,
-                               (int)ctx->n_orb)))
-      break;
+  if ((rc = ensure(ctx, keys, m * 8)) || (rc = ensure(ctx, keys_s, m * 8)) || (rc = ensure(ctx, vals, m * 4)) ||
+      (rc = ensure(ctx, vals_s, m * 4)) || (rc = ensure(ctx, flag, m * 4)) || (rc = ensure(ctx, incl, m * 4))) {
+    cleanup();
+    return rc;
+  }
+  k_minus_one<<<div_up(G, 256), 256, 0, st>>>(uniq, G, ne, P<uint64_t>(keys), P<uint32_t>(vals));
+  rc = radix_sort_pairs(ctx, P<uint64_t>(keys), P<uint64_t>(keys_s), P<uint32_t>(vals), P<uint32_t>(vals_s), m,
+                        (int)ctx->n_orb);
+  if (rc == SHCI_OK) {
     k_head_flags<<<div_up(m, 256), 256, 0, st>>>(P<uint64_t>(keys_s), P<uint32_t>(flag), m);
-    if ((rc = incl_scan(ctx, P<uint32_t>(flag), P<uint32_t>(incl), m))) break;
-    k_group_starts<<<div_up(m, 256), 256, 0, st>>>(P<uint32_t>(flag), P<uint32_t>(incl), P<uint32_t>(rid),
-                                                   P<uint32_t>(rstart), m);
-    if (hipMemsetAsync(cntb.p, 0, ((uint64_t)G + 1) * 8, st) != hipSuccess) {
-      rc = fail(ctx, SHCI_ERR_HIP, "memset");
-      break;
+    rc = incl_scan(ctx, P<uint32_t>(flag), P<uint32_t>(incl), m);
+  }
+  if (rc == SHCI_OK) {
+    k_scatter_run_id<<<div_up(m, 256), 256, 0, st>>>(P<uint32_t>(vals_s), P<uint32_t>(incl), m, P<uint32_t>(id));
+    if (hipMemcpyAsync(R, P<uint32_t>(incl) + (m - 1), 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      rc = fail(ctx, SHCI_ERR_HIP, "minus-one table");
+  }
+  cleanup();
+  return rc;
+}
+
+// Minus-one tables of the unique alpha and beta strings, then the sorted
+// mixed-double segments.
+int build_segments(shci_ctx* ctx) {
+  const hipStream_t st = ctx->stream;
+  const uint64_t n = ctx->n;
+  const uint32_t G = ctx->n_ugroups, H = ctx->n_dgroups, nu = ctx->n_up, nd = ctx->n_dn;
+  {
+    DevBuf uniq;
+    CHECK_RC(ensure(ctx, uniq, (uint64_t)std::max(G, H) * 8));
+    k_unique_strings<<<div_up(G, 256), 256, 0, st>>>(P<uint64_t>(ctx->up_s), P<uint32_t>(ctx->up_gstart), G,
+                                                     P<uint64_t>(uniq));
+    int rc = build_m1(ctx, P<uint64_t>(uniq), G, nu, ctx->m1a_id, &ctx->r_alpha);
+    if (rc == SHCI_OK) {
+      // beta strings in beta order: dn_s[dnord_idx[dn_gstart[h]]] -- dnord_up holds alpha, so
+      // gather the sorted beta keys through the beta-order index.
+      k_gather_u64<<<div_up(n, 256), 256, 0, st>>>(P<uint64_t>(ctx->dn_s), P<uint32_t>(ctx->dnord_idx),
+                                                   P<uint64_t>(ctx->ka), n);
+      k_unique_strings<<<div_up(H, 256), 256, 0, st>>>(P<uint64_t>(ctx->ka), P<uint32_t>(ctx->dn_gstart), H,
+                                                       P<uint64_t>(uniq));
+      rc = build_m1(ctx, P<uint64_t>(uniq), H, nd, ctx->m1b_id, &ctx->r_beta);
     }
-    k_singles_count<<<div_up(m, 256), 256, 0, st>>>(P<uint32_t>(rid), P<uint32_t>(rstart), P<uint32_t>(vals_s), m,
-                                                    P<unsigned long long>(cntb));
-    if ((rc = excl_scan(ctx, P<unsigned long long>(cntb), P<unsigned long long>(ctx->sing_ptr), (uint64_t)G + 1))) break;
-    unsigned long long total = 0;
-    if (hipMemcpyAsync(&total, P<unsigned long long>(ctx->sing_ptr) + G, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(cntb.p, ctx->sing_ptr.p, ((uint64_t)G + 1) * 8, hipMemcpyDeviceToDevice, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
-      rc = fail(ctx, SHCI_ERR_HIP, "singles total");
-      break;
-    }
-    ctx->n_singles = total;
-    if ((rc = ensure(ctx, ctx->sing, std::max<uint64_t>(total, 1) * 4))) break;
-    if (total > 0) {
-      DevBuf pairs, pairs_s;
-      if ((rc = ensure(ctx, pairs, total * 8)) || (rc = ensure(ctx, pairs_s, total * 8))) {
-        release(ctx, pairs);
-        release(ctx, pairs_s);
-        break;
-      }
-      // cursor = start offsets (copy of the exclusive scan)
-      k_singles_fill<<<div_up(m, 256), 256, 0, st>>>(P<uint32_t>(rid), P<uint32_t>(rstart), P<uint32_t>(vals_s), m,
-                                                     P<unsigned long long>(cntb), P<uint64_t>(pairs));
-      rc = sort_keys_u64(ctx, P<uint64_t>(pairs), P<uint64_t>(pairs_s), total, 32 + bits_for(G));
-      if (rc == SHCI_OK) k_low32<<<div_up(total, 256), 256, 0, st>>>(P<uint64_t>(pairs_s), P<uint32_t>(ctx->sing), total);
-      if (rc == SHCI_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(ctx, SHCI_ERR_HIP, "singles sync");
-      release(ctx, pairs);
-      release(ctx, pairs_s);
-    }
-  } while (false);
-  if (rc == SHCI_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(ctx, SHCI_ERR_HIP, "singles sync");
+    release(ctx, uniq);
+    CHECK_RC(rc);
+  }
+  const uint32_t K = nu * nd;
+  const uint64_t m = n * K;
+  ctx->n_dm1 = m;
+  if (m == 0 || m >= (1ull << 31)) {
+    if (m >= (1ull << 31)) return fail(ctx, SHCI_ERR_UNSUPPORTED, "n_dets * n_up * n_dn >= 2^31");
+    ctx->n_dm1_runs = 0;
+    return SHCI_OK;
+  }
+  const int bb = bits_for(ctx->r_beta), ba = bits_for(ctx->r_alpha);
+  if (ba + bb > 64) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment keys exceed 64 bits");
+  DevBuf keys, keys_s, vals, flag, incl;
+  auto cleanup = [&]() {
+    release(ctx, keys);
+    release(ctx, keys_s);
+    release(ctx, vals);
+    release(ctx, flag);
+    release(ctx, incl);
+  };
+  int rc = SHCI_OK;
+  if ((rc = ensure(ctx, keys, m * 8)) || (rc = ensure(ctx, keys_s, m * 8)) || (rc = ensure(ctx, vals, m * 4)) ||
+      (rc = ensure(ctx, ctx->dm1_vals, m * 4)) || (rc = ensure(ctx, ctx->dm1_seg, m * 4))) {
+    cleanup();
+    return rc;
+  }
+  k_dm1_keys<<<div_up(m, 256), 256, 0, st>>>(P<uint32_t>(ctx->up_gid), P<uint32_t>(ctx->dn_gid),
+                                             P<uint32_t>(ctx->m1a_id), P<uint32_t>(ctx->m1b_id), n, nu, nd, bb,
+                                             P<uint64_t>(keys), P<uint32_t>(vals));
+  rc = radix_sort_pairs(ctx, P<uint64_t>(keys), P<uint64_t>(keys_s), P<uint32_t>(vals), P<uint32_t>(ctx->dm1_vals),
+                        m, ba + bb);
+  release(ctx, keys);
+  release(ctx, vals);
+  if (rc == SHCI_OK) rc = ensure(ctx, flag, m * 4);
+  if (rc == SHCI_OK) rc = ensure(ctx, incl, m * 4);
+  if (rc == SHCI_OK) {
+    k_head_flags<<<div_up(m, 256), 256, 0, st>>>(P<uint64_t>(keys_s), P<uint32_t>(flag), m);
+    rc = incl_scan(ctx, P<uint32_t>(flag), P<uint32_t>(incl), m);
+  }
+  release(ctx, keys_s);
+  if (rc == SHCI_OK) rc = ensure(ctx, ctx->dm1_rstart, (m + 1) * 4);
+  if (rc == SHCI_OK) {
+    k_run_starts<<<div_up(m, 256), 256, 0, st>>>(P<uint32_t>(flag), P<uint32_t>(incl), P<uint32_t>(ctx->dm1_rstart),
+                                                 m);
+    k_scatter_run_id<<<div_up(m, 256), 256, 0, st>>>(P<uint32_t>(ctx->dm1_vals), P<uint32_t>(incl), m,
+                                                     P<uint32_t>(ctx->dm1_seg));
+    uint32_t runs = 0;
+    if (hipMemcpyAsync(&runs, P<uint32_t>(incl) + (m - 1), 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      rc = fail(ctx, SHCI_ERR_HIP, "segments");
+    ctx->n_dm1_runs = runs;
+  }
   cleanup();
   return rc;
 }
@@ -1260,11 +1338,15 @@ RowParams row_params(shci_ctx* ctx) {
   Pm.dn_gstart = P<uint32_t>(ctx->dn_gstart);
   Pm.dnord_idx = P<uint32_t>(ctx->dnord_idx);
   Pm.dnord_up = P<uint64_t>(ctx->dnord_up);
-  Pm.sing_ptr = P<uint64_t>(ctx->sing_ptr);
-  Pm.sing = P<uint32_t>(ctx->sing);
+  Pm.dm1_vals = P<uint32_t>(ctx->dm1_vals);
+  Pm.dm1_rstart = P<uint32_t>(ctx->dm1_rstart);
+  Pm.dm1_seg = P<uint32_t>(ctx->dm1_seg);
+  Pm.K = ctx->n_up * ctx->n_dn;
+  Pm.overflow = reinterpret_cast<unsigned int*>(ctx->flag.p);
   Pm.I = make_ints(ctx);
   Pm.cnt = P<uint32_t>(ctx->cnt);
   Pm.row_ptr = P<int64_t>(ctx->row_ptr);
+  Pm.row_end = P<int64_t>(ctx->row_end);
   Pm.col = P<uint32_t>(ctx->col);
   Pm.v64 = P<double>(ctx->v64);
   Pm.v32 = P<float>(ctx->v32);
@@ -1283,13 +1365,15 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
   HIPC(hipEventRecord(ctx->ev[0], st));
   CHECK_RC(build_groups(ctx));
   HIPC(hipEventRecord(ctx->ev[1], st));
-  CHECK_RC(build_singles(ctx));
+  CHECK_RC(build_segments(ctx));
   HIPC(hipEventRecord(ctx->ev[2], st));
   CHECK_RC(ensure(ctx, ctx->diag, n * 8));
   k_diag<<<div_up(n, 256), 256, 0, st>>>(make_ints(ctx), P<uint64_t>(ctx->up_s), P<uint64_t>(ctx->dn_s),
                                          P<double>(ctx->diag), n);
   HIPC(hipEventRecord(ctx->ev[3], st));
   // Pass 1: count candidates per row.
+  CHECK_RC(ensure(ctx, ctx->flag, 256));
+  HIPC(hipMemsetAsync(ctx->flag.p, 0, 256, st));
   CHECK_RC(ensure(ctx, ctx->cnt, n * 4));
   CHECK_RC(ensure(ctx, ctx->len, n * 4));
   CHECK_RC(ensure(ctx, ctx->row_ptr, (n + 1) * 8));
@@ -1313,13 +1397,21 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
   int64_t total = 0;
   HIPC(hipMemcpyAsync(&total, P<int64_t>(ctx->row_ptr) + n, 8, hipMemcpyDeviceToHost, st));
   HIPC(hipStreamSynchronize(st));
+  // Grow-only CSR storage sized by the candidate count; rows whose |H| cut
+  // drops entries keep a short row_end (no compaction, no reallocation).
   const uint64_t cap = std::max<int64_t>(total, 1);
-  release(ctx, ctx->col);
-  release(ctx, ctx->v64);
-  release(ctx, ctx->v32);
+  CHECK_RC(ensure(ctx, ctx->row_end, n * 8));
   CHECK_RC(ensure(ctx, ctx->col, cap * 4));
-  if (flags & SHCI_STORE_FP64) CHECK_RC(ensure(ctx, ctx->v64, cap * 8));
-  if (flags & SHCI_STORE_FP32) CHECK_RC(ensure(ctx, ctx->v32, cap * 4));
+  if (flags & SHCI_STORE_FP64) {
+    CHECK_RC(ensure(ctx, ctx->v64, cap * 8));
+  } else {
+    release(ctx, ctx->v64);
+  }
+  if (flags & SHCI_STORE_FP32) {
+    CHECK_RC(ensure(ctx, ctx->v32, cap * 4));
+  } else {
+    release(ctx, ctx->v32);
+  }
   HIPC(hipEventRecord(ctx->ev[5], st));
   // Pass 2: fill.
   {
@@ -1328,45 +1420,24 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
     HIPC(hipGetLastError());
   }
   HIPC(hipEventRecord(ctx->ev[6], st));
-  // Compact if the |H| cut dropped anything.
   {
-    DevBuf len64, new_rp;
-    CHECK_RC(ensure(ctx, len64, (n + 1) * 8));
-    HIPC(hipMemsetAsync(len64.p, 0, (n + 1) * 8, st));
-    k_widen<<<div_up(n, 256), 256, 0, st>>>(P<uint32_t>(ctx->len), P<int64_t>(len64), n);
-    int rc = ensure(ctx, new_rp, (n + 1) * 8);
-    if (rc == SHCI_OK) rc = excl_scan(ctx, P<int64_t>(len64), P<int64_t>(new_rp), n + 1);
-    int64_t kept = 0;
-    if (rc == SHCI_OK) {
-      if (hipMemcpyAsync(&kept, P<int64_t>(new_rp) + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-          hipStreamSynchronize(st) != hipSuccess)
-        rc = fail(ctx, SHCI_ERR_HIP, "compact count");
-    }
-    if (rc == SHCI_OK && kept != total) {
-      DevBuf col2, v642, v322;
-      const uint64_t cap2 = std::max<int64_t>(kept, 1);
-      rc = ensure(ctx, col2, cap2 * 4);
-      if (rc == SHCI_OK && ctx->v64.p) rc = ensure(ctx, v642, cap2 * 8);
-      if (rc == SHCI_OK && ctx->v32.p) rc = ensure(ctx, v322, cap2 * 4);
-      if (rc == SHCI_OK) {
-        k_compact<<<div_up(n, 4), 256, 0, st>>>(n, P<int64_t>(ctx->row_ptr), P<int64_t>(new_rp), P<uint32_t>(ctx->col),
-                                                P<double>(ctx->v64), P<float>(ctx->v32), P<uint32_t>(col2),
-                                                P<double>(v642), P<float>(v322));
-        if (hipMemcpyAsync(ctx->row_ptr.p, new_rp.p, (n + 1) * 8, hipMemcpyDeviceToDevice, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess)
-          rc = fail(ctx, SHCI_ERR_HIP, "compact");
-        std::swap(ctx->col, col2);
-        std::swap(ctx->v64, v642);
-        std::swap(ctx->v32, v322);
-      }
-      release(ctx, col2);
-      release(ctx, v642);
-      release(ctx, v322);
-      total = kept;
-    }
-    release(ctx, len64);
-    release(ctx, new_rp);
-    CHECK_RC(rc);
+    unsigned int ovf = 0;
+    HIPC(hipMemcpyAsync(&ovf, ctx->flag.p, 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    if (ovf) return fail(ctx, SHCI_ERR_STATE, "internal error: fill pass found more connections than the count pass");
+  }
+  // Stored off-diagonal count (sum of row lengths after the |H| cut).
+  {
+    CHECK_RC(ensure(ctx, ctx->scal, 64 * 8));
+    unsigned long long* sum = reinterpret_cast<unsigned long long*>(ctx->scal.p);
+    HIPC(hipMemsetAsync(sum, 0, 8, st));
+    k_sum_u32<<<std::min<uint32_t>(1024, std::max<uint32_t>(1, div_up(n, 256))), 256, 0, st>>>(P<uint32_t>(ctx->len), n,
+                                                                                               sum);
+    unsigned long long kept = 0;
+    HIPC(hipMemcpyAsync(&kept, sum, 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    ctx->n_dropped = (uint64_t)total - kept;
+    total = (int64_t)kept;
   }
   HIPC(hipEventRecord(ctx->ev[7], st));
   HIPC(hipEventSynchronize(ctx->ev[7]));
@@ -1386,13 +1457,14 @@ int launch_spmv(shci_ctx* ctx, const VT* val, double shift, const XT* x, XT* y) 
   const uint32_t grid = div_up(n, 256 / L);
   const hipStream_t st = ctx->stream;
   const int64_t* rp = P<int64_t>(ctx->row_ptr);
+  const int64_t* re = P<int64_t>(ctx->row_end);
   const uint32_t* col = P<uint32_t>(ctx->col);
   const double* d = P<double>(ctx->diag);
   switch (L) {
-    case 64: k_spmv<64, VT, XT, AT><<<grid, 256, 0, st>>>(n, rp, col, val, d, shift, x, y); break;
-    case 32: k_spmv<32, VT, XT, AT><<<grid, 256, 0, st>>>(n, rp, col, val, d, shift, x, y); break;
-    case 16: k_spmv<16, VT, XT, AT><<<grid, 256, 0, st>>>(n, rp, col, val, d, shift, x, y); break;
-    default: k_spmv<8, VT, XT, AT><<<grid, 256, 0, st>>>(n, rp, col, val, d, shift, x, y); break;
+    case 64: k_spmv<64, VT, XT, AT><<<grid, 256, 0, st>>>(n, rp, re, col, val, d, shift, x, y); break;
+    case 32: k_spmv<32, VT, XT, AT><<<grid, 256, 0, st>>>(n, rp, re, col, val, d, shift, x, y); break;
+    case 16: k_spmv<16, VT, XT, AT><<<grid, 256, 0, st>>>(n, rp, re, col, val, d, shift, x, y); break;
+    default: k_spmv<8, VT, XT, AT><<<grid, 256, 0, st>>>(n, rp, re, col, val, d, shift, x, y); break;
   }
   HIPC(hipGetLastError());
   return SHCI_OK;
@@ -1404,14 +1476,15 @@ int launch_spmm4(shci_ctx* ctx, double shift, const float* x, float* y) {
   const uint32_t grid = div_up(n, 256 / L);
   const hipStream_t st = ctx->stream;
   const int64_t* rp = P<int64_t>(ctx->row_ptr);
+  const int64_t* re = P<int64_t>(ctx->row_end);
   const uint32_t* col = P<uint32_t>(ctx->col);
   const float* v = P<float>(ctx->v32);
   const double* d = P<double>(ctx->diag);
   switch (L) {
-    case 64: k_spmm_f32<64, 4><<<grid, 256, 0, st>>>(n, rp, col, v, d, shift, x, y); break;
-    case 32: k_spmm_f32<32, 4><<<grid, 256, 0, st>>>(n, rp, col, v, d, shift, x, y); break;
-    case 16: k_spmm_f32<16, 4><<<grid, 256, 0, st>>>(n, rp, col, v, d, shift, x, y); break;
-    default: k_spmm_f32<8, 4><<<grid, 256, 0, st>>>(n, rp, col, v, d, shift, x, y); break;
+    case 64: k_spmm_f32<64, 4><<<grid, 256, 0, st>>>(n, rp, re, col, v, d, shift, x, y); break;
+    case 32: k_spmm_f32<32, 4><<<grid, 256, 0, st>>>(n, rp, re, col, v, d, shift, x, y); break;
+    case 16: k_spmm_f32<16, 4><<<grid, 256, 0, st>>>(n, rp, re, col, v, d, shift, x, y); break;
+    default: k_spmm_f32<8, 4><<<grid, 256, 0, st>>>(n, rp, re, col, v, d, shift, x, y); break;
   }
   HIPC(hipGetLastError());
   return SHCI_OK;
@@ -1678,8 +1751,9 @@ int shci_ctx_destroy(shci_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   DevBuf* bufs[] = {&ctx->h1, &ctx->eri, &ctx->tD, &ctx->tX, &ctx->up_c, &ctx->dn_c, &ctx->up_s, &ctx->dn_s,
                     &ctx->perm, &ctx->iperm, &ctx->up_gid, &ctx->up_gstart, &ctx->dn_gid, &ctx->dn_gstart,
-                    &ctx->dnord_idx, &ctx->dnord_up, &ctx->sing_ptr, &ctx->sing, &ctx->diag, &ctx->cnt,
-                    &ctx->row_ptr, &ctx->col, &ctx->v64, &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb,
+                    &ctx->dnord_idx, &ctx->dnord_up, &ctx->m1a_id, &ctx->m1b_id, &ctx->dm1_vals, &ctx->dm1_rstart,
+                    &ctx->dm1_seg, &ctx->flag, &ctx->diag, &ctx->cnt,
+                    &ctx->row_ptr, &ctx->row_end, &ctx->col, &ctx->v64, &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb,
                     &ctx->va, &ctx->vb, &ctx->scal, &ctx->part, &ctx->xv, &ctx->yv, &ctx->x32, &ctx->y32,
                     &ctx->dav};
   for (DevBuf* b : bufs) release(ctx, *b);
@@ -1782,7 +1856,7 @@ int shci_update_existing_elems(shci_ctx* ctx) {
   HIPC(hipSetDevice(ctx->device));
   k_update_values<<<div_up(ctx->n, 4), 256, 0, ctx->stream>>>(
       ctx->n, make_ints(ctx), P<uint64_t>(ctx->up_s), P<uint64_t>(ctx->dn_s), P<int64_t>(ctx->row_ptr),
-      P<uint32_t>(ctx->col), P<double>(ctx->v64), P<float>(ctx->v32), P<double>(ctx->diag));
+      P<int64_t>(ctx->row_end), P<uint32_t>(ctx->col), P<double>(ctx->v64), P<float>(ctx->v32), P<double>(ctx->diag));
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(ctx->stream));
   return SHCI_OK;
@@ -2068,6 +2142,7 @@ int shci_rdm(shci_ctx* ctx, uint32_t n_states, const double* coefs, double* rdm1
   R.up = P<uint64_t>(ctx->up_s);
   R.dn = P<uint64_t>(ctx->dn_s);
   R.rp = P<int64_t>(ctx->row_ptr);
+  R.re = P<int64_t>(ctx->row_end);
   R.col = P<uint32_t>(ctx->col);
   R.c = P<double>(ci);
   R.rdm1 = P<double>(r1);
@@ -2106,7 +2181,7 @@ int shci_export_upper(shci_ctx* ctx, int64_t* row_ptr, int64_t* cols, double* va
     cleanup();
     return fail(ctx, SHCI_ERR_HIP, "export memset");
   }
-  k_export_count<<<div_up(n, 4), 256, 0, st>>>(n, P<int64_t>(ctx->row_ptr), P<uint32_t>(ctx->col),
+  k_export_count<<<div_up(n, 4), 256, 0, st>>>(n, P<int64_t>(ctx->row_ptr), P<int64_t>(ctx->row_end), P<uint32_t>(ctx->col),
                                                 P<uint32_t>(ctx->perm), P<unsigned long long>(cnt));
   rc = excl_scan(ctx, P<unsigned long long>(cnt), P<unsigned long long>(orp), n + 1);
   unsigned long long total = 0;
@@ -2117,7 +2192,7 @@ int shci_export_upper(shci_ctx* ctx, int64_t* row_ptr, int64_t* cols, double* va
   if (rc == SHCI_OK) rc = ensure(ctx, oc, m * 8);
   if (rc == SHCI_OK) rc = ensure(ctx, ov, m * 8);
   if (rc == SHCI_OK) {
-    k_export_fill<<<div_up(n, 4), 256, 0, st>>>(n, P<int64_t>(ctx->row_ptr), P<uint32_t>(ctx->col), P<double>(ctx->v64),
+    k_export_fill<<<div_up(n, 4), 256, 0, st>>>(n, P<int64_t>(ctx->row_ptr), P<int64_t>(ctx->row_end), P<uint32_t>(ctx->col), P<double>(ctx->v64),
                                                P<double>(ctx->diag), P<uint32_t>(ctx->perm),
                                                P<unsigned long long>(orp), P<int64_t>(oc), P<double>(ov));
     if (hipMemcpyAsync(row_ptr, orp.p, (n + 1) * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -2128,6 +2203,23 @@ int shci_export_upper(shci_ctx* ctx, int64_t* row_ptr, int64_t* cols, double* va
   }
   cleanup();
   return rc;
+}
+
+int shci_clear(shci_ctx* ctx) {
+  if (!ctx) return SHCI_ERR_ARG;
+  HIPC(hipSetDevice(ctx->device));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  DevBuf* bufs[] = {&ctx->up_c, &ctx->dn_c, &ctx->up_s, &ctx->dn_s, &ctx->perm, &ctx->iperm, &ctx->up_gid,
+                    &ctx->up_gstart, &ctx->dn_gid, &ctx->dn_gstart, &ctx->dnord_idx, &ctx->dnord_up,
+                    &ctx->m1a_id, &ctx->m1b_id, &ctx->dm1_vals, &ctx->dm1_rstart, &ctx->dm1_seg, &ctx->diag,
+                    &ctx->cnt, &ctx->row_ptr, &ctx->row_end, &ctx->row_end, &ctx->col, &ctx->v64,
+                    &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb, &ctx->va, &ctx->vb, &ctx->xv,
+                    &ctx->yv, &ctx->x32, &ctx->y32, &ctx->dav};
+  for (DevBuf* b : bufs) release(ctx, *b);
+  ctx->n = 0;
+  ctx->nnz = 0;
+  ctx->built = false;
+  return SHCI_OK;
 }
 
 int shci_get_build_timings(shci_ctx* ctx, float* phases) {
@@ -2174,8 +2266,8 @@ int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, doubl
   if (bytes) {
     const double F = (double)ctx->nnz;  // off-diagonal full-symmetric entries
     const double vs = fp32 ? 4.0 : 8.0;
-    // values + columns, row pointers, FP64 diagonal, x once, y once.
-    *bytes = F * (vs + 4.0) + (double)(n + 1) * 8.0 + (double)n * 8.0 + 2.0 * (double)n * vs;
+    // values + columns, row start/end (int64 each), FP64 diagonal, x once, y once.
+    *bytes = F * (vs + 4.0) + (double)n * 16.0 + (double)n * 8.0 + 2.0 * (double)n * vs;
   }
   return SHCI_OK;
 }

@@ -227,8 +227,10 @@ class Hamiltonian:
         self.ctx.check(self.ctx.lib.shci_update_existing_elems(self.ctx.handle), "shci_update_existing_elems")
 
     def clear(self):
+        """hamiltonian.h:128-155: drops the dets and the matrix (device memory released)."""
         self.n_dets = self.n_dets_prev = 0
         self.matrix.clear()
+        self.ctx.check(self.ctx.lib.shci_clear(self.ctx.handle), "shci_clear")
 
     def rdm(self, coefs):
         """(1-RDM n x n, packed 2-RDM) of ``coefs`` (n_states x n_dets)."""
