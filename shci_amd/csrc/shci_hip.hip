@@ -27,6 +27,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -90,6 +91,7 @@ struct shci_ctx {
   DevBuf diag, cnt, row_ptr, row_end, col, v64, v32, len;
   uint64_t nnz = 0;        // stored off-diagonal entries (full symmetric)
   uint64_t n_dropped = 0;  // candidates removed by the |H| >= 1e-12 cut
+  uint64_t nnz_alloc = 0;  // allocated (padded) CSR slots
   int lanes = 16;
 
   // Scratch.
@@ -597,12 +599,21 @@ __global__ __launch_bounds__(256) void k_rows(RowParams Pm) {
       }
     }
   }
+  if (FILL) {
+    // Padding slots (rounding to 4 and entries dropped by the |H| cut): a
+    // zero-valued self reference, which every consumer skips or adds as 0.
+    for (uint64_t k = em.base + lane; k < em.end; k += kWave) {
+      Pm.col[k] = (uint32_t)r;
+      if (Pm.v64) Pm.v64[k] = 0.0;
+      if (Pm.v32) Pm.v32[k] = 0.0f;
+    }
+  }
   if (lane == 0) {
     if (FILL) {
       Pm.len[r] = (uint32_t)(em.base - (uint64_t)Pm.row_ptr[r]);
-      Pm.row_end[r] = (int64_t)em.base;
+      Pm.row_end[r] = (int64_t)em.end;
     } else {
-      Pm.cnt[r] = (uint32_t)em.base;
+      Pm.cnt[r] = (uint32_t)((em.base + 3) & ~3ull);  // rows padded to a multiple of 4 entries
     }
   }
 }
@@ -632,6 +643,7 @@ __global__ __launch_bounds__(256) void k_update_values(uint64_t n, Ints I, const
   if (lane_id() == 0) diag[r] = diag_elem(I, ui, di);
   for (int64_t k = rp[r] + lane_id(); k < re[r]; k += kWave) {
     const uint32_t j = col[k];
+    if (j == r) continue;  // padding
     const double v = elem_any(I, ui, di, up[j], dn[j]);
     if (v64) v64[k] = v;
     if (v32) v32[k] = (float)v;
@@ -643,6 +655,30 @@ __global__ __launch_bounds__(256) void k_update_values(uint64_t n, Ints I, const
 //   y_i = (d_i - shift) x_i + sum_j A_ij x_j
 // VT: stored value type; XT: vector type; AT: row accumulator type.
 // ---------------------------------------------------------------------------
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// Four stored values starting at a multiple of 4 (16-byte / 2x16-byte loads).
+__device__ __forceinline__ void load4(const float* __restrict__ v, int64_t k, float (&a)[4]) {
+  const f32x4 t = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(v + k));
+  a[0] = t.x;
+  a[1] = t.y;
+  a[2] = t.z;
+  a[3] = t.w;
+}
+__device__ __forceinline__ void load4(const double* __restrict__ v, int64_t k, double (&a)[4]) {
+  const f64x2 t0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(v + k));
+  const f64x2 t1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(v + k + 2));
+  a[0] = t0.x;
+  a[1] = t0.y;
+  a[2] = t1.x;
+  a[3] = t1.y;
+}
+
+// Rows start and end at multiples of 4 entries; each lane takes 4 consecutive
+// entries per step (one 16-byte column load + the value load), two steps in
+// flight, then the dependent x gathers.
 template <int L, typename VT, typename XT, typename AT>
 __global__ __launch_bounds__(256) void k_spmv(uint64_t n, const int64_t* __restrict__ rp,
                                               const int64_t* __restrict__ re, const uint32_t* __restrict__ col,
@@ -653,19 +689,26 @@ __global__ __launch_bounds__(256) void k_spmv(uint64_t n, const int64_t* __restr
   AT acc0 = 0, acc1 = 0;
   if (row < n) {
     const int64_t b = rp[row], e = re[row];
-    int64_t k = b + sub;
-    for (; k + L < e; k += 2 * L) {
-      const uint32_t c0 = __builtin_nontemporal_load(&col[k]);
-      const uint32_t c1 = __builtin_nontemporal_load(&col[k + L]);
-      const VT a0 = __builtin_nontemporal_load(&val[k]);
-      const VT a1 = __builtin_nontemporal_load(&val[k + L]);
-      acc0 += (AT)a0 * (AT)x[c0];
-      acc1 += (AT)a1 * (AT)x[c1];
+    int64_t k = b + 4 * (int64_t)sub;
+    for (; k + 4 * L < e; k += 8 * L) {
+      const u32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(col + k));
+      const u32x4 c1 = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(col + k + 4 * L));
+      VT a0[4], a1[4];
+      load4(val, k, a0);
+      load4(val, k + 4 * L, a1);
+      const XT x0 = x[c0.x], x1 = x[c0.y], x2 = x[c0.z], x3 = x[c0.w];
+      const XT x4 = x[c1.x], x5 = x[c1.y], x6 = x[c1.z], x7 = x[c1.w];
+      acc0 += (AT)a0[0] * (AT)x0 + (AT)a0[1] * (AT)x1;
+      acc1 += (AT)a0[2] * (AT)x2 + (AT)a0[3] * (AT)x3;
+      acc0 += (AT)a1[0] * (AT)x4 + (AT)a1[1] * (AT)x5;
+      acc1 += (AT)a1[2] * (AT)x6 + (AT)a1[3] * (AT)x7;
     }
     if (k < e) {
-      const uint32_t c0 = __builtin_nontemporal_load(&col[k]);
-      const VT a0 = __builtin_nontemporal_load(&val[k]);
-      acc0 += (AT)a0 * (AT)x[c0];
+      const u32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(col + k));
+      VT a0[4];
+      load4(val, k, a0);
+      acc0 += (AT)a0[0] * (AT)x[c0.x] + (AT)a0[1] * (AT)x[c0.y];
+      acc1 += (AT)a0[2] * (AT)x[c0.z] + (AT)a0[3] * (AT)x[c0.w];
     }
   }
   AT acc = acc0 + acc1;
@@ -1320,10 +1363,16 @@ int build_segments(shci_ctx* ctx) {
   return rc;
 }
 
+// Lanes per row for the SpMV: each lane consumes 8 entries per loop trip
+// (two 4-entry steps).  SHCI_SPMV_LANES overrides (8, 16, 32 or 64).
 int choose_lanes(double mean) {
-  if (mean >= 96) return 64;
-  if (mean >= 48) return 32;
-  if (mean >= 20) return 16;
+  if (const char* env = getenv("SHCI_SPMV_LANES")) {
+    const int v = atoi(env);
+    if (v == 8 || v == 16 || v == 32 || v == 64) return v;
+  }
+  if (mean >= 384) return 64;
+  if (mean >= 160) return 32;
+  if (mean >= 64) return 16;
   return 8;
 }
 
@@ -1400,6 +1449,7 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
   // Grow-only CSR storage sized by the candidate count; rows whose |H| cut
   // drops entries keep a short row_end (no compaction, no reallocation).
   const uint64_t cap = std::max<int64_t>(total, 1);
+  ctx->nnz_alloc = (uint64_t)total;
   CHECK_RC(ensure(ctx, ctx->row_end, n * 8));
   CHECK_RC(ensure(ctx, ctx->col, cap * 4));
   if (flags & SHCI_STORE_FP64) {
@@ -1442,7 +1492,7 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
   HIPC(hipEventRecord(ctx->ev[7], st));
   HIPC(hipEventSynchronize(ctx->ev[7]));
   ctx->nnz = (uint64_t)total;
-  ctx->lanes = choose_lanes(n ? (double)total / (double)n : 0.0);
+  ctx->lanes = choose_lanes(n ? (double)ctx->nnz_alloc / (double)n : 0.0);
   for (int k = 0; k < 7; k++) HIPC(hipEventElapsedTime(&ctx->phase_ms[k], ctx->ev[k], ctx->ev[k + 1]));
   HIPC(hipEventElapsedTime(&ctx->phase_ms[7], ctx->ev[0], ctx->ev[7]));
   ctx->built = true;
