@@ -86,6 +86,7 @@ def main():
     ap.add_argument("--cpu-sample-dets", type=int, default=250_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-7)
+    ap.add_argument("--layout", choices=["tiled", "csr"], default="tiled")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU row sharding is "
@@ -101,7 +102,7 @@ def main():
     init[0] = 1.0  # the HF determinant is det 0
 
     # ---------------- FP64 path ----------------
-    ham = Hamiltonian(w.n, w.h1, w.eri, w.core, store="fp64")
+    ham = Hamiltonian(w.n, w.h1, w.eri, w.core, store="fp64", layout=args.layout)
     t0 = time.time()
     ham.update(w.up, w.dn)
     t_upload_build64 = time.time() - t0
@@ -157,11 +158,17 @@ def main():
     spmv_total = spmv32_ms * max(1, n_spmv_est)
     F = nnz_full_off
     if fill_avg >= spmv_total:
-        fill_bytes = F * (4 + 4) + (n_dets + 1) * 8 + n_dets * 16
-        roof = dict(kernel="k_rows<fill> (Hamiltonian build, FP32 store)", bound="hbm",
-                    achieved=fill_bytes / (fill_avg / 1e3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s")
+        if args.layout == "tiled":
+            fill_bytes = (F // 2) * (4 + 2) + n_dets * 16
+            kname = "k_tile_pairs<fill> (Hamiltonian build, FP32 tiles)"
+        else:
+            fill_bytes = F * (4 + 4) + (n_dets + 1) * 8 + n_dets * 16
+            kname = "k_rows<fill> (Hamiltonian build, FP32 CSR)"
+        roof = dict(kernel=kname, bound="hbm", achieved=fill_bytes / (fill_avg / 1e3) / 1e9, peak=HBM_PEAK_GBS,
+                    unit="GB/s")
     else:
-        roof = dict(kernel="k_spmv<fp32>", bound="hbm", achieved=spmv32_bytes / (spmv32_ms / 1e3) / 1e9,
+        kname = "k_tile_spmv<float,float,float>" if args.layout == "tiled" else "k_spmv<fp32>"
+        roof = dict(kernel=kname, bound="hbm", achieved=spmv32_bytes / (spmv32_ms / 1e3) / 1e9,
                     peak=HBM_PEAK_GBS, unit="GB/s")
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = None
@@ -197,12 +204,13 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": (f"cfg{args.config}: synthetic FCIDUMP n_orb={w.n}, n_up=n_dn={w.spec.n_up}, "
-                         f"{n_dets} variational dets; step = device-resident H build (FP32 CSR) + mixed "
+                         f"{n_dets} variational dets; step = device-resident H build (FP32, {args.layout} layout) + mixed "
                          f"Davidson to |dE|<={args.tol:g} + FP64 energy re-evaluation"),
             "n_dets": n_dets,
             "stored_upper": stored_upper,
             "nnz_full_offdiag": nnz_full_off,
             "parallelism": "1 GPU",
+            "layout": args.layout,
         },
         "time_to_1uHa_mixed_s": t_mixed_first if t_mixed_first is not None else t_mixed,
         "time_to_1uHa_fp64_s": t_fp64_first if t_fp64_first is not None else t_fp64,
@@ -219,6 +227,8 @@ def main():
         "spmv_fp32_GBps": spmv32_bytes / spmv32_ms / 1e6,
         "spmv_fp64_GBps": spmv64_bytes / spmv64_ms / 1e6,
         "spmv_fp32_ms": spmv32_ms,
+        "spmv_fp32_csr_equiv_GBps": (F * 8 + n_dets * 32) / spmv32_ms / 1e6,
+        "spmv_fp64_csr_equiv_GBps": (F * 12 + n_dets * 40) / spmv64_ms / 1e6,
         "spmv_fp64_ms": spmv64_ms,
         "step_fp64_s": t_step64,
         "roofline": roof,

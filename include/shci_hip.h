@@ -44,7 +44,10 @@ enum {
   SHCI_STORE_FP32 = 1u << 1,   /* keep FP32 off-diagonal values (mixed path) */
   SHCI_KEEP_ZEROS = 1u << 2,   /* keep |H_ij| < 1e-12 connections (exact RDM adjoint) */
   SHCI_DEVICE_PTRS = 1u << 3,  /* vector/det arguments are device pointers */
-  SHCI_FP32 = 1u << 4          /* SpMV in FP32 storage / FP32 row accumulation */
+  SHCI_FP32 = 1u << 4,         /* SpMV in FP32 storage / FP32 row accumulation */
+  SHCI_LAYOUT_TILED = 1u << 5  /* build: tiled pair-block layout instead of the CSR
+                                  (each connected pair stored once; faster SpMV,
+                                  less memory; no row export) */
 };
 
 /* Davidson modes. */
@@ -138,6 +141,12 @@ int shci_get_build_timings(shci_ctx* ctx, float* phases);
 /* Average kernel time of `reps` SpMVs (device-resident vectors) and the
  * algorithmic bytes one SpMV moves.  flags: SHCI_FP32 selects the FP32 store. */
 int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, double* bytes);
+
+/* Layout statistics (no reference counterpart; diagnostics): stats[12] =
+ * n_dets, stored off-diagonal entries (both directions), allocated slots,
+ * tiled?, items, tiles, tile member slots, stored pairs, mixed-double
+ * segments, alpha groups, beta groups, SpMV lanes per row. */
+int shci_layout_stats(shci_ctx* ctx, uint64_t* stats);
 
 /* Device memory in use by the context (bytes). */
 int shci_memory(shci_ctx* ctx, uint64_t* bytes);

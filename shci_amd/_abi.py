@@ -16,6 +16,7 @@ SHCI_STORE_FP32 = 1 << 1
 SHCI_KEEP_ZEROS = 1 << 2
 SHCI_DEVICE_PTRS = 1 << 3
 SHCI_FP32 = 1 << 4
+SHCI_LAYOUT_TILED = 1 << 5
 SHCI_DAVIDSON_FP64 = 0
 SHCI_DAVIDSON_MIXED = 1
 
@@ -40,6 +41,7 @@ EXPORTS = {
     "shci_get_build_timings": (C.c_int, [_vp, _fp]),
     "shci_time_spmv": (C.c_int, [_vp, C.c_uint32, C.c_int, _fp, _dp]),
     "shci_memory": (C.c_int, [_vp, _u64p]),
+    "shci_layout_stats": (C.c_int, [_vp, _u64p]),
 }
 
 

@@ -99,6 +99,12 @@ struct shci_ctx {
   DevBuf xv, yv, x32, y32;
   DevBuf dav;  // Davidson vectors
 
+  // Tiled pair-block layout (SHCI_LAYOUT_TILED): every unordered connected
+  // pair stored once inside a tile of <= 255 member dets.
+  bool tiled = false;
+  uint64_t n_groups_all = 0, n_items = 0, n_tiles = 0, n_tm = 0, n_pairs = 0, n_pairs_alloc = 0;
+  DevBuf it_grp, it_cc, it_moff, tile_first, tm, tile_poff, tile_pend, pidx, pv32, pv64;
+
   hipEvent_t ev[9] = {};
   float phase_ms[8] = {};
 };
@@ -1126,6 +1132,387 @@ __global__ __launch_bounds__(256) void k_export_fill(uint64_t n, const int64_t* 
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Tiled pair-block layout.
+//
+// Every connected pair lies in exactly one "group": a beta group (same beta,
+// alpha single/double: the reference's same-beta scan, hamiltonian.h:430-442),
+// an alpha group (same alpha: hamiltonian.h:456-468) or a mixed-double segment
+// (dets sharing an (alpha - 1, beta - 1) core: hamiltonian.h:482-532).  Groups
+// are cut into chunks of <= 64 members; an "item" is a pair of chunks
+// (ca <= cb) of one group; consecutive items are packed into tiles of <= 255
+// member dets.  Each tile stores its connected pairs once, as a value plus
+// two 8-bit local member indices.  The SpMV stages a tile's x in LDS,
+// accumulates both directions of every pair into an LDS y with LDS atomics and
+// flushes one global atomic per member.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kChunk = 64;
+constexpr uint32_t kTileMembers = 128;  // tile id = member prefix / 128 -> <= 255 members per tile
+
+struct GroupSrc {
+  uint64_t n_c, n_a, n_b;                  // number of groups per source (C segments, A, B)
+  const uint32_t* __restrict__ c_start;    // dm1 run starts
+  const uint32_t* __restrict__ c_vals;     // dm1 sorted vals (det * K + slot)
+  uint32_t K;
+  const uint32_t* __restrict__ a_start;    // beta-group starts in beta order
+  const uint32_t* __restrict__ a_idx;      // beta order -> internal det
+  const uint32_t* __restrict__ b_start;    // alpha-group starts (internal order)
+};
+
+__device__ __forceinline__ void group_range(const GroupSrc& G, uint64_t gg, int& src, uint64_t& g, uint32_t& b,
+                                            uint32_t& m) {
+  if (gg < G.n_c) {
+    src = 0;
+    g = gg;
+    b = G.c_start[g];
+    m = G.c_start[g + 1] - b;
+  } else if (gg < G.n_c + G.n_a) {
+    src = 1;
+    g = gg - G.n_c;
+    b = G.a_start[g];
+    m = G.a_start[g + 1] - b;
+  } else {
+    src = 2;
+    g = gg - G.n_c - G.n_a;
+    b = G.b_start[g];
+    m = G.b_start[g + 1] - b;
+  }
+}
+
+__device__ __forceinline__ uint32_t group_member(const GroupSrc& G, int src, uint32_t pos) {
+  if (src == 0) return G.c_vals[pos] / G.K;
+  if (src == 1) return G.a_idx[pos];
+  return pos;
+}
+
+__device__ __forceinline__ uint32_t chunk_len_h(uint32_t m, uint32_t c) {
+  const uint32_t lo = c * kChunk;
+  return m - lo < kChunk ? m - lo : kChunk;
+}
+
+__global__ void k_group_items(GroupSrc G, uint64_t n_all, uint32_t* __restrict__ cnt) {
+  const uint64_t gg = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gg >= n_all) return;
+  int src;
+  uint64_t g;
+  uint32_t b, m;
+  group_range(G, gg, src, g, b, m);
+  const uint32_t nc = (m + kChunk - 1) / kChunk;
+  // every item has >= 2 members: a last chunk of one member has no triangle
+  cnt[gg] = m >= 2 ? nc * (nc + 1) / 2 - (chunk_len_h(m, nc - 1) == 1 ? 1u : 0u) : 0;
+}
+
+// items[off .. off + nc(nc+1)/2): (group, ca | cb << 16), ca <= cb, row-major.
+__global__ void k_fill_items(GroupSrc G, uint64_t n_all, const uint64_t* __restrict__ off,
+                             uint64_t* __restrict__ it_grp, uint32_t* __restrict__ it_cc) {
+  const uint64_t gg = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gg >= n_all) return;
+  int src;
+  uint64_t g;
+  uint32_t b, m;
+  group_range(G, gg, src, g, b, m);
+  if (m < 2) return;
+  const uint32_t nc = (m + kChunk - 1) / kChunk;
+  uint64_t k = off[gg];
+  for (uint32_t ca = 0; ca < nc; ca++)
+    for (uint32_t cb = ca; cb < nc; cb++) {
+      if (ca == cb && chunk_len_h(m, ca) < 2) continue;
+      it_grp[k] = gg;
+      it_cc[k] = ca | (cb << 16);
+      k++;
+    }
+}
+
+__device__ __forceinline__ uint32_t chunk_len(uint32_t m, uint32_t c) {
+  const uint32_t lo = c * kChunk;
+  return m - lo < kChunk ? m - lo : kChunk;
+}
+
+__global__ void k_item_members(GroupSrc G, const uint64_t* __restrict__ it_grp, const uint32_t* __restrict__ it_cc,
+                               uint64_t n_items, uint64_t* __restrict__ mc) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_items) return;
+  int src;
+  uint64_t g;
+  uint32_t b, m;
+  group_range(G, it_grp[i], src, g, b, m);
+  const uint32_t ca = it_cc[i] & 0xffffu, cb = it_cc[i] >> 16;
+  mc[i] = chunk_len(m, ca) + (cb != ca ? chunk_len(m, cb) : 0u);
+}
+
+// Tile boundaries: tile of item i = moff[i] / 128 (consecutive, no gaps).
+__global__ void k_tile_first(const uint64_t* __restrict__ moff, uint64_t n_items, uint64_t* __restrict__ tfirst,
+                             uint64_t n_tiles) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_items) return;
+  const uint64_t t = moff[i] / kTileMembers;
+  if (i == 0 || moff[i - 1] / kTileMembers != t) tfirst[t] = i;
+  if (i == n_items - 1) tfirst[n_tiles] = n_items;
+}
+
+__global__ void k_fill_tm(GroupSrc G, const uint64_t* __restrict__ it_grp, const uint32_t* __restrict__ it_cc,
+                          const uint64_t* __restrict__ moff, uint64_t n_items, uint32_t* __restrict__ tm) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_items) return;
+  int src;
+  uint64_t g;
+  uint32_t b, m;
+  group_range(G, it_grp[i], src, g, b, m);
+  const uint32_t ca = it_cc[i] & 0xffffu, cb = it_cc[i] >> 16;
+  uint64_t k = moff[i];
+  const uint32_t la = chunk_len(m, ca);
+  for (uint32_t t = 0; t < la; t++) tm[k++] = group_member(G, src, b + ca * kChunk + t);
+  if (cb != ca) {
+    const uint32_t lb = chunk_len(m, cb);
+    for (uint32_t t = 0; t < lb; t++) tm[k++] = group_member(G, src, b + cb * kChunk + t);
+  }
+}
+
+struct TileParams {
+  GroupSrc G;
+  const uint64_t* __restrict__ it_grp;
+  const uint32_t* __restrict__ it_cc;
+  const uint64_t* __restrict__ moff;
+  const uint64_t* __restrict__ tfirst;
+  const uint32_t* __restrict__ tm;
+  const uint64_t* __restrict__ up;
+  const uint64_t* __restrict__ dn;
+  Ints I;
+  uint32_t* __restrict__ tcnt;          // count pass: candidate pairs per tile
+  const uint64_t* __restrict__ poff;    // fill pass: tile pair offsets (bound)
+  uint64_t* __restrict__ pend;          // fill pass: end of kept pairs
+  uint16_t* __restrict__ pidx;
+  double* __restrict__ v64;
+  float* __restrict__ v32;
+  unsigned int* __restrict__ overflow;
+  int keep_zeros;
+};
+
+// Item layout inside a tile, in LDS.
+struct TileItems {
+  uint32_t q[128];    // pair prefix per item (q[ni] = total)
+  uint8_t src[127];
+  uint8_t ma[127];
+  uint8_t mb[127];    // 0 for a same-chunk (triangular) item
+  uint8_t base[127];  // local index of the item's first member
+};
+
+__device__ __forceinline__ void tri_pair(uint32_t m, uint32_t u, uint32_t& j, uint32_t& k) {
+  // row-major upper triangle of an m x m block: start(j) = j*m - j*(j+1)/2
+  const float fm = (float)(2 * m - 1);
+  int jj = (int)((fm - sqrtf(fm * fm - 8.0f * (float)u)) * 0.5f);
+  if (jj < 0) jj = 0;
+  if (jj > (int)m - 2) jj = (int)m - 2;
+  while (jj + 1 <= (int)m - 2 && (uint32_t)((jj + 1) * (int)m - (jj + 1) * (jj + 2) / 2) <= u) jj++;
+  while (jj > 0 && (uint32_t)(jj * (int)m - jj * (jj + 1) / 2) > u) jj--;
+  j = (uint32_t)jj;
+  k = j + 1 + (u - (uint32_t)(jj * (int)m - jj * (jj + 1) / 2));
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_tile_pairs(TileParams T) {
+  __shared__ uint64_t su[256], sd[256];
+  __shared__ uint32_t sid[256];
+  __shared__ TileItems it;
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t s_ni, s_nm;
+  const uint32_t tid = threadIdx.x;
+  const uint64_t t = blockIdx.x;
+  const uint64_t i0 = T.tfirst[t], i1 = T.tfirst[t + 1];
+  const uint64_t m0 = T.moff[i0];
+  const uint32_t nm = (uint32_t)(T.moff[i1] - m0);
+  const uint32_t ni = (uint32_t)(i1 - i0);
+  for (uint32_t l = tid; l < nm; l += 256) {
+    const uint32_t id = T.tm[m0 + l];
+    sid[l] = id;
+    su[l] = T.up[id];
+    sd[l] = T.dn[id];
+  }
+  if (tid == 0) {
+    uint32_t q = 0;
+    for (uint32_t r = 0; r < ni; r++) {
+      int src;
+      uint64_t g;
+      uint32_t b, m;
+      group_range(T.G, T.it_grp[i0 + r], src, g, b, m);
+      const uint32_t ca = T.it_cc[i0 + r] & 0xffffu, cb = T.it_cc[i0 + r] >> 16;
+      const uint32_t la = chunk_len(m, ca), lb = cb != ca ? chunk_len(m, cb) : 0u;
+      it.q[r] = q;
+      it.src[r] = (uint8_t)src;
+      it.ma[r] = (uint8_t)la;
+      it.mb[r] = (uint8_t)lb;
+      it.base[r] = (uint8_t)(T.moff[i0 + r] - m0);
+      q += lb ? la * lb : la * (la - 1) / 2;
+    }
+    it.q[ni] = q;
+    s_ni = ni;
+    s_nm = nm;
+  }
+  __syncthreads();
+  const uint32_t Q = it.q[ni];
+  uint64_t base = FILL ? T.poff[t] : 0;
+  const uint64_t end = FILL ? T.poff[t + 1] : 0;
+  uint32_t cnt = 0;
+  const uint32_t lane = tid & 63, wv = tid >> 6;
+  for (uint32_t q0 = 0; q0 < Q; q0 += 256) {
+    const uint32_t q = q0 + tid;
+    bool hit = false;
+    double v = 0.0;
+    uint32_t lj = 0, lk = 0;
+    if (q < Q) {
+      uint32_t lo = 0, hi = ni;  // largest r with it.q[r] <= q
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (it.q[mid] <= q) lo = mid; else hi = mid;
+      }
+      const uint32_t r = lo, u = q - it.q[r];
+      const uint32_t ma = it.ma[r], mb = it.mb[r];
+      uint32_t j, k;
+      if (mb) {
+        j = u / mb;
+        k = ma + (u - j * mb);
+      } else {
+        tri_pair(ma, u, j, k);
+      }
+      lj = it.base[r] + j;
+      lk = it.base[r] + k;
+      const uint64_t uj = su[lj], uk = su[lk], dj = sd[lj], dk = sd[lk];
+      const int src = it.src[r];
+      if (src == 0) {
+        hit = uj != uk && dj != dk;
+        if (FILL && hit) v = double_mixed(T.I, uj, uk, dj, dk);
+      } else if (src == 1) {
+        const int c = __popcll(uj ^ uk);
+        hit = c == 2 || c == 4;
+        if (FILL && hit) v = c == 2 ? single_elem(T.I, uj, uk, dj) : double_same(T.I, uj, uk);
+      } else {
+        const int c = __popcll(dj ^ dk);
+        hit = c == 2 || c == 4;
+        if (FILL && hit) v = c == 2 ? single_elem(T.I, dj, dk, uj) : double_same(T.I, dj, dk);
+      }
+    }
+    if (!FILL) {
+      cnt += hit ? 1u : 0u;
+    } else {
+      const bool keep = hit && (T.keep_zeros || fabs(v) >= kEps);
+      const uint64_t bm = __ballot(keep);
+      if (lane == 0) wsum[wv] = (uint32_t)__popcll(bm);
+      __syncthreads();
+      uint32_t before = 0, total = 0;
+      for (uint32_t w = 0; w < 4; w++) {
+        if (w < wv) before += wsum[w];
+        total += wsum[w];
+      }
+      if (keep) {
+        const uint64_t pos = base + before + mbcnt(bm);
+        if (pos < end) {
+          T.pidx[pos] = (uint16_t)(lj | (lk << 8));
+          if (T.v64) T.v64[pos] = v;
+          if (T.v32) T.v32[pos] = (float)v;
+        } else {
+          atomicOr(T.overflow, 1u);
+        }
+      }
+      base += total;
+      __syncthreads();
+    }
+  }
+  if (!FILL) {
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, kWave);
+    if (lane == 0) wsum[wv] = cnt;
+    __syncthreads();
+    if (tid == 0) T.tcnt[t] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  } else if (tid == 0) {
+    T.pend[t] = base;
+  }
+}
+
+// y = (d - shift) x  (the diagonal part of the tiled SpMV)
+template <typename XT>
+__global__ void k_diag_apply(const double* __restrict__ diag, double shift, const XT* __restrict__ x,
+                             XT* __restrict__ y, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (XT)((diag[i] - shift) * (double)x[i]);
+}
+
+// y += offdiag(H) x over tiles; VT stored value, XT vector type, AT LDS accumulator.
+template <typename VT, typename XT, typename AT>
+__global__ __launch_bounds__(256) void k_tile_spmv(const uint64_t* __restrict__ tfirst,
+                                                   const uint64_t* __restrict__ moff, const uint32_t* __restrict__ tm,
+                                                   const uint64_t* __restrict__ poff, const uint64_t* __restrict__ pend,
+                                                   const uint16_t* __restrict__ pidx, const VT* __restrict__ val,
+                                                   const XT* __restrict__ x, XT* __restrict__ y, uint64_t n_tiles) {
+  __shared__ AT sx[256], sy[256];
+  __shared__ uint32_t sid[256];
+  const uint64_t t = xcd_remap(blockIdx.x, gridDim.x);
+  if (t >= n_tiles) return;
+  const uint32_t tid = threadIdx.x;
+  const uint64_t m0 = moff[tfirst[t]];
+  const uint32_t nm = (uint32_t)(moff[tfirst[t + 1]] - m0);
+  for (uint32_t l = tid; l < nm; l += 256) {
+    const uint32_t id = tm[m0 + l];
+    sid[l] = id;
+    sx[l] = (AT)x[id];
+    sy[l] = (AT)0;
+  }
+  __syncthreads();
+  const uint64_t pe = pend[t];
+  for (uint64_t p = poff[t] + tid; p < pe; p += 256) {
+    const uint32_t ix = __builtin_nontemporal_load(&pidx[p]);
+    const AT a = (AT)__builtin_nontemporal_load(&val[p]);
+    const uint32_t lj = ix & 255u, lk = ix >> 8;
+    atomicAdd(&sy[lj], a * sx[lk]);
+    atomicAdd(&sy[lk], a * sx[lj]);
+  }
+  __syncthreads();
+  for (uint32_t l = tid; l < nm; l += 256) {
+    const AT v = sy[l];
+    if (v != (AT)0) atomicAdd(&y[sid[l]], (XT)v);
+  }
+}
+
+// RDM over tiles: each stored pair once, plus every diagonal.
+__global__ __launch_bounds__(256) void k_rdm_tiles(RdmParams R, const uint64_t* __restrict__ tfirst,
+                                                   const uint64_t* __restrict__ moff, const uint32_t* __restrict__ tm,
+                                                   const uint64_t* __restrict__ poff, const uint64_t* __restrict__ pend,
+                                                   const uint16_t* __restrict__ pidx, uint64_t n_tiles) {
+  const uint64_t t = blockIdx.x;
+  if (t >= n_tiles) return;
+  const uint64_t m0 = moff[tfirst[t]];
+  for (uint64_t p = poff[t] + threadIdx.x; p < pend[t]; p += 256) {
+    const uint32_t ix = pidx[p];
+    const uint32_t a = tm[m0 + (ix & 255u)], b = tm[m0 + (ix >> 8)];
+    rdm_pair(R, R.up[a], R.dn[a], a, R.up[b], R.dn[b], b);
+  }
+}
+
+__global__ void k_rdm_diag(RdmParams R) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < R.n) rdm_pair(R, R.up[i], R.dn[i], i, R.up[i], R.dn[i], i);
+}
+
+// Recompute stored pair values on the fixed tiled pattern.
+__global__ __launch_bounds__(256) void k_update_tiles(Ints I, const uint64_t* __restrict__ up,
+                                                      const uint64_t* __restrict__ dn,
+                                                      const uint64_t* __restrict__ tfirst,
+                                                      const uint64_t* __restrict__ moff, const uint32_t* __restrict__ tm,
+                                                      const uint64_t* __restrict__ poff,
+                                                      const uint64_t* __restrict__ pend,
+                                                      const uint16_t* __restrict__ pidx, double* __restrict__ v64,
+                                                      float* __restrict__ v32, uint64_t n_tiles) {
+  const uint64_t t = blockIdx.x;
+  if (t >= n_tiles) return;
+  const uint64_t m0 = moff[tfirst[t]];
+  for (uint64_t p = poff[t] + threadIdx.x; p < pend[t]; p += 256) {
+    const uint32_t ix = pidx[p];
+    const uint32_t a = tm[m0 + (ix & 255u)], b = tm[m0 + (ix >> 8)];
+    const double v = elem_any(I, up[a], dn[a], up[b], dn[b]);
+    if (v64) v64[p] = v;
+    if (v32) v32[p] = (float)v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host-side helpers
 // ---------------------------------------------------------------------------
@@ -1363,6 +1750,196 @@ int build_segments(shci_ctx* ctx) {
   return rc;
 }
 
+
+// ---------------------------------------------------------------------------
+// Tiled layout build (SHCI_LAYOUT_TILED).
+// ---------------------------------------------------------------------------
+__global__ void k_widen_u32_u64(const uint32_t* __restrict__ a, uint64_t* __restrict__ b, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+__global__ void k_sum_ranges(const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi, uint64_t n,
+                             unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[256];
+  unsigned long long v = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+    v += hi[i] - lo[i];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
+GroupSrc group_src(shci_ctx* ctx) {
+  GroupSrc G;
+  G.n_c = ctx->n_dm1_runs;
+  G.n_a = ctx->n_dgroups;
+  G.n_b = ctx->n_ugroups;
+  G.c_start = P<uint32_t>(ctx->dm1_rstart);
+  G.c_vals = P<uint32_t>(ctx->dm1_vals);
+  G.K = ctx->n_up * ctx->n_dn;
+  G.a_start = P<uint32_t>(ctx->dn_gstart);
+  G.a_idx = P<uint32_t>(ctx->dnord_idx);
+  G.b_start = P<uint32_t>(ctx->up_gstart);
+  return G;
+}
+
+// u32 counts -> u64 exclusive offsets (n + 1 entries); returns the total.
+int offsets_u64(shci_ctx* ctx, const uint32_t* cnt, uint64_t n, DevBuf& off, uint64_t* total) {
+  const hipStream_t st = ctx->stream;
+  DevBuf wide;
+  CHECK_RC(ensure(ctx, wide, (n + 1) * 8));
+  CHECK_RC(ensure(ctx, off, (n + 1) * 8));
+  int rc = SHCI_OK;
+  if (hipMemsetAsync(wide.p, 0, (n + 1) * 8, st) != hipSuccess) rc = fail(ctx, SHCI_ERR_HIP, "memset");
+  if (rc == SHCI_OK && n) k_widen_u32_u64<<<div_up(n, 256), 256, 0, st>>>(cnt, P<uint64_t>(wide), n);
+  if (rc == SHCI_OK) rc = excl_scan(ctx, P<uint64_t>(wide), P<uint64_t>(off), n + 1);
+  if (rc == SHCI_OK && (hipMemcpyAsync(total, P<uint64_t>(off) + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                        hipStreamSynchronize(st) != hipSuccess))
+    rc = fail(ctx, SHCI_ERR_HIP, "offsets");
+  release(ctx, wide);
+  return rc;
+}
+
+TileParams tile_params(shci_ctx* ctx) {
+  TileParams T;
+  T.G = group_src(ctx);
+  T.it_grp = P<uint64_t>(ctx->it_grp);
+  T.it_cc = P<uint32_t>(ctx->it_cc);
+  T.moff = P<uint64_t>(ctx->it_moff);
+  T.tfirst = P<uint64_t>(ctx->tile_first);
+  T.tm = P<uint32_t>(ctx->tm);
+  T.up = P<uint64_t>(ctx->up_s);
+  T.dn = P<uint64_t>(ctx->dn_s);
+  T.I = make_ints(ctx);
+  T.tcnt = P<uint32_t>(ctx->cnt);
+  T.poff = P<uint64_t>(ctx->tile_poff);
+  T.pend = P<uint64_t>(ctx->tile_pend);
+  T.pidx = P<uint16_t>(ctx->pidx);
+  T.v64 = P<double>(ctx->pv64);
+  T.v32 = P<float>(ctx->pv32);
+  T.overflow = reinterpret_cast<unsigned int*>(ctx->flag.p);
+  T.keep_zeros = (ctx->build_flags & SHCI_KEEP_ZEROS) ? 1 : 0;
+  return T;
+}
+
+int build_tiles(shci_ctx* ctx, uint32_t flags) {
+  const hipStream_t st = ctx->stream;
+  const GroupSrc G = group_src(ctx);
+  const uint64_t n_all = G.n_c + G.n_a + G.n_b;
+  ctx->n_groups_all = n_all;
+  // items per group
+  CHECK_RC(ensure(ctx, ctx->cnt, std::max<uint64_t>(n_all, ctx->n) * 4));
+  k_group_items<<<div_up(n_all, 256), 256, 0, st>>>(G, n_all, P<uint32_t>(ctx->cnt));
+  DevBuf goff;
+  uint64_t n_items = 0;
+  CHECK_RC(offsets_u64(ctx, P<uint32_t>(ctx->cnt), n_all, goff, &n_items));
+  ctx->n_items = n_items;
+  int rc = ensure(ctx, ctx->it_grp, std::max<uint64_t>(n_items, 1) * 8);
+  if (rc == SHCI_OK) rc = ensure(ctx, ctx->it_cc, std::max<uint64_t>(n_items, 1) * 4);
+  if (rc == SHCI_OK)
+    k_fill_items<<<div_up(n_all, 256), 256, 0, st>>>(G, n_all, P<uint64_t>(goff), P<uint64_t>(ctx->it_grp),
+                                                    P<uint32_t>(ctx->it_cc));
+  release(ctx, goff);
+  CHECK_RC(rc);
+  if (n_items == 0) {
+    ctx->n_tiles = ctx->n_tm = ctx->n_pairs = 0;
+    return SHCI_OK;
+  }
+  // member prefix -> tiles
+  {
+    DevBuf mc;
+    CHECK_RC(ensure(ctx, mc, (n_items + 1) * 8));
+    CHECK_RC(ensure(ctx, ctx->it_moff, (n_items + 1) * 8));
+    HIPC(hipMemsetAsync(mc.p, 0, (n_items + 1) * 8, st));
+    k_item_members<<<div_up(n_items, 256), 256, 0, st>>>(G, P<uint64_t>(ctx->it_grp), P<uint32_t>(ctx->it_cc),
+                                                         n_items, P<uint64_t>(mc));
+    rc = excl_scan(ctx, P<uint64_t>(mc), P<uint64_t>(ctx->it_moff), n_items + 1);
+    release(ctx, mc);
+    CHECK_RC(rc);
+  }
+  uint64_t last_moff = 0, n_tm = 0;
+  HIPC(hipMemcpyAsync(&last_moff, P<uint64_t>(ctx->it_moff) + (n_items - 1), 8, hipMemcpyDeviceToHost, st));
+  HIPC(hipMemcpyAsync(&n_tm, P<uint64_t>(ctx->it_moff) + n_items, 8, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  const uint64_t n_tiles = last_moff / kTileMembers + 1;
+  if (n_tiles >= (1ull << 31)) return fail(ctx, SHCI_ERR_UNSUPPORTED, "too many tiles");
+  ctx->n_tiles = n_tiles;
+  ctx->n_tm = n_tm;
+  CHECK_RC(ensure(ctx, ctx->tile_first, (n_tiles + 1) * 8));
+  k_tile_first<<<div_up(n_items, 256), 256, 0, st>>>(P<uint64_t>(ctx->it_moff), n_items, P<uint64_t>(ctx->tile_first),
+                                                     n_tiles);
+  CHECK_RC(ensure(ctx, ctx->tm, n_tm * 4));
+  k_fill_tm<<<div_up(n_items, 256), 256, 0, st>>>(G, P<uint64_t>(ctx->it_grp), P<uint32_t>(ctx->it_cc),
+                                                  P<uint64_t>(ctx->it_moff), n_items, P<uint32_t>(ctx->tm));
+  HIPC(hipGetLastError());
+  // count pass
+  CHECK_RC(ensure(ctx, ctx->cnt, std::max<uint64_t>(n_tiles, ctx->n) * 4));
+  {
+    TileParams T = tile_params(ctx);
+    k_tile_pairs<false><<<(uint32_t)n_tiles, 256, 0, st>>>(T);
+    HIPC(hipGetLastError());
+  }
+  HIPC(hipEventRecord(ctx->ev[4], st));
+  uint64_t cap = 0;
+  CHECK_RC(offsets_u64(ctx, P<uint32_t>(ctx->cnt), n_tiles, ctx->tile_poff, &cap));
+  cap = std::max<uint64_t>(cap, 1);
+  ctx->n_pairs_alloc = cap;
+  CHECK_RC(ensure(ctx, ctx->tile_pend, n_tiles * 8));
+  CHECK_RC(ensure(ctx, ctx->pidx, cap * 2));
+  if (flags & SHCI_STORE_FP64) {
+    CHECK_RC(ensure(ctx, ctx->pv64, cap * 8));
+  } else {
+    release(ctx, ctx->pv64);
+  }
+  if (flags & SHCI_STORE_FP32) {
+    CHECK_RC(ensure(ctx, ctx->pv32, cap * 4));
+  } else {
+    release(ctx, ctx->pv32);
+  }
+  HIPC(hipEventRecord(ctx->ev[5], st));
+  {
+    TileParams T = tile_params(ctx);
+    k_tile_pairs<true><<<(uint32_t)n_tiles, 256, 0, st>>>(T);
+    HIPC(hipGetLastError());
+  }
+  HIPC(hipEventRecord(ctx->ev[6], st));
+  unsigned int ovf = 0;
+  HIPC(hipMemcpyAsync(&ovf, ctx->flag.p, 4, hipMemcpyDeviceToHost, st));
+  CHECK_RC(ensure(ctx, ctx->scal, 64 * 8));
+  unsigned long long* sum = reinterpret_cast<unsigned long long*>(ctx->scal.p);
+  HIPC(hipMemsetAsync(sum, 0, 8, st));
+  k_sum_ranges<<<std::min<uint32_t>(1024, div_up(n_tiles, 256)), 256, 0, st>>>(
+      P<uint64_t>(ctx->tile_poff), P<uint64_t>(ctx->tile_pend), n_tiles, sum);
+  unsigned long long kept = 0;
+  HIPC(hipMemcpyAsync(&kept, sum, 8, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  if (ovf) return fail(ctx, SHCI_ERR_STATE, "internal error: tile fill found more pairs than the count pass");
+  ctx->n_pairs = kept;
+  return SHCI_OK;
+}
+
+template <typename VT, typename XT, typename AT>
+int launch_tile_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
+  const hipStream_t st = ctx->stream;
+  const VT* val = sizeof(VT) == 4 ? reinterpret_cast<const VT*>(ctx->pv32.p) : reinterpret_cast<const VT*>(ctx->pv64.p);
+  if (!val && ctx->n_pairs) return fail(ctx, SHCI_ERR_STATE, "requested precision not stored");
+  k_diag_apply<XT><<<div_up(ctx->n, 256), 256, 0, st>>>(P<double>(ctx->diag), shift, x, y, ctx->n);
+  if (ctx->n_tiles)
+    k_tile_spmv<VT, XT, AT><<<(uint32_t)ctx->n_tiles, 256, 0, st>>>(
+        P<uint64_t>(ctx->tile_first), P<uint64_t>(ctx->it_moff), P<uint32_t>(ctx->tm), P<uint64_t>(ctx->tile_poff),
+        P<uint64_t>(ctx->tile_pend), P<uint16_t>(ctx->pidx), val, x, y, ctx->n_tiles);
+  HIPC(hipGetLastError());
+  return SHCI_OK;
+}
+
+bool has_fp32(const shci_ctx* ctx) { return ctx->tiled ? (ctx->pv32.p || !ctx->n_pairs) : ctx->v32.p != nullptr; }
+bool has_fp64(const shci_ctx* ctx) { return ctx->tiled ? (ctx->pv64.p || !ctx->n_pairs) : ctx->v64.p != nullptr; }
+
 // Lanes per row for the SpMV: each lane consumes 8 entries per loop trip
 // (two 4-entry steps).  SHCI_SPMV_LANES overrides (8, 16, 32 or 64).
 int choose_lanes(double mean) {
@@ -1420,9 +1997,35 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
   k_diag<<<div_up(n, 256), 256, 0, st>>>(make_ints(ctx), P<uint64_t>(ctx->up_s), P<uint64_t>(ctx->dn_s),
                                          P<double>(ctx->diag), n);
   HIPC(hipEventRecord(ctx->ev[3], st));
-  // Pass 1: count candidates per row.
   CHECK_RC(ensure(ctx, ctx->flag, 256));
   HIPC(hipMemsetAsync(ctx->flag.p, 0, 256, st));
+  ctx->tiled = (flags & SHCI_LAYOUT_TILED) != 0;
+  if (ctx->tiled) {
+    // The tiled layout replaces the CSR: release it.
+    DevBuf* csr[] = {&ctx->row_ptr, &ctx->row_end, &ctx->col, &ctx->v64, &ctx->v32, &ctx->len};
+    for (DevBuf* b : csr) release(ctx, *b);
+    CHECK_RC(build_tiles(ctx, flags));
+    HIPC(hipEventRecord(ctx->ev[7], st));
+    HIPC(hipEventSynchronize(ctx->ev[7]));
+    ctx->nnz = 2 * ctx->n_pairs;
+    ctx->nnz_alloc = 2 * ctx->n_pairs_alloc;
+    for (int k = 0; k < 7; k++) {
+      if (ctx->n_tiles == 0 && k >= 3 && k < 6) {
+        ctx->phase_ms[k] = 0.f;
+        continue;
+      }
+      HIPC(hipEventElapsedTime(&ctx->phase_ms[k], ctx->ev[k], ctx->ev[k + 1]));
+    }
+    HIPC(hipEventElapsedTime(&ctx->phase_ms[7], ctx->ev[0], ctx->ev[7]));
+    ctx->built = true;
+    return SHCI_OK;
+  }
+  {
+    DevBuf* tiles[] = {&ctx->it_grp, &ctx->it_cc, &ctx->it_moff, &ctx->tile_first, &ctx->tm, &ctx->tile_poff,
+                       &ctx->tile_pend, &ctx->pidx, &ctx->pv32, &ctx->pv64};
+    for (DevBuf* b : tiles) release(ctx, *b);
+  }
+  // Pass 1: count candidates per row.
   CHECK_RC(ensure(ctx, ctx->cnt, n * 4));
   CHECK_RC(ensure(ctx, ctx->len, n * 4));
   CHECK_RC(ensure(ctx, ctx->row_ptr, (n + 1) * 8));
@@ -1502,6 +2105,7 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
 // SpMV launch on internal-order device vectors.
 template <typename VT, typename XT, typename AT>
 int launch_spmv(shci_ctx* ctx, const VT* val, double shift, const XT* x, XT* y) {
+  if (ctx->tiled) return launch_tile_spmv<VT, XT, AT>(ctx, shift, x, y);
   const uint64_t n = ctx->n;
   const int L = ctx->lanes;
   const uint32_t grid = div_up(n, 256 / L);
@@ -1802,7 +2406,9 @@ int shci_ctx_destroy(shci_ctx* ctx) {
   DevBuf* bufs[] = {&ctx->h1, &ctx->eri, &ctx->tD, &ctx->tX, &ctx->up_c, &ctx->dn_c, &ctx->up_s, &ctx->dn_s,
                     &ctx->perm, &ctx->iperm, &ctx->up_gid, &ctx->up_gstart, &ctx->dn_gid, &ctx->dn_gstart,
                     &ctx->dnord_idx, &ctx->dnord_up, &ctx->m1a_id, &ctx->m1b_id, &ctx->dm1_vals, &ctx->dm1_rstart,
-                    &ctx->dm1_seg, &ctx->flag, &ctx->diag, &ctx->cnt,
+                    &ctx->dm1_seg, &ctx->flag, &ctx->it_grp, &ctx->it_cc, &ctx->it_moff, &ctx->tile_first,
+                    &ctx->tm, &ctx->tile_poff, &ctx->tile_pend, &ctx->pidx, &ctx->pv32, &ctx->pv64, &ctx->diag,
+                    &ctx->cnt,
                     &ctx->row_ptr, &ctx->row_end, &ctx->col, &ctx->v64, &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb,
                     &ctx->va, &ctx->vb, &ctx->scal, &ctx->part, &ctx->xv, &ctx->yv, &ctx->x32, &ctx->y32,
                     &ctx->dav};
@@ -1904,6 +2510,18 @@ int shci_update_existing_elems(shci_ctx* ctx) {
   if (!ctx) return SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
   HIPC(hipSetDevice(ctx->device));
+  if (ctx->tiled) {
+    k_diag<<<div_up(ctx->n, 256), 256, 0, ctx->stream>>>(make_ints(ctx), P<uint64_t>(ctx->up_s), P<uint64_t>(ctx->dn_s),
+                                                        P<double>(ctx->diag), ctx->n);
+    if (ctx->n_tiles)
+      k_update_tiles<<<(uint32_t)ctx->n_tiles, 256, 0, ctx->stream>>>(
+          make_ints(ctx), P<uint64_t>(ctx->up_s), P<uint64_t>(ctx->dn_s), P<uint64_t>(ctx->tile_first),
+          P<uint64_t>(ctx->it_moff), P<uint32_t>(ctx->tm), P<uint64_t>(ctx->tile_poff), P<uint64_t>(ctx->tile_pend),
+          P<uint16_t>(ctx->pidx), P<double>(ctx->pv64), P<float>(ctx->pv32), ctx->n_tiles);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return SHCI_OK;
+  }
   k_update_values<<<div_up(ctx->n, 4), 256, 0, ctx->stream>>>(
       ctx->n, make_ints(ctx), P<uint64_t>(ctx->up_s), P<uint64_t>(ctx->dn_s), P<int64_t>(ctx->row_ptr),
       P<int64_t>(ctx->row_end), P<uint32_t>(ctx->col), P<double>(ctx->v64), P<float>(ctx->v32), P<double>(ctx->diag));
@@ -1915,7 +2533,7 @@ int shci_update_existing_elems(shci_ctx* ctx) {
 int shci_get_counts(shci_ctx* ctx, uint64_t* n_dets, uint64_t* n_stored_upper, uint64_t* n_elems) {
   if (!ctx) return SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
-  const uint64_t up = ctx->nnz / 2 + ctx->n;
+  const uint64_t up = ctx->nnz / 2 + ctx->n;  // tiled: nnz = 2 * stored pairs
   if (n_dets) *n_dets = ctx->n;
   if (n_stored_upper) *n_stored_upper = up;
   if (n_elems) *n_elems = 2 * up - ctx->n;
@@ -1942,8 +2560,8 @@ int shci_spmv(shci_ctx* ctx, const double* x, double* y, uint32_t flags) {
   const uint32_t g = div_up(n, 256);
   const hipStream_t st = ctx->stream;
   const bool fp32 = flags & SHCI_FP32;
-  if (fp32 && !ctx->v32.p) return fail(ctx, SHCI_ERR_STATE, "FP32 store not built");
-  if (!fp32 && !ctx->v64.p) return fail(ctx, SHCI_ERR_STATE, "FP64 store not built");
+  if (fp32 && !has_fp32(ctx)) return fail(ctx, SHCI_ERR_STATE, "FP32 store not built");
+  if (!fp32 && !has_fp64(ctx)) return fail(ctx, SHCI_ERR_STATE, "FP64 store not built");
   CHECK_RC(ensure(ctx, ctx->xv, n * 8));
   CHECK_RC(ensure(ctx, ctx->yv, n * 8));
   const double* xd = x;
@@ -1979,7 +2597,7 @@ int shci_spmm(shci_ctx* ctx, const double* X, double* Y, uint32_t k, uint32_t fl
   HIPC(hipSetDevice(ctx->device));
   const uint64_t n = ctx->n;
   const hipStream_t st = ctx->stream;
-  if (!(flags & SHCI_FP32) || !ctx->v32.p) {
+  if (!(flags & SHCI_FP32) || !ctx->v32.p || ctx->tiled) {
     // FP64 block product as k SpMVs.
     for (uint32_t v = 0; v < k; v++) CHECK_RC(shci_spmv(ctx, X + (size_t)v * n, Y + (size_t)v * n, flags));
     return SHCI_OK;
@@ -2032,8 +2650,8 @@ int shci_davidson(shci_ctx* ctx, uint32_t n_states, const double* init, double t
   if (!ctx || !init || !evals || n_states == 0 || n_states > 4) return ctx ? fail(ctx, SHCI_ERR_ARG, "bad davidson args") : SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
   const bool mixed = mode == SHCI_DAVIDSON_MIXED;
-  if (mixed && !ctx->v32.p) return fail(ctx, SHCI_ERR_STATE, "mixed Davidson needs the FP32 store");
-  if (!mixed && !ctx->v64.p) return fail(ctx, SHCI_ERR_STATE, "FP64 Davidson needs the FP64 store");
+  if (mixed && !has_fp32(ctx)) return fail(ctx, SHCI_ERR_STATE, "mixed Davidson needs the FP32 store");
+  if (!mixed && !has_fp64(ctx)) return fail(ctx, SHCI_ERR_STATE, "FP64 Davidson needs the FP64 store");
   HIPC(hipSetDevice(ctx->device));
   const uint64_t n = ctx->n;
   const uint32_t g = div_up(n, 256);
@@ -2057,11 +2675,14 @@ int shci_davidson(shci_ctx* ctx, uint32_t n_states, const double* init, double t
   const size_t es = mixed ? 4 : 8;
   // Layout: v[K], hv[K], w[ns], hw[ns], tmp, init64[ns], work64[2].
   const size_t n_vec = 2 * K + 2 * ns + 1;
-  CHECK_RC(ensure(ctx, ctx->dav, n_vec * n * es + (ns + 2) * n * 8));
+  // Every vector starts on a 256-byte boundary (stride rounded up to 64
+  // elements): FP64 global atomics in the tiled SpMV need natural alignment.
+  const size_t nst = (n + 63) & ~(size_t)63;
+  CHECK_RC(ensure(ctx, ctx->dav, n_vec * nst * es + (ns + 2) * nst * 8));
   char* base = static_cast<char*>(ctx->dav.p);
-  double* init_i = reinterpret_cast<double*>(base + n_vec * n * es);
-  double* w64 = init_i + ns * n;
-  double* y64 = w64 + n;
+  double* init_i = reinterpret_cast<double*>(base + n_vec * nst * es);
+  double* w64 = init_i + ns * nst;
+  double* y64 = w64 + nst;
   // init -> internal order (FP64)
   {
     DevBuf hbuf;
@@ -2091,11 +2712,11 @@ int shci_davidson(shci_ctx* ctx, uint32_t n_states, const double* init, double t
     D.traj_cap = traj_cap;
     D.n_traj = n_traj;
     T* vb = reinterpret_cast<T*>(base);
-    for (size_t i = 0; i < K; i++) D.v.push_back(vb + i * n);
-    for (size_t i = 0; i < K; i++) D.hv.push_back(vb + (K + i) * n);
-    for (size_t i = 0; i < ns; i++) D.w.push_back(vb + (2 * K + i) * n);
-    for (size_t i = 0; i < ns; i++) D.hw.push_back(vb + (2 * K + ns + i) * n);
-    D.tmp = vb + (2 * K + 2 * ns) * n;
+    for (size_t i = 0; i < K; i++) D.v.push_back(vb + i * nst);
+    for (size_t i = 0; i < K; i++) D.hv.push_back(vb + (K + i) * nst);
+    for (size_t i = 0; i < ns; i++) D.w.push_back(vb + (2 * K + i) * nst);
+    for (size_t i = 0; i < ns; i++) D.hw.push_back(vb + (2 * K + ns + i) * nst);
+    D.tmp = vb + (2 * K + 2 * ns) * nst;
     D.shift = 0.0;
     if (mixed) {
       // Shift = FP64 Rayleigh quotient of the initial ground-state vector
@@ -2197,7 +2818,16 @@ int shci_rdm(shci_ctx* ctx, uint32_t n_states, const double* coefs, double* rdm1
   R.c = P<double>(ci);
   R.rdm1 = P<double>(r1);
   R.rdm2 = P<double>(r2);
-  k_rdm<<<div_up(n, 4), 256, 0, st>>>(R);
+  if (ctx->tiled) {
+    k_rdm_diag<<<div_up(n, 256), 256, 0, st>>>(R);
+    if (ctx->n_tiles)
+      k_rdm_tiles<<<(uint32_t)ctx->n_tiles, 256, 0, st>>>(R, P<uint64_t>(ctx->tile_first), P<uint64_t>(ctx->it_moff),
+                                                          P<uint32_t>(ctx->tm), P<uint64_t>(ctx->tile_poff),
+                                                          P<uint64_t>(ctx->tile_pend), P<uint16_t>(ctx->pidx),
+                                                          ctx->n_tiles);
+  } else {
+    k_rdm<<<div_up(n, 4), 256, 0, st>>>(R);
+  }
   if (hipGetLastError() != hipSuccess) rc = fail(ctx, SHCI_ERR_HIP, "rdm launch");
   if (rc == SHCI_OK && rdm1 && hipMemcpyAsync(rdm1, r1.p, no * no * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
     rc = fail(ctx, SHCI_ERR_HIP, "rdm1 copy");
@@ -2211,6 +2841,7 @@ int shci_rdm(shci_ctx* ctx, uint32_t n_states, const double* coefs, double* rdm1
 int shci_export_upper(shci_ctx* ctx, int64_t* row_ptr, int64_t* cols, double* vals) {
   if (!ctx || !row_ptr) return SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
+  if (ctx->tiled) return fail(ctx, SHCI_ERR_STATE, "row export needs the CSR layout (build without SHCI_LAYOUT_TILED)");
   HIPC(hipSetDevice(ctx->device));
   const uint64_t n = ctx->n;
   const uint64_t m = ctx->nnz / 2 + n;
@@ -2262,6 +2893,8 @@ int shci_clear(shci_ctx* ctx) {
   DevBuf* bufs[] = {&ctx->up_c, &ctx->dn_c, &ctx->up_s, &ctx->dn_s, &ctx->perm, &ctx->iperm, &ctx->up_gid,
                     &ctx->up_gstart, &ctx->dn_gid, &ctx->dn_gstart, &ctx->dnord_idx, &ctx->dnord_up,
                     &ctx->m1a_id, &ctx->m1b_id, &ctx->dm1_vals, &ctx->dm1_rstart, &ctx->dm1_seg, &ctx->diag,
+                    &ctx->it_grp, &ctx->it_cc, &ctx->it_moff, &ctx->tile_first, &ctx->tm, &ctx->tile_poff,
+                    &ctx->tile_pend, &ctx->pidx, &ctx->pv32, &ctx->pv64,
                     &ctx->cnt, &ctx->row_ptr, &ctx->row_end, &ctx->row_end, &ctx->col, &ctx->v64,
                     &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb, &ctx->va, &ctx->vb, &ctx->xv,
                     &ctx->yv, &ctx->x32, &ctx->y32, &ctx->dav};
@@ -2284,8 +2917,8 @@ int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, doubl
   HIPC(hipSetDevice(ctx->device));
   const uint64_t n = ctx->n;
   const bool fp32 = flags & SHCI_FP32;
-  if (fp32 && !ctx->v32.p) return fail(ctx, SHCI_ERR_STATE, "FP32 store not built");
-  if (!fp32 && !ctx->v64.p) return fail(ctx, SHCI_ERR_STATE, "FP64 store not built");
+  if (fp32 && !has_fp32(ctx)) return fail(ctx, SHCI_ERR_STATE, "FP32 store not built");
+  if (!fp32 && !has_fp64(ctx)) return fail(ctx, SHCI_ERR_STATE, "FP64 store not built");
   CHECK_RC(ensure(ctx, ctx->xv, n * 8));
   CHECK_RC(ensure(ctx, ctx->yv, n * 8));
   const hipStream_t st = ctx->stream;
@@ -2316,9 +2949,33 @@ int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, doubl
   if (bytes) {
     const double F = (double)ctx->nnz;  // off-diagonal full-symmetric entries
     const double vs = fp32 ? 4.0 : 8.0;
-    // values + columns, row start/end (int64 each), FP64 diagonal, x once, y once.
-    *bytes = F * (vs + 4.0) + (double)n * 16.0 + (double)n * 8.0 + 2.0 * (double)n * vs;
+    if (ctx->tiled) {
+      // each pair once: value + 2-byte local index pair; tile member ids;
+      // tile descriptors; FP64 diagonal; x once, y once.
+      *bytes = (double)ctx->n_pairs * (vs + 2.0) + (double)ctx->n_tm * 4.0 + (double)ctx->n_tiles * 32.0 +
+               (double)n * 8.0 + 2.0 * (double)n * vs;
+    } else {
+      // values + columns, row start/end (int64 each), FP64 diagonal, x once, y once.
+      *bytes = F * (vs + 4.0) + (double)n * 16.0 + (double)n * 8.0 + 2.0 * (double)n * vs;
+    }
   }
+  return SHCI_OK;
+}
+
+int shci_layout_stats(shci_ctx* ctx, uint64_t* stats) {
+  if (!ctx || !stats) return SHCI_ERR_ARG;
+  stats[0] = ctx->n;
+  stats[1] = ctx->nnz;
+  stats[2] = ctx->nnz_alloc;
+  stats[3] = ctx->tiled ? 1 : 0;
+  stats[4] = ctx->n_items;
+  stats[5] = ctx->n_tiles;
+  stats[6] = ctx->n_tm;
+  stats[7] = ctx->n_pairs;
+  stats[8] = ctx->n_dm1_runs;
+  stats[9] = ctx->n_ugroups;
+  stats[10] = ctx->n_dgroups;
+  stats[11] = (uint64_t)ctx->lanes;
   return SHCI_OK;
 }
 

@@ -17,14 +17,15 @@ from helpers import variation
 
 
 @pytest.mark.gpu
-def test_examples_C_gpu_iterations(golden_dir):
+@pytest.mark.parametrize("layout", ["csr", "tiled"])
+def test_examples_C_gpu_iterations(golden_dir, layout):
     from shci_amd.hamiltonian import Davidson, Hamiltonian
     from shci_amd.host import HostSystem
 
     gold = json.loads((golden_dir / "examples_C.json").read_text())
     host = HostSystem.from_fcidump(golden_dir / "C_FCIDUMP", 4, 2)
     h1, eri, _ = host.integrals()
-    ham = Hamiltonian(host.n_orbs, h1, eri, host.core, store="fp64")
+    ham = Hamiltonian(host.n_orbs, h1, eri, host.core, store="fp64", layout=layout)
     dav = Davidson(1)
 
     def backend(up, dn, init, tol):
