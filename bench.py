@@ -83,10 +83,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--target-dets", type=int, default=0)
-    ap.add_argument("--cpu-sample-dets", type=int, default=250_000)
+    ap.add_argument("--cpu-sample-dets", type=int, default=500_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-7)
-    ap.add_argument("--layout", choices=["tiled", "csr"], default="tiled")
+    ap.add_argument("--layout", choices=["segments", "csr"], default="segments")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU row sharding is "
@@ -157,17 +157,20 @@ def main():
     fill_avg = float(np.mean(fill_ms))
     spmv_total = spmv32_ms * max(1, n_spmv_est)
     F = nnz_full_off
+    ls = ham.matrix.layout_stats()
     if fill_avg >= spmv_total:
-        if args.layout == "tiled":
-            fill_bytes = (F // 2) * (4 + 2) + n_dets * 16
-            kname = "k_tile_pairs<fill> (Hamiltonian build, FP32 tiles)"
+        if args.layout == "segments":
+            # value slots written, member ids read, one det pair (16 B) per membership
+            fill_bytes = ls["value_slots"] * 4 + ls["memberships"] * (4 + 16)
+            kname = "k_seg_fill_small<S>/k_seg_fill_large (Hamiltonian build, FP32 segments)"
         else:
             fill_bytes = F * (4 + 4) + (n_dets + 1) * 8 + n_dets * 16
             kname = "k_rows<fill> (Hamiltonian build, FP32 CSR)"
         roof = dict(kernel=kname, bound="hbm", achieved=fill_bytes / (fill_avg / 1e3) / 1e9, peak=HBM_PEAK_GBS,
                     unit="GB/s")
     else:
-        kname = "k_tile_spmv<float,float,float>" if args.layout == "tiled" else "k_spmv<fp32>"
+        kname = ("segment SpMV: k_diag_apply + k_seg_spmv_small<S> + k_seg_spmv_large (FP32)"
+                 if args.layout == "segments" else "k_spmv<fp32>")
         roof = dict(kernel=kname, bound="hbm", achieved=spmv32_bytes / (spmv32_ms / 1e3) / 1e9,
                     peak=HBM_PEAK_GBS, unit="GB/s")
     roof["frac"] = roof["achieved"] / roof["peak"]
@@ -211,6 +214,7 @@ def main():
             "nnz_full_offdiag": nnz_full_off,
             "parallelism": "1 GPU",
             "layout": args.layout,
+            "layout_stats": ls,
         },
         "time_to_1uHa_mixed_s": t_mixed_first if t_mixed_first is not None else t_mixed,
         "time_to_1uHa_fp64_s": t_fp64_first if t_fp64_first is not None else t_fp64,

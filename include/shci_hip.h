@@ -45,9 +45,11 @@ enum {
   SHCI_KEEP_ZEROS = 1u << 2,   /* keep |H_ij| < 1e-12 connections (exact RDM adjoint) */
   SHCI_DEVICE_PTRS = 1u << 3,  /* vector/det arguments are device pointers */
   SHCI_FP32 = 1u << 4,         /* SpMV in FP32 storage / FP32 row accumulation */
-  SHCI_LAYOUT_TILED = 1u << 5  /* build: tiled pair-block layout instead of the CSR
-                                  (each connected pair stored once; faster SpMV,
-                                  less memory; no row export) */
+  SHCI_LAYOUT_SEGMENTS = 1u << 5 /* build: segment layout instead of the CSR: dets
+                                    grouped by shared (alpha-1, beta-1), (alpha-2) and
+                                    (beta-2) cores, each connected pair stored once as a
+                                    value slot of its owning segment, no column indices;
+                                    faster SpMV, less memory; no row export */
 };
 
 /* Davidson modes. */
@@ -143,9 +145,10 @@ int shci_get_build_timings(shci_ctx* ctx, float* phases);
 int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, double* bytes);
 
 /* Layout statistics (no reference counterpart; diagnostics): stats[12] =
- * n_dets, stored off-diagonal entries (both directions), allocated slots,
- * tiled?, items, tiles, tile member slots, stored pairs, mixed-double
- * segments, alpha groups, beta groups, SpMV lanes per row. */
+ * n_dets, stored off-diagonal entries (both directions), allocated entries
+ * (both directions), segment layout?, segments, segment memberships,
+ * segments of more than 64 members, value slots, mixed-double runs of the CSR
+ * search, alpha groups, beta groups, CSR SpMV lanes per row. */
 int shci_layout_stats(shci_ctx* ctx, uint64_t* stats);
 
 /* Device memory in use by the context (bytes). */

@@ -16,7 +16,7 @@ SHCI_STORE_FP32 = 1 << 1
 SHCI_KEEP_ZEROS = 1 << 2
 SHCI_DEVICE_PTRS = 1 << 3
 SHCI_FP32 = 1 << 4
-SHCI_LAYOUT_TILED = 1 << 5
+SHCI_LAYOUT_SEGMENTS = 1 << 5
 SHCI_DAVIDSON_FP64 = 0
 SHCI_DAVIDSON_MIXED = 1
 

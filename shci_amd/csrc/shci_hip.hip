@@ -99,11 +99,16 @@ struct shci_ctx {
   DevBuf xv, yv, x32, y32;
   DevBuf dav;  // Davidson vectors
 
-  // Tiled pair-block layout (SHCI_LAYOUT_TILED): every unordered connected
-  // pair stored once inside a tile of <= 255 member dets.
-  bool tiled = false;
-  uint64_t n_groups_all = 0, n_items = 0, n_tiles = 0, n_tm = 0, n_pairs = 0, n_pairs_alloc = 0;
-  DevBuf it_grp, it_cc, it_moff, tile_first, tm, tile_poff, tile_pend, pidx, pv32, pv64;
+  // Segment layout (SHCI_LAYOUT_SEGMENTS): every connected pair stored once,
+  // as a value slot of the segment (shared core) that owns it.
+  bool segl = false;
+  uint32_t ra = 0, rb = 0;              // electrons the A / B cores remove (min(n, 2))
+  uint32_t r2_alpha = 0, r2_beta = 0;   // distinct minus-two strings
+  DevBuf mA_id, mB_id;                  // [group * C(n_e, 2) + pair] -> minus-two string id
+  uint64_t n_seg = 0, n_mem = 0, n_slots = 0, n_pairs = 0;
+  uint64_t seg_cs[10] = {};             // segment ranges of the size classes
+  uint32_t seg_maxm[3] = {};            // largest segment of each one-wavefront class
+  DevBuf seg_desc, seg_type, seg_core, seg_mem, sv32, sv64;
 
   hipEvent_t ev[9] = {};
   float phase_ms[8] = {};
@@ -1134,356 +1139,480 @@ __global__ __launch_bounds__(256) void k_export_fill(uint64_t n, const int64_t* 
 
 
 // ---------------------------------------------------------------------------
-// Tiled pair-block layout.
+// Segment layout (SHCI_LAYOUT_SEGMENTS).
 //
-// Every connected pair lies in exactly one "group": a beta group (same beta,
-// alpha single/double: the reference's same-beta scan, hamiltonian.h:430-442),
-// an alpha group (same alpha: hamiltonian.h:456-468) or a mixed-double segment
-// (dets sharing an (alpha - 1, beta - 1) core: hamiltonian.h:482-532).  Groups
-// are cut into chunks of <= 64 members; an "item" is a pair of chunks
-// (ca <= cb) of one group; consecutive items are packed into tiles of <= 255
-// member dets.  Each tile stores its connected pairs once, as a value plus
-// two 8-bit local member indices.  The SpMV stages a tile's x in LDS,
-// accumulates both directions of every pair into an LDS y with LDS atomics and
-// flushes one global atomic per member.
+// Determinants are gathered into "segments" of dets that share a core:
+//   C  an (alpha - 1, beta - 1) core: members differ by at most one alpha and
+//      one beta excitation (the reference's alpha-single x beta-single search,
+//      hamiltonian.h:482-532);
+//   A  an alpha core two electrons short (one if n_up == 1) and one beta
+//      string: members differ by an alpha single or double (the same-beta
+//      scan, hamiltonian.h:430-442);
+//   B  the same with the spins exchanged (the same-alpha scan,
+//      hamiltonian.h:456-468).
+// Every connected pair of distinct dets is owned by exactly one segment: a C
+// segment owns the pairs that differ in both spins; an A segment owns its
+// alpha doubles (their common part is the core) and the alpha singles whose
+// lowest common orbital lies outside the core (a single shares n_up - 1 such
+// cores); B likewise.  A segment of m members keeps m(m-1)/2 value slots in
+// rotation order: step d = 1..m/2 holds the pairs (j, (j + d) mod m) for
+// j < m, or for j < m/2 in the last step of an even m.  Slots of pairs the
+// segment does not own, and of pairs under the |H| >= 1e-12 cut, hold 0.
+// No column indices are stored: a lane group of S >= m lanes (m <= 64), or
+// one wavefront with the member x staged in LDS (m > 64), walks the steps,
+// applies each value in both directions (the partner's x and the backward
+// value arrive by cross-lane permutes or from LDS) and adds each member's sum
+// to y with one global atomic per member and segment.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kChunk = 64;
-constexpr uint32_t kTileMembers = 128;  // tile id = member prefix / 128 -> <= 255 members per tile
+constexpr uint32_t kSegMaxM = 4096;  // C: (n_orb-n_up+1)(n_orb-n_dn+1) <= 63*63; A, B: C(n_orb-n+2, 2) <= 2016
+constexpr int kSegClasses = 9;       // m > 1024, > 256, > 64 (one wavefront each); lane groups of 64 .. 2
 
-struct GroupSrc {
-  uint64_t n_c, n_a, n_b;                  // number of groups per source (C segments, A, B)
-  const uint32_t* __restrict__ c_start;    // dm1 run starts
-  const uint32_t* __restrict__ c_vals;     // dm1 sorted vals (det * K + slot)
-  uint32_t K;
-  const uint32_t* __restrict__ a_start;    // beta-group starts in beta order
-  const uint32_t* __restrict__ a_idx;      // beta order -> internal det
-  const uint32_t* __restrict__ b_start;    // alpha-group starts (internal order)
+enum : uint32_t { kSegC = 0, kSegA = 1, kSegB = 2 };
+
+struct SegDesc {
+  uint32_t moff;  // first member in the member array
+  uint32_t m;     // members
+  uint64_t voff;  // first value slot
 };
 
-__device__ __forceinline__ void group_range(const GroupSrc& G, uint64_t gg, int& src, uint64_t& g, uint32_t& b,
-                                            uint32_t& m) {
-  if (gg < G.n_c) {
-    src = 0;
-    g = gg;
-    b = G.c_start[g];
-    m = G.c_start[g + 1] - b;
-  } else if (gg < G.n_c + G.n_a) {
-    src = 1;
-    g = gg - G.n_c;
-    b = G.a_start[g];
-    m = G.a_start[g + 1] - b;
-  } else {
-    src = 2;
-    g = gg - G.n_c - G.n_a;
-    b = G.b_start[g];
-    m = G.b_start[g + 1] - b;
+__device__ __forceinline__ uint32_t seg_class(uint32_t m) {
+  return m > 1024 ? 0 : m > 256 ? 1 : m > 64 ? 2 : m > 32 ? 3 : m > 16 ? 4 : m > 8 ? 5 : m > 4 ? 6 : m > 2 ? 7 : 8;
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, kWave);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, kWave);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Does a segment of this type and core own the pair of dets (j, l)?
+// n_rm: electrons the A (or B) core removes from its spin string.
+__device__ __forceinline__ bool seg_owns(uint32_t type, uint64_t core, uint32_t n_rm, uint64_t uj, uint64_t dj,
+                                         uint64_t ul, uint64_t dl) {
+  if (type == kSegC) return uj != ul && dj != dl;
+  const uint64_t sj = type == kSegA ? uj : dj, sl = type == kSegA ? ul : dl;
+  if (__popcll(sj ^ sl) != 2 || n_rm < 2) return true;
+  const uint64_t c = sj & sl;
+  return ((c & (~c + 1ull)) & ~core) != 0;
+}
+
+// String s without the electrons of removal pattern k: the k-th electron
+// (rm == 1) or the k-th electron pair in (lower, upper) lexicographic order
+// (rm == 2), counting electrons from the lowest orbital.
+__device__ __forceinline__ uint64_t drop_electrons(uint64_t s, uint32_t k, uint32_t rm) {
+  if (rm == 1) {
+    uint64_t m = s;
+    for (uint32_t e = 0; e < k; e++) m &= m - 1;
+    return s & ~(m & (~m + 1ull));
   }
+  uint32_t c = 0;
+  for (uint64_t a = s; a; a &= a - 1) {
+    const uint64_t ba = a & (~a + 1ull);
+    for (uint64_t b = a & (a - 1); b; b &= b - 1) {
+      if (c == k) return s & ~ba & ~(b & (~b + 1ull));
+      c++;
+    }
+  }
+  return s;
 }
 
-__device__ __forceinline__ uint32_t group_member(const GroupSrc& G, int src, uint32_t pos) {
-  if (src == 0) return G.c_vals[pos] / G.K;
-  if (src == 1) return G.a_idx[pos];
-  return pos;
-}
-
-__device__ __forceinline__ uint32_t chunk_len_h(uint32_t m, uint32_t c) {
-  const uint32_t lo = c * kChunk;
-  return m - lo < kChunk ? m - lo : kChunk;
-}
-
-__global__ void k_group_items(GroupSrc G, uint64_t n_all, uint32_t* __restrict__ cnt) {
-  const uint64_t gg = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gg >= n_all) return;
-  int src;
-  uint64_t g;
-  uint32_t b, m;
-  group_range(G, gg, src, g, b, m);
-  const uint32_t nc = (m + kChunk - 1) / kChunk;
-  // every item has >= 2 members: a last chunk of one member has no triangle
-  cnt[gg] = m >= 2 ? nc * (nc + 1) / 2 - (chunk_len_h(m, nc - 1) == 1 ? 1u : 0u) : 0;
-}
-
-// items[off .. off + nc(nc+1)/2): (group, ca | cb << 16), ca <= cb, row-major.
-__global__ void k_fill_items(GroupSrc G, uint64_t n_all, const uint64_t* __restrict__ off,
-                             uint64_t* __restrict__ it_grp, uint32_t* __restrict__ it_cc) {
-  const uint64_t gg = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gg >= n_all) return;
-  int src;
-  uint64_t g;
-  uint32_t b, m;
-  group_range(G, gg, src, g, b, m);
-  if (m < 2) return;
-  const uint32_t nc = (m + kChunk - 1) / kChunk;
-  uint64_t k = off[gg];
-  for (uint32_t ca = 0; ca < nc; ca++)
-    for (uint32_t cb = ca; cb < nc; cb++) {
-      if (ca == cb && chunk_len_h(m, ca) < 2) continue;
-      it_grp[k] = gg;
-      it_cc[k] = ca | (cb << 16);
+// Minus-two keys of unique spin strings: key = string without an electron
+// pair, val = g * C(n_e, 2) + pair index (the two-electron analogue of the
+// abm1 map, src/solver/hamiltonian.h:256-305).
+__global__ void k_minus_two(const uint64_t* __restrict__ uniq, uint32_t G, uint32_t n_e, uint64_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const uint64_t s = uniq[g];
+  const uint64_t P = (uint64_t)n_e * (n_e - 1) / 2;
+  uint64_t k = 0;
+  for (uint64_t a = s; a; a &= a - 1) {
+    const uint64_t ba = a & (~a + 1ull);
+    for (uint64_t b = a & (a - 1); b; b &= b - 1) {
+      keys[g * P + k] = s & ~ba & ~(b & (~b + 1ull));
+      vals[g * P + k] = (uint32_t)(g * P + k);
       k++;
     }
-}
-
-__device__ __forceinline__ uint32_t chunk_len(uint32_t m, uint32_t c) {
-  const uint32_t lo = c * kChunk;
-  return m - lo < kChunk ? m - lo : kChunk;
-}
-
-__global__ void k_item_members(GroupSrc G, const uint64_t* __restrict__ it_grp, const uint32_t* __restrict__ it_cc,
-                               uint64_t n_items, uint64_t* __restrict__ mc) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_items) return;
-  int src;
-  uint64_t g;
-  uint32_t b, m;
-  group_range(G, it_grp[i], src, g, b, m);
-  const uint32_t ca = it_cc[i] & 0xffffu, cb = it_cc[i] >> 16;
-  mc[i] = chunk_len(m, ca) + (cb != ca ? chunk_len(m, cb) : 0u);
-}
-
-// Tile boundaries: tile of item i = moff[i] / 128 (consecutive, no gaps).
-__global__ void k_tile_first(const uint64_t* __restrict__ moff, uint64_t n_items, uint64_t* __restrict__ tfirst,
-                             uint64_t n_tiles) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_items) return;
-  const uint64_t t = moff[i] / kTileMembers;
-  if (i == 0 || moff[i - 1] / kTileMembers != t) tfirst[t] = i;
-  if (i == n_items - 1) tfirst[n_tiles] = n_items;
-}
-
-__global__ void k_fill_tm(GroupSrc G, const uint64_t* __restrict__ it_grp, const uint32_t* __restrict__ it_cc,
-                          const uint64_t* __restrict__ moff, uint64_t n_items, uint32_t* __restrict__ tm) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_items) return;
-  int src;
-  uint64_t g;
-  uint32_t b, m;
-  group_range(G, it_grp[i], src, g, b, m);
-  const uint32_t ca = it_cc[i] & 0xffffu, cb = it_cc[i] >> 16;
-  uint64_t k = moff[i];
-  const uint32_t la = chunk_len(m, ca);
-  for (uint32_t t = 0; t < la; t++) tm[k++] = group_member(G, src, b + ca * kChunk + t);
-  if (cb != ca) {
-    const uint32_t lb = chunk_len(m, cb);
-    for (uint32_t t = 0; t < lb; t++) tm[k++] = group_member(G, src, b + cb * kChunk + t);
   }
 }
 
-struct TileParams {
-  GroupSrc G;
-  const uint64_t* __restrict__ it_grp;
-  const uint32_t* __restrict__ it_cc;
-  const uint64_t* __restrict__ moff;
-  const uint64_t* __restrict__ tfirst;
-  const uint32_t* __restrict__ tm;
+struct SegKeyParams {
+  uint64_t n;
+  uint32_t PC, PA, PB;  // keys per det in the C, A, B regions
+  uint32_t nu, nd, ra, rb;
+  const uint32_t* __restrict__ up_gid;
+  const uint32_t* __restrict__ dn_gid;
+  const uint32_t* __restrict__ m1a;  // minus-one ids of unique alpha strings (C cores)
+  const uint32_t* __restrict__ m1b;
+  const uint32_t* __restrict__ mA;   // A core ids of unique alpha strings
+  const uint32_t* __restrict__ mB;
+  const uint64_t* __restrict__ up;
+  const uint64_t* __restrict__ dn;
+  int bC, bA, bB;  // widths of the low key fields
+  int tag_shift;
+};
+
+// Region, det and removal pattern of key index t.
+__device__ __forceinline__ void seg_decode(const SegKeyParams& K, uint64_t t, uint32_t& type, uint64_t& i,
+                                           uint32_t& k) {
+  const uint64_t eC = K.n * K.PC, eA = eC + K.n * K.PA;
+  if (t < eC) {
+    type = kSegC;
+    i = t / K.PC;
+    k = (uint32_t)(t - i * K.PC);
+  } else if (t < eA) {
+    type = kSegA;
+    const uint64_t u = t - eC;
+    i = u / K.PA;
+    k = (uint32_t)(u - i * K.PA);
+  } else {
+    type = kSegB;
+    const uint64_t u = t - eA;
+    i = u / K.PB;
+    k = (uint32_t)(u - i * K.PB);
+  }
+}
+
+__global__ void k_seg_keys(SegKeyParams K, uint64_t total, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  uint32_t type, k;
+  uint64_t i;
+  seg_decode(K, t, type, i, k);
+  uint64_t payload;
+  if (type == kSegC) {
+    const uint32_t k1 = k / K.nd, k2 = k - k1 * K.nd;
+    payload = ((uint64_t)K.m1a[(uint64_t)K.up_gid[i] * K.nu + k1] << K.bC) | K.m1b[(uint64_t)K.dn_gid[i] * K.nd + k2];
+  } else if (type == kSegA) {
+    payload = ((uint64_t)K.mA[(uint64_t)K.up_gid[i] * K.PA + k] << K.bA) | K.dn_gid[i];
+  } else {
+    payload = ((uint64_t)K.mB[(uint64_t)K.dn_gid[i] * K.PB + k] << K.bB) | K.up_gid[i];
+  }
+  keys[t] = ((uint64_t)type << K.tag_shift) | payload;
+  vals[t] = (uint32_t)t;
+}
+
+// Runs of equal keys -> kept segments (m >= 2).
+__global__ void k_seg_keep(const uint32_t* __restrict__ rstart, uint64_t R, uint32_t* __restrict__ keep,
+                           unsigned int* __restrict__ bad) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > R) return;
+  if (r == R) {
+    keep[r] = 0;
+    return;
+  }
+  const uint32_t m = rstart[r + 1] - rstart[r];
+  keep[r] = m >= 2 ? 1u : 0u;
+  if (m > kSegMaxM) atomicOr(bad, 1u);
+}
+
+__global__ void k_seg_compact(const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ keep,
+                              const uint32_t* __restrict__ sidx, uint64_t R, uint32_t* __restrict__ skey,
+                              uint32_t* __restrict__ srun) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R || !keep[r]) return;
+  skey[sidx[r]] = kSegMaxM - (rstart[r + 1] - rstart[r]);
+  srun[sidx[r]] = (uint32_t)r;
+}
+
+// Ordered segment f (largest first): sizes, slot counts, run -> segment map,
+// class histogram.
+__global__ void k_seg_sizes(const uint32_t* __restrict__ srun, const uint32_t* __restrict__ rstart, uint64_t S,
+                            uint64_t* __restrict__ msz, uint64_t* __restrict__ nslot, uint32_t* __restrict__ finv,
+                            unsigned int* __restrict__ hist) {
+  __shared__ unsigned int h[kSegClasses];
+  if (threadIdx.x < kSegClasses) h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < S) {
+    const uint32_t r = srun[f];
+    const uint64_t m = rstart[r + 1] - rstart[r];
+    msz[f] = m;
+    nslot[f] = m * (m - 1) / 2;
+    finv[r] = (uint32_t)f;
+    atomicAdd(&h[seg_class((uint32_t)m)], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kSegClasses && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_seg_desc(SegKeyParams K, const uint32_t* __restrict__ srun, const uint32_t* __restrict__ rstart,
+                           const uint32_t* __restrict__ vals_s, const uint64_t* __restrict__ moff,
+                           const uint64_t* __restrict__ voff, uint64_t S, SegDesc* __restrict__ desc,
+                           uint8_t* __restrict__ type, uint64_t* __restrict__ core) {
+  const uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= S) return;
+  const uint32_t r = srun[f];
+  SegDesc D;
+  D.moff = (uint32_t)moff[f];
+  D.m = rstart[r + 1] - rstart[r];
+  D.voff = voff[f];
+  desc[f] = D;
+  uint32_t ty, k;
+  uint64_t i;
+  seg_decode(K, vals_s[rstart[r]], ty, i, k);
+  type[f] = (uint8_t)ty;
+  core[f] = ty == kSegC ? 0ull : ty == kSegA ? drop_electrons(K.up[i], k, K.ra) : drop_electrons(K.dn[i], k, K.rb);
+}
+
+__global__ void k_seg_members(SegKeyParams K, const uint32_t* __restrict__ vals_s, const uint32_t* __restrict__ incl,
+                              const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ finv,
+                              const SegDesc* __restrict__ desc, uint64_t M, uint32_t* __restrict__ mem) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M) return;
+  const uint32_t r = incl[p] - 1;
+  const uint32_t f = finv[r];
+  if (f == 0xFFFFFFFFu) return;
+  uint32_t ty, k;
+  uint64_t i;
+  seg_decode(K, vals_s[p], ty, i, k);
+  mem[(uint64_t)desc[f].moff + (p - rstart[r])] = (uint32_t)i;
+}
+
+// ---- value fill / update ----
+struct SegFillParams {
+  const SegDesc* __restrict__ desc;
+  const uint32_t* __restrict__ mem;
+  const uint8_t* __restrict__ type;
+  const uint64_t* __restrict__ core;
   const uint64_t* __restrict__ up;
   const uint64_t* __restrict__ dn;
   Ints I;
-  uint32_t* __restrict__ tcnt;          // count pass: candidate pairs per tile
-  const uint64_t* __restrict__ poff;    // fill pass: tile pair offsets (bound)
-  uint64_t* __restrict__ pend;          // fill pass: end of kept pairs
-  uint16_t* __restrict__ pidx;
   double* __restrict__ v64;
   float* __restrict__ v32;
-  unsigned int* __restrict__ overflow;
+  unsigned long long* __restrict__ kept;
+  uint32_t ra, rb;
+  uint32_t s0, s1;
   int keep_zeros;
+  int update;  // recompute the values of the stored pairs only (update_existing_elems)
 };
 
-// Item layout inside a tile, in LDS.
-struct TileItems {
-  uint32_t q[128];    // pair prefix per item (q[ni] = total)
-  uint8_t src[127];
-  uint8_t ma[127];
-  uint8_t mb[127];    // 0 for a same-chunk (triangular) item
-  uint8_t base[127];  // local index of the item's first member
-};
-
-__device__ __forceinline__ void tri_pair(uint32_t m, uint32_t u, uint32_t& j, uint32_t& k) {
-  // row-major upper triangle of an m x m block: start(j) = j*m - j*(j+1)/2
-  const float fm = (float)(2 * m - 1);
-  int jj = (int)((fm - sqrtf(fm * fm - 8.0f * (float)u)) * 0.5f);
-  if (jj < 0) jj = 0;
-  if (jj > (int)m - 2) jj = (int)m - 2;
-  while (jj + 1 <= (int)m - 2 && (uint32_t)((jj + 1) * (int)m - (jj + 1) * (jj + 2) / 2) <= u) jj++;
-  while (jj > 0 && (uint32_t)(jj * (int)m - jj * (jj + 1) / 2) > u) jj--;
-  j = (uint32_t)jj;
-  k = j + 1 + (u - (uint32_t)(jj * (int)m - jj * (jj + 1) / 2));
+// One slot of the pair (j, l): ownership, element, cut.  Returns 1 if kept.
+__device__ __forceinline__ uint32_t seg_slot(const SegFillParams& F, uint32_t ty, uint64_t core, uint64_t slot,
+                                             uint64_t uj, uint64_t dj, uint64_t ul, uint64_t dl) {
+  double v = 0.0;
+  bool keep = false;
+  if (seg_owns(ty, core, ty == kSegA ? F.ra : F.rb, uj, dj, ul, dl)) {
+    if (F.update) {
+      keep = F.keep_zeros || (F.v64 ? F.v64[slot] != 0.0 : F.v32[slot] != 0.0f);
+      if (keep) v = elem_any(F.I, uj, dj, ul, dl);
+    } else {
+      v = elem_any(F.I, uj, dj, ul, dl);
+      keep = F.keep_zeros || fabs(v) >= kEps;
+      if (!keep) v = 0.0;
+    }
+  }
+  if (F.v64) F.v64[slot] = v;
+  if (F.v32) F.v32[slot] = (float)v;
+  return keep ? 1u : 0u;
 }
 
-template <bool FILL>
-__global__ __launch_bounds__(256) void k_tile_pairs(TileParams T) {
-  __shared__ uint64_t su[256], sd[256];
-  __shared__ uint32_t sid[256];
-  __shared__ TileItems it;
-  __shared__ uint32_t wsum[4];
-  __shared__ uint32_t s_ni, s_nm;
-  const uint32_t tid = threadIdx.x;
-  const uint64_t t = blockIdx.x;
-  const uint64_t i0 = T.tfirst[t], i1 = T.tfirst[t + 1];
-  const uint64_t m0 = T.moff[i0];
-  const uint32_t nm = (uint32_t)(T.moff[i1] - m0);
-  const uint32_t ni = (uint32_t)(i1 - i0);
-  for (uint32_t l = tid; l < nm; l += 256) {
-    const uint32_t id = T.tm[m0 + l];
-    sid[l] = id;
-    su[l] = T.up[id];
-    sd[l] = T.dn[id];
+__device__ __forceinline__ void seg_count(const SegFillParams& F, uint32_t kept) {
+  for (int o = 32; o > 0; o >>= 1) kept += __shfl_xor(kept, o, kWave);
+  if (lane_id() == 0 && kept) atomicAdd(F.kept, (unsigned long long)kept);
+}
+
+// Lane groups of S lanes, one segment (m <= S) per group.
+template <int S>
+__global__ __launch_bounds__(256) void k_seg_fill_small(SegFillParams F) {
+  const uint64_t s = (uint64_t)F.s0 + (uint64_t)blockIdx.x * (256 / S) + threadIdx.x / S;
+  const uint32_t j = threadIdx.x & (S - 1);
+  const int gbase = (int)(lane_id() & ~(uint32_t)(S - 1));
+  SegDesc D{0u, 0u, 0ull};
+  uint32_t ty = 0;
+  uint64_t core = 0;
+  if (s < F.s1) {
+    D = F.desc[s];
+    ty = F.type[s];
+    core = F.core[s];
   }
-  if (tid == 0) {
-    uint32_t q = 0;
-    for (uint32_t r = 0; r < ni; r++) {
-      int src;
-      uint64_t g;
-      uint32_t b, m;
-      group_range(T.G, T.it_grp[i0 + r], src, g, b, m);
-      const uint32_t ca = T.it_cc[i0 + r] & 0xffffu, cb = T.it_cc[i0 + r] >> 16;
-      const uint32_t la = chunk_len(m, ca), lb = cb != ca ? chunk_len(m, cb) : 0u;
-      it.q[r] = q;
-      it.src[r] = (uint8_t)src;
-      it.ma[r] = (uint8_t)la;
-      it.mb[r] = (uint8_t)lb;
-      it.base[r] = (uint8_t)(T.moff[i0 + r] - m0);
-      q += lb ? la * lb : la * (la - 1) / 2;
-    }
-    it.q[ni] = q;
-    s_ni = ni;
-    s_nm = nm;
+  const uint32_t m = D.m, mh = m >> 1;
+  const bool valid = j < m;
+  uint64_t uj = 0, dj = 0;
+  if (valid) {
+    const uint32_t det = F.mem[D.moff + j];
+    uj = F.up[det];
+    dj = F.dn[det];
+  }
+  uint32_t kept = 0, p = j;
+  for (uint32_t d = 1; d <= mh; d++) {
+    if (valid) p = p + 1 == m ? 0 : p + 1;
+    const uint64_t ul = shfl64(uj, gbase + (int)p), dl = shfl64(dj, gbase + (int)p);
+    if (valid && (2 * d != m || j < mh))
+      kept += seg_slot(F, ty, core, D.voff + (uint64_t)(d - 1) * m + j, uj, dj, ul, dl);
+  }
+  seg_count(F, kept);
+}
+
+// One wavefront per segment (m > 64), member dets staged in LDS.
+__global__ __launch_bounds__(64) void k_seg_fill_large(SegFillParams F) {
+  extern __shared__ __align__(16) unsigned char seg_lds[];
+  const uint32_t s = F.s0 + blockIdx.x;
+  if (s >= F.s1) return;
+  const SegDesc D = F.desc[s];
+  const uint32_t ty = F.type[s];
+  const uint64_t core = F.core[s];
+  const uint32_t m = D.m, mh = m >> 1, t = threadIdx.x;
+  uint64_t* su = reinterpret_cast<uint64_t*>(seg_lds);
+  uint64_t* sd = su + m;
+  for (uint32_t j = t; j < m; j += kWave) {
+    const uint32_t det = F.mem[D.moff + j];
+    su[j] = F.up[det];
+    sd[j] = F.dn[det];
   }
   __syncthreads();
-  const uint32_t Q = it.q[ni];
-  uint64_t base = FILL ? T.poff[t] : 0;
-  const uint64_t end = FILL ? T.poff[t + 1] : 0;
-  uint32_t cnt = 0;
-  const uint32_t lane = tid & 63, wv = tid >> 6;
-  for (uint32_t q0 = 0; q0 < Q; q0 += 256) {
-    const uint32_t q = q0 + tid;
-    bool hit = false;
-    double v = 0.0;
-    uint32_t lj = 0, lk = 0;
-    if (q < Q) {
-      uint32_t lo = 0, hi = ni;  // largest r with it.q[r] <= q
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (it.q[mid] <= q) lo = mid; else hi = mid;
-      }
-      const uint32_t r = lo, u = q - it.q[r];
-      const uint32_t ma = it.ma[r], mb = it.mb[r];
-      uint32_t j, k;
-      if (mb) {
-        j = u / mb;
-        k = ma + (u - j * mb);
-      } else {
-        tri_pair(ma, u, j, k);
-      }
-      lj = it.base[r] + j;
-      lk = it.base[r] + k;
-      const uint64_t uj = su[lj], uk = su[lk], dj = sd[lj], dk = sd[lk];
-      const int src = it.src[r];
-      if (src == 0) {
-        hit = uj != uk && dj != dk;
-        if (FILL && hit) v = double_mixed(T.I, uj, uk, dj, dk);
-      } else if (src == 1) {
-        const int c = __popcll(uj ^ uk);
-        hit = c == 2 || c == 4;
-        if (FILL && hit) v = c == 2 ? single_elem(T.I, uj, uk, dj) : double_same(T.I, uj, uk);
-      } else {
-        const int c = __popcll(dj ^ dk);
-        hit = c == 2 || c == 4;
-        if (FILL && hit) v = c == 2 ? single_elem(T.I, dj, dk, uj) : double_same(T.I, dj, dk);
-      }
-    }
-    if (!FILL) {
-      cnt += hit ? 1u : 0u;
-    } else {
-      const bool keep = hit && (T.keep_zeros || fabs(v) >= kEps);
-      const uint64_t bm = __ballot(keep);
-      if (lane == 0) wsum[wv] = (uint32_t)__popcll(bm);
-      __syncthreads();
-      uint32_t before = 0, total = 0;
-      for (uint32_t w = 0; w < 4; w++) {
-        if (w < wv) before += wsum[w];
-        total += wsum[w];
-      }
-      if (keep) {
-        const uint64_t pos = base + before + mbcnt(bm);
-        if (pos < end) {
-          T.pidx[pos] = (uint16_t)(lj | (lk << 8));
-          if (T.v64) T.v64[pos] = v;
-          if (T.v32) T.v32[pos] = (float)v;
-        } else {
-          atomicOr(T.overflow, 1u);
-        }
-      }
-      base += total;
-      __syncthreads();
+  uint32_t kept = 0;
+  for (uint32_t d = 1; d <= mh; d++) {
+    const uint32_t lim = 2 * d == m ? mh : m;
+    for (uint32_t j = t; j < lim; j += kWave) {
+      uint32_t p = j + d;
+      if (p >= m) p -= m;
+      kept += seg_slot(F, ty, core, D.voff + (uint64_t)(d - 1) * m + j, su[j], sd[j], su[p], sd[p]);
     }
   }
-  if (!FILL) {
-    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, kWave);
-    if (lane == 0) wsum[wv] = cnt;
-    __syncthreads();
-    if (tid == 0) T.tcnt[t] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-  } else if (tid == 0) {
-    T.pend[t] = base;
-  }
+  seg_count(F, kept);
 }
 
-// y = (d - shift) x  (the diagonal part of the tiled SpMV)
-template <typename XT>
+// ---- SpMV ----
+// y = (d - shift) x for NV interleaved vectors (the diagonal part).
+template <int NV, typename XT>
 __global__ void k_diag_apply(const double* __restrict__ diag, double shift, const XT* __restrict__ x,
                              XT* __restrict__ y, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = (XT)((diag[i] - shift) * (double)x[i]);
+  if (i >= n) return;
+  const double dd = diag[i] - shift;
+  for (int k = 0; k < NV; k++) y[i * NV + k] = (XT)(dd * (double)x[i * NV + k]);
 }
 
-// y += offdiag(H) x over tiles; VT stored value, XT vector type, AT LDS accumulator.
-template <typename VT, typename XT, typename AT>
-__global__ __launch_bounds__(256) void k_tile_spmv(const uint64_t* __restrict__ tfirst,
-                                                   const uint64_t* __restrict__ moff, const uint32_t* __restrict__ tm,
-                                                   const uint64_t* __restrict__ poff, const uint64_t* __restrict__ pend,
-                                                   const uint16_t* __restrict__ pidx, const VT* __restrict__ val,
-                                                   const XT* __restrict__ x, XT* __restrict__ y, uint64_t n_tiles) {
-  __shared__ AT sx[256], sy[256];
-  __shared__ uint32_t sid[256];
-  const uint64_t t = xcd_remap(blockIdx.x, gridDim.x);
-  if (t >= n_tiles) return;
-  const uint32_t tid = threadIdx.x;
-  const uint64_t m0 = moff[tfirst[t]];
-  const uint32_t nm = (uint32_t)(moff[tfirst[t + 1]] - m0);
-  for (uint32_t l = tid; l < nm; l += 256) {
-    const uint32_t id = tm[m0 + l];
-    sid[l] = id;
-    sx[l] = (AT)x[id];
-    sy[l] = (AT)0;
-  }
-  __syncthreads();
-  const uint64_t pe = pend[t];
-  for (uint64_t p = poff[t] + tid; p < pe; p += 256) {
-    const uint32_t ix = __builtin_nontemporal_load(&pidx[p]);
-    const AT a = (AT)__builtin_nontemporal_load(&val[p]);
-    const uint32_t lj = ix & 255u, lk = ix >> 8;
-    atomicAdd(&sy[lj], a * sx[lk]);
-    atomicAdd(&sy[lk], a * sx[lj]);
-  }
-  __syncthreads();
-  for (uint32_t l = tid; l < nm; l += 256) {
-    const AT v = sy[l];
-    if (v != (AT)0) atomicAdd(&y[sid[l]], (XT)v);
+template <int NV, typename XT>
+__device__ __forceinline__ void load_x(const XT* __restrict__ x, uint32_t det, XT (&v)[NV]) {
+  if constexpr (NV == 4 && sizeof(XT) == 4) {
+    const float4 q = *reinterpret_cast<const float4*>(x + (uint64_t)det * 4);
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+  } else {
+    for (int k = 0; k < NV; k++) v[k] = x[(uint64_t)det * NV + k];
   }
 }
 
-// RDM over tiles: each stored pair once, plus every diagonal.
-__global__ __launch_bounds__(256) void k_rdm_tiles(RdmParams R, const uint64_t* __restrict__ tfirst,
-                                                   const uint64_t* __restrict__ moff, const uint32_t* __restrict__ tm,
-                                                   const uint64_t* __restrict__ poff, const uint64_t* __restrict__ pend,
-                                                   const uint16_t* __restrict__ pidx, uint64_t n_tiles) {
-  const uint64_t t = blockIdx.x;
-  if (t >= n_tiles) return;
-  const uint64_t m0 = moff[tfirst[t]];
-  for (uint64_t p = poff[t] + threadIdx.x; p < pend[t]; p += 256) {
-    const uint32_t ix = pidx[p];
-    const uint32_t a = tm[m0 + (ix & 255u)], b = tm[m0 + (ix >> 8)];
-    rdm_pair(R, R.up[a], R.dn[a], a, R.up[b], R.dn[b], b);
+// Off-diagonal part, lane groups of S lanes (one segment of m <= S members
+// per group).  VT stored value type, XT vector (and accumulation) type.
+template <int S, int NV, typename VT, typename XT>
+__global__ __launch_bounds__(256) void k_seg_spmv_small(const SegDesc* __restrict__ desc, uint32_t s0, uint32_t s1,
+                                                        const uint32_t* __restrict__ mem, const VT* __restrict__ val,
+                                                        const XT* __restrict__ x, XT* __restrict__ y) {
+  const uint64_t s = (uint64_t)s0 + (uint64_t)blockIdx.x * (256 / S) + threadIdx.x / S;
+  const uint32_t j = threadIdx.x & (S - 1);
+  const int gbase = (int)(lane_id() & ~(uint32_t)(S - 1));
+  SegDesc D{0u, 0u, 0ull};
+  if (s < s1) D = desc[s];
+  const uint32_t m = D.m, mh = m >> 1;
+  const bool valid = j < m;
+  uint32_t det = 0;
+  XT xj[NV], acc[NV];
+  for (int k = 0; k < NV; k++) {
+    xj[k] = (XT)0;
+    acc[k] = (XT)0;
+  }
+  if (valid) {
+    det = mem[D.moff + j];
+    load_x<NV, XT>(x, det, xj);
+  }
+  const VT* __restrict__ vs = val + D.voff;
+  uint32_t p = j, q = j;  // (j + d) mod m, (j - d) mod m
+  for (uint32_t d = 1; d <= mh; d++) {
+    if (valid) {
+      p = p + 1 == m ? 0 : p + 1;
+      q = q == 0 ? m - 1 : q - 1;
+    }
+    const bool half = 2 * d == m;
+    const uint32_t vj = half && j >= mh ? j - mh : j;
+    const XT v = valid ? (XT)__builtin_nontemporal_load(vs + (uint64_t)(d - 1) * m + vj) : (XT)0;
+    const XT vb = __shfl(v, gbase + (int)q, kWave);  // value of the pair (q, j), held by lane q
+    if (half) {
+      for (int k = 0; k < NV; k++) acc[k] += v * __shfl(xj[k], gbase + (int)p, kWave);
+    } else {
+      for (int k = 0; k < NV; k++) {
+        const XT xp = __shfl(xj[k], gbase + (int)p, kWave);
+        const XT xq = __shfl(xj[k], gbase + (int)q, kWave);
+        acc[k] += v * xp + vb * xq;
+      }
+    }
+  }
+  if (valid)
+    for (int k = 0; k < NV; k++) atomicAdd(&y[(uint64_t)det * NV + k], acc[k]);
+}
+
+// Off-diagonal part, one wavefront per segment (m > 64): member x and the
+// member sums in LDS; every lane updates only its own members (no atomics).
+template <int NV, typename VT, typename XT>
+__global__ __launch_bounds__(64) void k_seg_spmv_large(const SegDesc* __restrict__ desc, uint32_t s0, uint32_t s1,
+                                                       const uint32_t* __restrict__ mem, const VT* __restrict__ val,
+                                                       const XT* __restrict__ x, XT* __restrict__ y) {
+  extern __shared__ __align__(16) unsigned char seg_lds[];
+  const uint32_t s = s0 + blockIdx.x;
+  if (s >= s1) return;
+  const SegDesc D = desc[s];
+  const uint32_t m = D.m, mh = m >> 1, t = threadIdx.x;
+  XT* sx = reinterpret_cast<XT*>(seg_lds);
+  XT* sa = sx + (size_t)m * NV;
+  for (uint32_t j = t; j < m; j += kWave) {
+    XT v[NV];
+    load_x<NV, XT>(x, mem[D.moff + j], v);
+    for (int k = 0; k < NV; k++) {
+      sx[j * NV + k] = v[k];
+      sa[j * NV + k] = (XT)0;
+    }
+  }
+  __syncthreads();
+  const VT* __restrict__ vs = val + D.voff;
+  for (uint32_t d = 1; d <= mh; d++) {
+    const VT* __restrict__ vd = vs + (uint64_t)(d - 1) * m;
+    if (2 * d == m) {
+      for (uint32_t j = t; j < m; j += kWave) {
+        const uint32_t vj = j < mh ? j : j - mh, p = j < mh ? j + mh : j - mh;
+        const XT v = (XT)vd[vj];
+        for (int k = 0; k < NV; k++) sa[j * NV + k] += v * sx[p * NV + k];
+      }
+    } else {
+      for (uint32_t j = t; j < m; j += kWave) {
+        uint32_t p = j + d;
+        if (p >= m) p -= m;
+        const uint32_t q = j >= d ? j - d : j + m - d;
+        const XT vf = (XT)vd[j], vb = (XT)vd[q];
+        for (int k = 0; k < NV; k++) sa[j * NV + k] += vf * sx[p * NV + k] + vb * sx[q * NV + k];
+      }
+    }
+  }
+  for (uint32_t j = t; j < m; j += kWave) {
+    const uint32_t det = mem[D.moff + j];
+    for (int k = 0; k < NV; k++) atomicAdd(&y[(uint64_t)det * NV + k], sa[j * NV + k]);
+  }
+}
+
+// ---- RDM over segments: every stored pair once (one wavefront per segment) ----
+__global__ __launch_bounds__(256) void k_seg_rdm(RdmParams R, const SegDesc* __restrict__ desc,
+                                                 const uint32_t* __restrict__ mem, const uint8_t* __restrict__ type,
+                                                 const uint64_t* __restrict__ core, const double* __restrict__ v64,
+                                                 const float* __restrict__ v32, uint32_t ra, uint32_t rb,
+                                                 int keep_zeros, uint64_t n_seg) {
+  const uint64_t s = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= n_seg) return;
+  const SegDesc D = desc[s];
+  const uint32_t ty = type[s];
+  const uint64_t cs = core[s];
+  const uint64_t slots = (uint64_t)D.m * (D.m - 1) / 2;
+  for (uint64_t k = lane_id(); k < slots; k += kWave) {
+    const uint32_t d = (uint32_t)(k / D.m) + 1, j = (uint32_t)(k % D.m);
+    uint32_t p = j + d;
+    if (p >= D.m) p -= D.m;
+    const uint32_t a = mem[D.moff + j], b = mem[D.moff + p];
+    const uint64_t ua = R.up[a], da = R.dn[a], ub = R.up[b], db = R.dn[b];
+    if (!seg_owns(ty, cs, ty == kSegA ? ra : rb, ua, da, ub, db)) continue;
+    if (!keep_zeros && !(v64 ? v64[D.voff + k] != 0.0 : v32[D.voff + k] != 0.0f)) continue;
+    rdm_pair(R, ua, da, a, ub, db, b);
   }
 }
 
@@ -1492,32 +1621,12 @@ __global__ void k_rdm_diag(RdmParams R) {
   if (i < R.n) rdm_pair(R, R.up[i], R.dn[i], i, R.up[i], R.dn[i], i);
 }
 
-// Recompute stored pair values on the fixed tiled pattern.
-__global__ __launch_bounds__(256) void k_update_tiles(Ints I, const uint64_t* __restrict__ up,
-                                                      const uint64_t* __restrict__ dn,
-                                                      const uint64_t* __restrict__ tfirst,
-                                                      const uint64_t* __restrict__ moff, const uint32_t* __restrict__ tm,
-                                                      const uint64_t* __restrict__ poff,
-                                                      const uint64_t* __restrict__ pend,
-                                                      const uint16_t* __restrict__ pidx, double* __restrict__ v64,
-                                                      float* __restrict__ v32, uint64_t n_tiles) {
-  const uint64_t t = blockIdx.x;
-  if (t >= n_tiles) return;
-  const uint64_t m0 = moff[tfirst[t]];
-  for (uint64_t p = poff[t] + threadIdx.x; p < pend[t]; p += 256) {
-    const uint32_t ix = pidx[p];
-    const uint32_t a = tm[m0 + (ix & 255u)], b = tm[m0 + (ix >> 8)];
-    const double v = elem_any(I, up[a], dn[a], up[b], dn[b]);
-    if (v64) v64[p] = v;
-    if (v32) v32[p] = (float)v;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Host-side helpers
 // ---------------------------------------------------------------------------
-int radix_sort_pairs(shci_ctx* ctx, const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout,
-                     uint64_t n, int end_bit) {
+template <typename KT>
+int radix_sort_pairs(shci_ctx* ctx, const KT* kin, KT* kout, const uint32_t* vin, uint32_t* vout, uint64_t n,
+                     int end_bit) {
   size_t tb = 0;
   HIPC(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kin, kout, vin, vout, (int)n, 0, end_bit, ctx->stream));
   CHECK_RC(ensure(ctx, ctx->tmp, tb));
@@ -1629,12 +1738,13 @@ int build_groups(shci_ctx* ctx) {
   return SHCI_OK;
 }
 
-// Minus-one table of G unique strings (n_e electrons each): id[g*n_e+k] =
-// index of the distinct string "g without its k-th electron"; returns the
-// number of distinct minus-one strings in *R.
-int build_m1(shci_ctx* ctx, const uint64_t* uniq, uint32_t G, uint32_t ne, DevBuf& id, uint32_t* R) {
+// Minus-one (rm = 1) or minus-two (rm = 2) table of G unique strings with
+// n_e electrons each: id[g*P+k] = index of the distinct string "g without its
+// k-th electron (pair)", P = n_e or C(n_e, 2); the number of distinct strings
+// goes to *R.
+int build_m1(shci_ctx* ctx, const uint64_t* uniq, uint32_t G, uint32_t ne, DevBuf& id, uint32_t* R, uint32_t rm = 1) {
   const hipStream_t st = ctx->stream;
-  const uint64_t m = (uint64_t)G * ne;
+  const uint64_t m = (uint64_t)G * (rm == 1 ? ne : (uint64_t)ne * (ne - 1) / 2);
   CHECK_RC(ensure(ctx, id, std::max<uint64_t>(m, 1) * 4));
   if (m == 0) {
     *R = 0;
@@ -1655,7 +1765,10 @@ int build_m1(shci_ctx* ctx, const uint64_t* uniq, uint32_t G, uint32_t ne, DevBu
     cleanup();
     return rc;
   }
-  k_minus_one<<<div_up(G, 256), 256, 0, st>>>(uniq, G, ne, P<uint64_t>(keys), P<uint32_t>(vals));
+  if (rm == 1)
+    k_minus_one<<<div_up(G, 256), 256, 0, st>>>(uniq, G, ne, P<uint64_t>(keys), P<uint32_t>(vals));
+  else
+    k_minus_two<<<div_up(G, 256), 256, 0, st>>>(uniq, G, ne, P<uint64_t>(keys), P<uint32_t>(vals));
   rc = radix_sort_pairs(ctx, P<uint64_t>(keys), P<uint64_t>(keys_s), P<uint32_t>(vals), P<uint32_t>(vals_s), m,
                         (int)ctx->n_orb);
   if (rc == SHCI_OK) {
@@ -1672,9 +1785,10 @@ int build_m1(shci_ctx* ctx, const uint64_t* uniq, uint32_t G, uint32_t ne, DevBu
   return rc;
 }
 
-// Minus-one tables of the unique alpha and beta strings, then the sorted
+// Minus-one tables of the unique alpha and beta strings (and, for the segment
+// layout, the minus-two tables), then for the CSR search the sorted
 // mixed-double segments.
-int build_segments(shci_ctx* ctx) {
+int build_segments(shci_ctx* ctx, bool segl) {
   const hipStream_t st = ctx->stream;
   const uint64_t n = ctx->n;
   const uint32_t G = ctx->n_ugroups, H = ctx->n_dgroups, nu = ctx->n_up, nd = ctx->n_dn;
@@ -1684,6 +1798,7 @@ int build_segments(shci_ctx* ctx) {
     k_unique_strings<<<div_up(G, 256), 256, 0, st>>>(P<uint64_t>(ctx->up_s), P<uint32_t>(ctx->up_gstart), G,
                                                      P<uint64_t>(uniq));
     int rc = build_m1(ctx, P<uint64_t>(uniq), G, nu, ctx->m1a_id, &ctx->r_alpha);
+    if (rc == SHCI_OK && segl && nu >= 2) rc = build_m1(ctx, P<uint64_t>(uniq), G, nu, ctx->mA_id, &ctx->r2_alpha, 2);
     if (rc == SHCI_OK) {
       // beta strings in beta order: dn_s[dnord_idx[dn_gstart[h]]] -- dnord_up holds alpha, so
       // gather the sorted beta keys through the beta-order index.
@@ -1692,9 +1807,17 @@ int build_segments(shci_ctx* ctx) {
       k_unique_strings<<<div_up(H, 256), 256, 0, st>>>(P<uint64_t>(ctx->ka), P<uint32_t>(ctx->dn_gstart), H,
                                                        P<uint64_t>(uniq));
       rc = build_m1(ctx, P<uint64_t>(uniq), H, nd, ctx->m1b_id, &ctx->r_beta);
+      if (rc == SHCI_OK && segl && nd >= 2) rc = build_m1(ctx, P<uint64_t>(uniq), H, nd, ctx->mB_id, &ctx->r2_beta, 2);
     }
     release(ctx, uniq);
     CHECK_RC(rc);
+  }
+  if (segl) {
+    // The segment layout sorts its own keys; the CSR search tables go.
+    DevBuf* csr[] = {&ctx->dm1_vals, &ctx->dm1_rstart, &ctx->dm1_seg};
+    for (DevBuf* b : csr) release(ctx, *b);
+    ctx->n_dm1 = ctx->n_dm1_runs = 0;
+    return SHCI_OK;
   }
   const uint32_t K = nu * nd;
   const uint64_t m = n * K;
@@ -1752,193 +1875,252 @@ int build_segments(shci_ctx* ctx) {
 
 
 // ---------------------------------------------------------------------------
-// Tiled layout build (SHCI_LAYOUT_TILED).
+// Segment layout build (SHCI_LAYOUT_SEGMENTS).
 // ---------------------------------------------------------------------------
-__global__ void k_widen_u32_u64(const uint32_t* __restrict__ a, uint64_t* __restrict__ b, uint64_t n) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = a[i];
-}
-
-__global__ void k_sum_ranges(const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi, uint64_t n,
-                             unsigned long long* __restrict__ out) {
-  __shared__ unsigned long long red[256];
-  unsigned long long v = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
-    v += hi[i] - lo[i];
-  red[threadIdx.x] = v;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
+// Device buffer released when it leaves scope (build temporaries).
+struct TmpBuf {
+  shci_ctx* ctx;
+  DevBuf b;
+  explicit TmpBuf(shci_ctx* c) : ctx(c) {}
+  ~TmpBuf() { release(ctx, b); }
+  TmpBuf(const TmpBuf&) = delete;
+  TmpBuf& operator=(const TmpBuf&) = delete;
+  template <class T>
+  T* p() const {
+    return static_cast<T*>(b.p);
   }
-  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+};
+
+template <typename T>
+int copy_back(shci_ctx* ctx, T* host, const T* dev, size_t count = 1) {
+  HIPC(hipMemcpyAsync(host, dev, count * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return SHCI_OK;
 }
 
-GroupSrc group_src(shci_ctx* ctx) {
-  GroupSrc G;
-  G.n_c = ctx->n_dm1_runs;
-  G.n_a = ctx->n_dgroups;
-  G.n_b = ctx->n_ugroups;
-  G.c_start = P<uint32_t>(ctx->dm1_rstart);
-  G.c_vals = P<uint32_t>(ctx->dm1_vals);
-  G.K = ctx->n_up * ctx->n_dn;
-  G.a_start = P<uint32_t>(ctx->dn_gstart);
-  G.a_idx = P<uint32_t>(ctx->dnord_idx);
-  G.b_start = P<uint32_t>(ctx->up_gstart);
-  return G;
+SegKeyParams seg_key_params(shci_ctx* ctx) {
+  SegKeyParams K;
+  const uint32_t nu = ctx->n_up, nd = ctx->n_dn;
+  K.n = ctx->n;
+  K.nu = nu;
+  K.nd = nd;
+  K.ra = ctx->ra;
+  K.rb = ctx->rb;
+  K.PC = nu * nd;
+  K.PA = nu >= 2 ? nu * (nu - 1) / 2 : nu;
+  K.PB = nd >= 2 ? nd * (nd - 1) / 2 : nd;
+  K.up_gid = P<uint32_t>(ctx->up_gid);
+  K.dn_gid = P<uint32_t>(ctx->dn_gid);
+  K.m1a = P<uint32_t>(ctx->m1a_id);
+  K.m1b = P<uint32_t>(ctx->m1b_id);
+  K.mA = nu >= 2 ? P<uint32_t>(ctx->mA_id) : P<uint32_t>(ctx->m1a_id);
+  K.mB = nd >= 2 ? P<uint32_t>(ctx->mB_id) : P<uint32_t>(ctx->m1b_id);
+  K.up = P<uint64_t>(ctx->up_s);
+  K.dn = P<uint64_t>(ctx->dn_s);
+  const uint32_t rA = nu >= 2 ? ctx->r2_alpha : ctx->r_alpha, rB = nd >= 2 ? ctx->r2_beta : ctx->r_beta;
+  K.bC = bits_for(ctx->r_beta);
+  K.bA = bits_for(ctx->n_dgroups);
+  K.bB = bits_for(ctx->n_ugroups);
+  K.tag_shift = std::max({bits_for(ctx->r_alpha) + K.bC, bits_for(rA) + K.bA, bits_for(rB) + K.bB});
+  return K;
 }
 
-// u32 counts -> u64 exclusive offsets (n + 1 entries); returns the total.
-int offsets_u64(shci_ctx* ctx, const uint32_t* cnt, uint64_t n, DevBuf& off, uint64_t* total) {
+// Segments (sorted by size, largest first), their members and slot offsets.
+int build_seg_layout(shci_ctx* ctx) {
   const hipStream_t st = ctx->stream;
-  DevBuf wide;
-  CHECK_RC(ensure(ctx, wide, (n + 1) * 8));
-  CHECK_RC(ensure(ctx, off, (n + 1) * 8));
-  int rc = SHCI_OK;
-  if (hipMemsetAsync(wide.p, 0, (n + 1) * 8, st) != hipSuccess) rc = fail(ctx, SHCI_ERR_HIP, "memset");
-  if (rc == SHCI_OK && n) k_widen_u32_u64<<<div_up(n, 256), 256, 0, st>>>(cnt, P<uint64_t>(wide), n);
-  if (rc == SHCI_OK) rc = excl_scan(ctx, P<uint64_t>(wide), P<uint64_t>(off), n + 1);
-  if (rc == SHCI_OK && (hipMemcpyAsync(total, P<uint64_t>(off) + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                        hipStreamSynchronize(st) != hipSuccess))
-    rc = fail(ctx, SHCI_ERR_HIP, "offsets");
-  release(ctx, wide);
-  return rc;
-}
-
-TileParams tile_params(shci_ctx* ctx) {
-  TileParams T;
-  T.G = group_src(ctx);
-  T.it_grp = P<uint64_t>(ctx->it_grp);
-  T.it_cc = P<uint32_t>(ctx->it_cc);
-  T.moff = P<uint64_t>(ctx->it_moff);
-  T.tfirst = P<uint64_t>(ctx->tile_first);
-  T.tm = P<uint32_t>(ctx->tm);
-  T.up = P<uint64_t>(ctx->up_s);
-  T.dn = P<uint64_t>(ctx->dn_s);
-  T.I = make_ints(ctx);
-  T.tcnt = P<uint32_t>(ctx->cnt);
-  T.poff = P<uint64_t>(ctx->tile_poff);
-  T.pend = P<uint64_t>(ctx->tile_pend);
-  T.pidx = P<uint16_t>(ctx->pidx);
-  T.v64 = P<double>(ctx->pv64);
-  T.v32 = P<float>(ctx->pv32);
-  T.overflow = reinterpret_cast<unsigned int*>(ctx->flag.p);
-  T.keep_zeros = (ctx->build_flags & SHCI_KEEP_ZEROS) ? 1 : 0;
-  return T;
-}
-
-int build_tiles(shci_ctx* ctx, uint32_t flags) {
-  const hipStream_t st = ctx->stream;
-  const GroupSrc G = group_src(ctx);
-  const uint64_t n_all = G.n_c + G.n_a + G.n_b;
-  ctx->n_groups_all = n_all;
-  // items per group
-  CHECK_RC(ensure(ctx, ctx->cnt, std::max<uint64_t>(n_all, ctx->n) * 4));
-  k_group_items<<<div_up(n_all, 256), 256, 0, st>>>(G, n_all, P<uint32_t>(ctx->cnt));
-  DevBuf goff;
-  uint64_t n_items = 0;
-  CHECK_RC(offsets_u64(ctx, P<uint32_t>(ctx->cnt), n_all, goff, &n_items));
-  ctx->n_items = n_items;
-  int rc = ensure(ctx, ctx->it_grp, std::max<uint64_t>(n_items, 1) * 8);
-  if (rc == SHCI_OK) rc = ensure(ctx, ctx->it_cc, std::max<uint64_t>(n_items, 1) * 4);
-  if (rc == SHCI_OK)
-    k_fill_items<<<div_up(n_all, 256), 256, 0, st>>>(G, n_all, P<uint64_t>(goff), P<uint64_t>(ctx->it_grp),
-                                                    P<uint32_t>(ctx->it_cc));
-  release(ctx, goff);
-  CHECK_RC(rc);
-  if (n_items == 0) {
-    ctx->n_tiles = ctx->n_tm = ctx->n_pairs = 0;
-    return SHCI_OK;
-  }
-  // member prefix -> tiles
-  {
-    DevBuf mc;
-    CHECK_RC(ensure(ctx, mc, (n_items + 1) * 8));
-    CHECK_RC(ensure(ctx, ctx->it_moff, (n_items + 1) * 8));
-    HIPC(hipMemsetAsync(mc.p, 0, (n_items + 1) * 8, st));
-    k_item_members<<<div_up(n_items, 256), 256, 0, st>>>(G, P<uint64_t>(ctx->it_grp), P<uint32_t>(ctx->it_cc),
-                                                         n_items, P<uint64_t>(mc));
-    rc = excl_scan(ctx, P<uint64_t>(mc), P<uint64_t>(ctx->it_moff), n_items + 1);
-    release(ctx, mc);
-    CHECK_RC(rc);
-  }
-  uint64_t last_moff = 0, n_tm = 0;
-  HIPC(hipMemcpyAsync(&last_moff, P<uint64_t>(ctx->it_moff) + (n_items - 1), 8, hipMemcpyDeviceToHost, st));
-  HIPC(hipMemcpyAsync(&n_tm, P<uint64_t>(ctx->it_moff) + n_items, 8, hipMemcpyDeviceToHost, st));
-  HIPC(hipStreamSynchronize(st));
-  const uint64_t n_tiles = last_moff / kTileMembers + 1;
-  if (n_tiles >= (1ull << 31)) return fail(ctx, SHCI_ERR_UNSUPPORTED, "too many tiles");
-  ctx->n_tiles = n_tiles;
-  ctx->n_tm = n_tm;
-  CHECK_RC(ensure(ctx, ctx->tile_first, (n_tiles + 1) * 8));
-  k_tile_first<<<div_up(n_items, 256), 256, 0, st>>>(P<uint64_t>(ctx->it_moff), n_items, P<uint64_t>(ctx->tile_first),
-                                                     n_tiles);
-  CHECK_RC(ensure(ctx, ctx->tm, n_tm * 4));
-  k_fill_tm<<<div_up(n_items, 256), 256, 0, st>>>(G, P<uint64_t>(ctx->it_grp), P<uint32_t>(ctx->it_cc),
-                                                  P<uint64_t>(ctx->it_moff), n_items, P<uint32_t>(ctx->tm));
+  const SegKeyParams K = seg_key_params(ctx);
+  const uint64_t M = K.n * ((uint64_t)K.PC + K.PA + K.PB);
+  ctx->n_seg = ctx->n_mem = ctx->n_slots = 0;
+  for (auto& c : ctx->seg_cs) c = 0;
+  for (auto& c : ctx->seg_maxm) c = 0;
+  if (M == 0) return SHCI_OK;
+  if (M >= (1ull << 31)) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment layout: n_dets * keys per det >= 2^31");
+  if (K.tag_shift + 2 > 64) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment keys exceed 64 bits");
+  // Keys of every (det, core) membership, one radix sort.
+  TmpBuf keys(ctx), keys_s(ctx), vals(ctx), vals_s(ctx);
+  CHECK_RC(ensure(ctx, keys.b, M * 8));
+  CHECK_RC(ensure(ctx, vals.b, M * 4));
+  CHECK_RC(ensure(ctx, keys_s.b, M * 8));
+  CHECK_RC(ensure(ctx, vals_s.b, M * 4));
+  k_seg_keys<<<div_up(M, 256), 256, 0, st>>>(K, M, keys.p<uint64_t>(), vals.p<uint32_t>());
+  CHECK_RC(radix_sort_pairs(ctx, keys.p<uint64_t>(), keys_s.p<uint64_t>(), vals.p<uint32_t>(), vals_s.p<uint32_t>(),
+                            M, K.tag_shift + 2));
+  release(ctx, vals.b);
+  release(ctx, ctx->tmp);
+  // Runs of equal keys: head flags and their inclusive scan share the old key buffer.
+  uint32_t* flag = reinterpret_cast<uint32_t*>(keys.b.p);
+  uint32_t* incl = flag + M;
+  k_head_flags<<<div_up(M, 256), 256, 0, st>>>(keys_s.p<uint64_t>(), flag, M);
   HIPC(hipGetLastError());
-  // count pass
-  CHECK_RC(ensure(ctx, ctx->cnt, std::max<uint64_t>(n_tiles, ctx->n) * 4));
-  {
-    TileParams T = tile_params(ctx);
-    k_tile_pairs<false><<<(uint32_t)n_tiles, 256, 0, st>>>(T);
-    HIPC(hipGetLastError());
-  }
-  HIPC(hipEventRecord(ctx->ev[4], st));
-  uint64_t cap = 0;
-  CHECK_RC(offsets_u64(ctx, P<uint32_t>(ctx->cnt), n_tiles, ctx->tile_poff, &cap));
-  cap = std::max<uint64_t>(cap, 1);
-  ctx->n_pairs_alloc = cap;
-  CHECK_RC(ensure(ctx, ctx->tile_pend, n_tiles * 8));
-  CHECK_RC(ensure(ctx, ctx->pidx, cap * 2));
-  if (flags & SHCI_STORE_FP64) {
-    CHECK_RC(ensure(ctx, ctx->pv64, cap * 8));
-  } else {
-    release(ctx, ctx->pv64);
-  }
-  if (flags & SHCI_STORE_FP32) {
-    CHECK_RC(ensure(ctx, ctx->pv32, cap * 4));
-  } else {
-    release(ctx, ctx->pv32);
-  }
-  HIPC(hipEventRecord(ctx->ev[5], st));
-  {
-    TileParams T = tile_params(ctx);
-    k_tile_pairs<true><<<(uint32_t)n_tiles, 256, 0, st>>>(T);
-    HIPC(hipGetLastError());
-  }
-  HIPC(hipEventRecord(ctx->ev[6], st));
-  unsigned int ovf = 0;
-  HIPC(hipMemcpyAsync(&ovf, ctx->flag.p, 4, hipMemcpyDeviceToHost, st));
+  release(ctx, keys_s.b);
+  CHECK_RC(incl_scan(ctx, flag, incl, M));
+  uint32_t R = 0;
+  CHECK_RC(copy_back(ctx, &R, incl + (M - 1)));
+  TmpBuf rstart(ctx), keep(ctx), sidx(ctx);
+  CHECK_RC(ensure(ctx, rstart.b, ((uint64_t)R + 1) * 4));
+  CHECK_RC(ensure(ctx, keep.b, ((uint64_t)R + 1) * 4));
+  CHECK_RC(ensure(ctx, sidx.b, ((uint64_t)R + 1) * 4));
+  k_run_starts<<<div_up(M, 256), 256, 0, st>>>(flag, incl, rstart.p<uint32_t>(), M);
   CHECK_RC(ensure(ctx, ctx->scal, 64 * 8));
-  unsigned long long* sum = reinterpret_cast<unsigned long long*>(ctx->scal.p);
-  HIPC(hipMemsetAsync(sum, 0, 8, st));
-  k_sum_ranges<<<std::min<uint32_t>(1024, div_up(n_tiles, 256)), 256, 0, st>>>(
-      P<uint64_t>(ctx->tile_poff), P<uint64_t>(ctx->tile_pend), n_tiles, sum);
-  unsigned long long kept = 0;
-  HIPC(hipMemcpyAsync(&kept, sum, 8, hipMemcpyDeviceToHost, st));
-  HIPC(hipStreamSynchronize(st));
-  if (ovf) return fail(ctx, SHCI_ERR_STATE, "internal error: tile fill found more pairs than the count pass");
-  ctx->n_pairs = kept;
+  unsigned int* bad = reinterpret_cast<unsigned int*>(ctx->scal.p);
+  HIPC(hipMemsetAsync(bad, 0, 4, st));
+  k_seg_keep<<<div_up((uint64_t)R + 1, 256), 256, 0, st>>>(rstart.p<uint32_t>(), R, keep.p<uint32_t>(), bad);
+  CHECK_RC(excl_scan(ctx, keep.p<uint32_t>(), sidx.p<uint32_t>(), (uint64_t)R + 1));
+  uint32_t S = 0, hbad = 0;
+  CHECK_RC(copy_back(ctx, &S, sidx.p<uint32_t>() + R));
+  CHECK_RC(copy_back(ctx, &hbad, bad));
+  if (hbad) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment with more than 4096 members");
+  if (S == 0) return SHCI_OK;
+  // Order the segments by size, largest first (stable: key order within a size).
+  TmpBuf skey(ctx), srun(ctx), skey_s(ctx), srun_s(ctx);
+  CHECK_RC(ensure(ctx, skey.b, (uint64_t)S * 4));
+  CHECK_RC(ensure(ctx, srun.b, (uint64_t)S * 4));
+  CHECK_RC(ensure(ctx, skey_s.b, (uint64_t)S * 4));
+  CHECK_RC(ensure(ctx, srun_s.b, (uint64_t)S * 4));
+  k_seg_compact<<<div_up(R, 256), 256, 0, st>>>(rstart.p<uint32_t>(), keep.p<uint32_t>(), sidx.p<uint32_t>(), R,
+                                                skey.p<uint32_t>(), srun.p<uint32_t>());
+  release(ctx, keep.b);
+  release(ctx, sidx.b);
+  CHECK_RC(radix_sort_pairs(ctx, skey.p<uint32_t>(), skey_s.p<uint32_t>(), srun.p<uint32_t>(), srun_s.p<uint32_t>(), S,
+                            13));
+  release(ctx, skey.b);
+  release(ctx, srun.b);
+  release(ctx, skey_s.b);
+  TmpBuf msz(ctx), nslot(ctx), finv(ctx), moff(ctx), voff(ctx);
+  CHECK_RC(ensure(ctx, msz.b, (uint64_t)S * 8));
+  CHECK_RC(ensure(ctx, nslot.b, (uint64_t)S * 8));
+  CHECK_RC(ensure(ctx, finv.b, (uint64_t)R * 4));
+  CHECK_RC(ensure(ctx, moff.b, (uint64_t)S * 8));
+  CHECK_RC(ensure(ctx, voff.b, (uint64_t)S * 8));
+  HIPC(hipMemsetAsync(finv.b.p, 0xFF, (uint64_t)R * 4, st));
+  unsigned int* hist = bad + 4;
+  HIPC(hipMemsetAsync(hist, 0, kSegClasses * 4, st));
+  k_seg_sizes<<<div_up(S, 256), 256, 0, st>>>(srun_s.p<uint32_t>(), rstart.p<uint32_t>(), S, msz.p<uint64_t>(),
+                                             nslot.p<uint64_t>(), finv.p<uint32_t>(), hist);
+  CHECK_RC(excl_scan(ctx, msz.p<uint64_t>(), moff.p<uint64_t>(), S));
+  CHECK_RC(excl_scan(ctx, nslot.p<uint64_t>(), voff.p<uint64_t>(), S));
+  uint64_t last[4];
+  CHECK_RC(copy_back(ctx, &last[0], moff.p<uint64_t>() + (S - 1)));
+  CHECK_RC(copy_back(ctx, &last[1], msz.p<uint64_t>() + (S - 1)));
+  CHECK_RC(copy_back(ctx, &last[2], voff.p<uint64_t>() + (S - 1)));
+  CHECK_RC(copy_back(ctx, &last[3], nslot.p<uint64_t>() + (S - 1)));
+  const uint64_t n_mem = last[0] + last[1], n_slots = last[2] + last[3];
+  if (n_mem >= (1ull << 32)) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment memberships >= 2^32");
+  unsigned int h[kSegClasses];
+  CHECK_RC(copy_back(ctx, h, hist, kSegClasses));
+  CHECK_RC(ensure(ctx, ctx->seg_desc, (uint64_t)S * sizeof(SegDesc)));
+  CHECK_RC(ensure(ctx, ctx->seg_type, (uint64_t)S));
+  CHECK_RC(ensure(ctx, ctx->seg_core, (uint64_t)S * 8));
+  CHECK_RC(ensure(ctx, ctx->seg_mem, n_mem * 4));
+  SegDesc* desc = P<SegDesc>(ctx->seg_desc);
+  k_seg_desc<<<div_up(S, 256), 256, 0, st>>>(K, srun_s.p<uint32_t>(), rstart.p<uint32_t>(), vals_s.p<uint32_t>(),
+                                            moff.p<uint64_t>(), voff.p<uint64_t>(), S, desc,
+                                            P<uint8_t>(ctx->seg_type), P<uint64_t>(ctx->seg_core));
+  k_seg_members<<<div_up(M, 256), 256, 0, st>>>(K, vals_s.p<uint32_t>(), incl, rstart.p<uint32_t>(),
+                                               finv.p<uint32_t>(), desc, M, P<uint32_t>(ctx->seg_mem));
+  HIPC(hipGetLastError());
+  ctx->n_seg = S;
+  ctx->n_mem = n_mem;
+  ctx->n_slots = n_slots;
+  ctx->seg_cs[0] = 0;
+  for (int c = 0; c < kSegClasses; c++) ctx->seg_cs[c + 1] = ctx->seg_cs[c] + h[c];
+  for (int c = 0; c < 3; c++) {
+    if (h[c] == 0) continue;
+    SegDesc d0;
+    CHECK_RC(copy_back(ctx, &d0, desc + ctx->seg_cs[c]));
+    ctx->seg_maxm[c] = d0.m;
+  }
   return SHCI_OK;
 }
 
-template <typename VT, typename XT, typename AT>
-int launch_tile_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
+SegFillParams seg_fill_params(shci_ctx* ctx, int update) {
+  SegFillParams F;
+  F.desc = P<SegDesc>(ctx->seg_desc);
+  F.mem = P<uint32_t>(ctx->seg_mem);
+  F.type = P<uint8_t>(ctx->seg_type);
+  F.core = P<uint64_t>(ctx->seg_core);
+  F.up = P<uint64_t>(ctx->up_s);
+  F.dn = P<uint64_t>(ctx->dn_s);
+  F.I = make_ints(ctx);
+  F.v64 = P<double>(ctx->sv64);
+  F.v32 = P<float>(ctx->sv32);
+  F.kept = reinterpret_cast<unsigned long long*>(ctx->scal.p) + 8;
+  F.ra = ctx->ra;
+  F.rb = ctx->rb;
+  F.s0 = F.s1 = 0;
+  F.keep_zeros = (ctx->build_flags & SHCI_KEEP_ZEROS) ? 1 : 0;
+  F.update = update;
+  return F;
+}
+
+// Value slots of every segment (build) or of the stored pairs (update).
+int seg_fill(shci_ctx* ctx, int update) {
   const hipStream_t st = ctx->stream;
-  const VT* val = sizeof(VT) == 4 ? reinterpret_cast<const VT*>(ctx->pv32.p) : reinterpret_cast<const VT*>(ctx->pv64.p);
-  if (!val && ctx->n_pairs) return fail(ctx, SHCI_ERR_STATE, "requested precision not stored");
-  k_diag_apply<XT><<<div_up(ctx->n, 256), 256, 0, st>>>(P<double>(ctx->diag), shift, x, y, ctx->n);
-  if (ctx->n_tiles)
-    k_tile_spmv<VT, XT, AT><<<(uint32_t)ctx->n_tiles, 256, 0, st>>>(
-        P<uint64_t>(ctx->tile_first), P<uint64_t>(ctx->it_moff), P<uint32_t>(ctx->tm), P<uint64_t>(ctx->tile_poff),
-        P<uint64_t>(ctx->tile_pend), P<uint16_t>(ctx->pidx), val, x, y, ctx->n_tiles);
+  CHECK_RC(ensure(ctx, ctx->scal, 64 * 8));
+  SegFillParams F = seg_fill_params(ctx, update);
+  HIPC(hipMemsetAsync(F.kept, 0, 8, st));
+  for (int c = 0; c < kSegClasses; c++) {
+    F.s0 = (uint32_t)ctx->seg_cs[c];
+    F.s1 = (uint32_t)ctx->seg_cs[c + 1];
+    const uint32_t cnt = F.s1 - F.s0;
+    if (cnt == 0) continue;
+    if (c < 3) {
+      k_seg_fill_large<<<cnt, kWave, ctx->seg_maxm[c] * 16, st>>>(F);
+      continue;
+    }
+    const int S = 64 >> (c - 3);
+    const uint32_t grid = div_up((uint64_t)cnt * S, 256);
+    switch (S) {
+      case 64: k_seg_fill_small<64><<<grid, 256, 0, st>>>(F); break;
+      case 32: k_seg_fill_small<32><<<grid, 256, 0, st>>>(F); break;
+      case 16: k_seg_fill_small<16><<<grid, 256, 0, st>>>(F); break;
+      case 8: k_seg_fill_small<8><<<grid, 256, 0, st>>>(F); break;
+      case 4: k_seg_fill_small<4><<<grid, 256, 0, st>>>(F); break;
+      default: k_seg_fill_small<2><<<grid, 256, 0, st>>>(F); break;
+    }
+  }
+  HIPC(hipGetLastError());
+  unsigned long long kept = 0;
+  CHECK_RC(copy_back(ctx, &kept, F.kept));
+  if (!update) ctx->n_pairs = kept;
+  return SHCI_OK;
+}
+
+// y = (H - shift) x on internal-order device vectors, NV interleaved vectors.
+template <int NV, typename VT, typename XT>
+int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
+  const hipStream_t st = ctx->stream;
+  const VT* val = sizeof(VT) == 4 ? reinterpret_cast<const VT*>(ctx->sv32.p) : reinterpret_cast<const VT*>(ctx->sv64.p);
+  if (!val && ctx->n_slots) return fail(ctx, SHCI_ERR_STATE, "requested precision not stored");
+  k_diag_apply<NV, XT><<<div_up(ctx->n, 256), 256, 0, st>>>(P<double>(ctx->diag), shift, x, y, ctx->n);
+  const SegDesc* desc = P<SegDesc>(ctx->seg_desc);
+  const uint32_t* mem = P<uint32_t>(ctx->seg_mem);
+  for (int c = 0; c < kSegClasses; c++) {
+    const uint32_t s0 = (uint32_t)ctx->seg_cs[c], s1 = (uint32_t)ctx->seg_cs[c + 1];
+    if (s1 == s0) continue;
+    if (c < 3) {
+      const size_t lds = (size_t)ctx->seg_maxm[c] * NV * 2 * sizeof(XT);
+      if (lds > 65536) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment too large for the block product");
+      k_seg_spmv_large<NV, VT, XT><<<s1 - s0, kWave, lds, st>>>(desc, s0, s1, mem, val, x, y);
+      continue;
+    }
+    const int S = 64 >> (c - 3);
+    const uint32_t grid = div_up((uint64_t)(s1 - s0) * S, 256);
+    switch (S) {
+      case 64: k_seg_spmv_small<64, NV, VT, XT><<<grid, 256, 0, st>>>(desc, s0, s1, mem, val, x, y); break;
+      case 32: k_seg_spmv_small<32, NV, VT, XT><<<grid, 256, 0, st>>>(desc, s0, s1, mem, val, x, y); break;
+      case 16: k_seg_spmv_small<16, NV, VT, XT><<<grid, 256, 0, st>>>(desc, s0, s1, mem, val, x, y); break;
+      case 8: k_seg_spmv_small<8, NV, VT, XT><<<grid, 256, 0, st>>>(desc, s0, s1, mem, val, x, y); break;
+      case 4: k_seg_spmv_small<4, NV, VT, XT><<<grid, 256, 0, st>>>(desc, s0, s1, mem, val, x, y); break;
+      default: k_seg_spmv_small<2, NV, VT, XT><<<grid, 256, 0, st>>>(desc, s0, s1, mem, val, x, y); break;
+    }
+  }
   HIPC(hipGetLastError());
   return SHCI_OK;
 }
 
-bool has_fp32(const shci_ctx* ctx) { return ctx->tiled ? (ctx->pv32.p || !ctx->n_pairs) : ctx->v32.p != nullptr; }
-bool has_fp64(const shci_ctx* ctx) { return ctx->tiled ? (ctx->pv64.p || !ctx->n_pairs) : ctx->v64.p != nullptr; }
+bool has_fp32(const shci_ctx* ctx) { return ctx->segl ? (ctx->sv32.p || !ctx->n_slots) : ctx->v32.p != nullptr; }
+bool has_fp64(const shci_ctx* ctx) { return ctx->segl ? (ctx->sv64.p || !ctx->n_slots) : ctx->v64.p != nullptr; }
 
 // Lanes per row for the SpMV: each lane consumes 8 entries per loop trip
 // (two 4-entry steps).  SHCI_SPMV_LANES overrides (8, 16, 32 or 64).
@@ -1991,7 +2173,8 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
   HIPC(hipEventRecord(ctx->ev[0], st));
   CHECK_RC(build_groups(ctx));
   HIPC(hipEventRecord(ctx->ev[1], st));
-  CHECK_RC(build_segments(ctx));
+  const bool segl = (flags & SHCI_LAYOUT_SEGMENTS) != 0;
+  CHECK_RC(build_segments(ctx, segl));
   HIPC(hipEventRecord(ctx->ev[2], st));
   CHECK_RC(ensure(ctx, ctx->diag, n * 8));
   k_diag<<<div_up(n, 256), 256, 0, st>>>(make_ints(ctx), P<uint64_t>(ctx->up_s), P<uint64_t>(ctx->dn_s),
@@ -1999,31 +2182,43 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
   HIPC(hipEventRecord(ctx->ev[3], st));
   CHECK_RC(ensure(ctx, ctx->flag, 256));
   HIPC(hipMemsetAsync(ctx->flag.p, 0, 256, st));
-  ctx->tiled = (flags & SHCI_LAYOUT_TILED) != 0;
-  if (ctx->tiled) {
-    // The tiled layout replaces the CSR: release it.
-    DevBuf* csr[] = {&ctx->row_ptr, &ctx->row_end, &ctx->col, &ctx->v64, &ctx->v32, &ctx->len};
+  ctx->segl = segl;
+  if (segl) {
+    // The segment layout replaces the CSR: release it.
+    DevBuf* csr[] = {&ctx->row_ptr, &ctx->row_end, &ctx->col, &ctx->v64, &ctx->v32, &ctx->len, &ctx->cnt};
     for (DevBuf* b : csr) release(ctx, *b);
-    CHECK_RC(build_tiles(ctx, flags));
+    ctx->ra = std::min(ctx->n_up, 2u);
+    ctx->rb = std::min(ctx->n_dn, 2u);
+    CHECK_RC(build_seg_layout(ctx));
+    HIPC(hipEventRecord(ctx->ev[4], st));
+    if (flags & SHCI_STORE_FP64) {
+      CHECK_RC(ensure(ctx, ctx->sv64, ctx->n_slots * 8));
+    } else {
+      release(ctx, ctx->sv64);
+    }
+    if (flags & SHCI_STORE_FP32) {
+      CHECK_RC(ensure(ctx, ctx->sv32, ctx->n_slots * 4));
+    } else {
+      release(ctx, ctx->sv32);
+    }
+    HIPC(hipEventRecord(ctx->ev[5], st));
+    CHECK_RC(seg_fill(ctx, 0));
+    HIPC(hipEventRecord(ctx->ev[6], st));
     HIPC(hipEventRecord(ctx->ev[7], st));
     HIPC(hipEventSynchronize(ctx->ev[7]));
     ctx->nnz = 2 * ctx->n_pairs;
-    ctx->nnz_alloc = 2 * ctx->n_pairs_alloc;
-    for (int k = 0; k < 7; k++) {
-      if (ctx->n_tiles == 0 && k >= 3 && k < 6) {
-        ctx->phase_ms[k] = 0.f;
-        continue;
-      }
-      HIPC(hipEventElapsedTime(&ctx->phase_ms[k], ctx->ev[k], ctx->ev[k + 1]));
-    }
+    ctx->nnz_alloc = 2 * ctx->n_slots;
+    ctx->n_dropped = 0;
+    for (int k = 0; k < 7; k++) HIPC(hipEventElapsedTime(&ctx->phase_ms[k], ctx->ev[k], ctx->ev[k + 1]));
     HIPC(hipEventElapsedTime(&ctx->phase_ms[7], ctx->ev[0], ctx->ev[7]));
     ctx->built = true;
     return SHCI_OK;
   }
   {
-    DevBuf* tiles[] = {&ctx->it_grp, &ctx->it_cc, &ctx->it_moff, &ctx->tile_first, &ctx->tm, &ctx->tile_poff,
-                       &ctx->tile_pend, &ctx->pidx, &ctx->pv32, &ctx->pv64};
-    for (DevBuf* b : tiles) release(ctx, *b);
+    DevBuf* seg[] = {&ctx->seg_desc, &ctx->seg_type, &ctx->seg_core, &ctx->seg_mem, &ctx->sv32, &ctx->sv64,
+                     &ctx->mA_id, &ctx->mB_id};
+    for (DevBuf* b : seg) release(ctx, *b);
+    ctx->n_seg = ctx->n_mem = ctx->n_slots = ctx->n_pairs = 0;
   }
   // Pass 1: count candidates per row.
   CHECK_RC(ensure(ctx, ctx->cnt, n * 4));
@@ -2105,7 +2300,7 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
 // SpMV launch on internal-order device vectors.
 template <typename VT, typename XT, typename AT>
 int launch_spmv(shci_ctx* ctx, const VT* val, double shift, const XT* x, XT* y) {
-  if (ctx->tiled) return launch_tile_spmv<VT, XT, AT>(ctx, shift, x, y);
+  if (ctx->segl) return launch_seg_spmv<1, VT, XT>(ctx, shift, x, y);
   const uint64_t n = ctx->n;
   const int L = ctx->lanes;
   const uint32_t grid = div_up(n, 256 / L);
@@ -2406,8 +2601,8 @@ int shci_ctx_destroy(shci_ctx* ctx) {
   DevBuf* bufs[] = {&ctx->h1, &ctx->eri, &ctx->tD, &ctx->tX, &ctx->up_c, &ctx->dn_c, &ctx->up_s, &ctx->dn_s,
                     &ctx->perm, &ctx->iperm, &ctx->up_gid, &ctx->up_gstart, &ctx->dn_gid, &ctx->dn_gstart,
                     &ctx->dnord_idx, &ctx->dnord_up, &ctx->m1a_id, &ctx->m1b_id, &ctx->dm1_vals, &ctx->dm1_rstart,
-                    &ctx->dm1_seg, &ctx->flag, &ctx->it_grp, &ctx->it_cc, &ctx->it_moff, &ctx->tile_first,
-                    &ctx->tm, &ctx->tile_poff, &ctx->tile_pend, &ctx->pidx, &ctx->pv32, &ctx->pv64, &ctx->diag,
+                    &ctx->dm1_seg, &ctx->flag, &ctx->mA_id, &ctx->mB_id, &ctx->seg_desc, &ctx->seg_type,
+                    &ctx->seg_core, &ctx->seg_mem, &ctx->sv32, &ctx->sv64, &ctx->diag,
                     &ctx->cnt,
                     &ctx->row_ptr, &ctx->row_end, &ctx->col, &ctx->v64, &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb,
                     &ctx->va, &ctx->vb, &ctx->scal, &ctx->part, &ctx->xv, &ctx->yv, &ctx->x32, &ctx->y32,
@@ -2510,15 +2705,11 @@ int shci_update_existing_elems(shci_ctx* ctx) {
   if (!ctx) return SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
   HIPC(hipSetDevice(ctx->device));
-  if (ctx->tiled) {
+  if (ctx->segl) {
     k_diag<<<div_up(ctx->n, 256), 256, 0, ctx->stream>>>(make_ints(ctx), P<uint64_t>(ctx->up_s), P<uint64_t>(ctx->dn_s),
                                                         P<double>(ctx->diag), ctx->n);
-    if (ctx->n_tiles)
-      k_update_tiles<<<(uint32_t)ctx->n_tiles, 256, 0, ctx->stream>>>(
-          make_ints(ctx), P<uint64_t>(ctx->up_s), P<uint64_t>(ctx->dn_s), P<uint64_t>(ctx->tile_first),
-          P<uint64_t>(ctx->it_moff), P<uint32_t>(ctx->tm), P<uint64_t>(ctx->tile_poff), P<uint64_t>(ctx->tile_pend),
-          P<uint16_t>(ctx->pidx), P<double>(ctx->pv64), P<float>(ctx->pv32), ctx->n_tiles);
     HIPC(hipGetLastError());
+    CHECK_RC(seg_fill(ctx, 1));
     HIPC(hipStreamSynchronize(ctx->stream));
     return SHCI_OK;
   }
@@ -2533,7 +2724,7 @@ int shci_update_existing_elems(shci_ctx* ctx) {
 int shci_get_counts(shci_ctx* ctx, uint64_t* n_dets, uint64_t* n_stored_upper, uint64_t* n_elems) {
   if (!ctx) return SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
-  const uint64_t up = ctx->nnz / 2 + ctx->n;  // tiled: nnz = 2 * stored pairs
+  const uint64_t up = ctx->nnz / 2 + ctx->n;
   if (n_dets) *n_dets = ctx->n;
   if (n_stored_upper) *n_stored_upper = up;
   if (n_elems) *n_elems = 2 * up - ctx->n;
@@ -2597,7 +2788,11 @@ int shci_spmm(shci_ctx* ctx, const double* X, double* Y, uint32_t k, uint32_t fl
   HIPC(hipSetDevice(ctx->device));
   const uint64_t n = ctx->n;
   const hipStream_t st = ctx->stream;
-  if (!(flags & SHCI_FP32) || !ctx->v32.p || ctx->tiled) {
+  // The segment layout's block kernels stage 4 vectors of x and sums per
+  // member of a one-wavefront segment in LDS (64 KiB: m <= 2048).
+  const uint32_t seg_big = std::max({ctx->seg_maxm[0], ctx->seg_maxm[1], ctx->seg_maxm[2]});
+  const bool block_ok = ctx->segl ? (size_t)seg_big * 4 * 2 * sizeof(float) <= 65536 : true;
+  if (!(flags & SHCI_FP32) || !has_fp32(ctx) || !block_ok) {
     // FP64 block product as k SpMVs.
     for (uint32_t v = 0; v < k; v++) CHECK_RC(shci_spmv(ctx, X + (size_t)v * n, Y + (size_t)v * n, flags));
     return SHCI_OK;
@@ -2631,7 +2826,8 @@ int shci_spmm(shci_ctx* ctx, const double* X, double* Y, uint32_t k, uint32_t fl
       return rc;
     }
     k_interleave<<<div_up(n, 256), 256, 0, st>>>(Xs, P<uint32_t>(ctx->perm), P<float>(xi), kk, Kp, n);
-    rc = launch_spmm4(ctx, 0.0, P<float>(xi), P<float>(yi));
+    rc = ctx->segl ? launch_seg_spmv<4, float, float>(ctx, 0.0, P<float>(xi), P<float>(yi))
+                   : launch_spmm4(ctx, 0.0, P<float>(xi), P<float>(yi));
     double* Yo = (flags & SHCI_DEVICE_PTRS) ? Ys : P<double>(Yd);
     if (rc == SHCI_OK) k_deinterleave<<<div_up(n, 256), 256, 0, st>>>(P<float>(yi), P<uint32_t>(ctx->perm), Yo, kk, Kp, n);
     if (rc == SHCI_OK && !(flags & SHCI_DEVICE_PTRS) &&
@@ -2676,7 +2872,7 @@ int shci_davidson(shci_ctx* ctx, uint32_t n_states, const double* init, double t
   // Layout: v[K], hv[K], w[ns], hw[ns], tmp, init64[ns], work64[2].
   const size_t n_vec = 2 * K + 2 * ns + 1;
   // Every vector starts on a 256-byte boundary (stride rounded up to 64
-  // elements): FP64 global atomics in the tiled SpMV need natural alignment.
+  // elements): FP64 global atomics in the segment SpMV need natural alignment.
   const size_t nst = (n + 63) & ~(size_t)63;
   CHECK_RC(ensure(ctx, ctx->dav, n_vec * nst * es + (ns + 2) * nst * 8));
   char* base = static_cast<char*>(ctx->dav.p);
@@ -2818,13 +3014,13 @@ int shci_rdm(shci_ctx* ctx, uint32_t n_states, const double* coefs, double* rdm1
   R.c = P<double>(ci);
   R.rdm1 = P<double>(r1);
   R.rdm2 = P<double>(r2);
-  if (ctx->tiled) {
+  if (ctx->segl) {
     k_rdm_diag<<<div_up(n, 256), 256, 0, st>>>(R);
-    if (ctx->n_tiles)
-      k_rdm_tiles<<<(uint32_t)ctx->n_tiles, 256, 0, st>>>(R, P<uint64_t>(ctx->tile_first), P<uint64_t>(ctx->it_moff),
-                                                          P<uint32_t>(ctx->tm), P<uint64_t>(ctx->tile_poff),
-                                                          P<uint64_t>(ctx->tile_pend), P<uint16_t>(ctx->pidx),
-                                                          ctx->n_tiles);
+    if (ctx->n_seg)
+      k_seg_rdm<<<div_up(ctx->n_seg, 4), 256, 0, st>>>(
+          R, P<SegDesc>(ctx->seg_desc), P<uint32_t>(ctx->seg_mem), P<uint8_t>(ctx->seg_type),
+          P<uint64_t>(ctx->seg_core), P<double>(ctx->sv64), P<float>(ctx->sv32), ctx->ra, ctx->rb,
+          (ctx->build_flags & SHCI_KEEP_ZEROS) ? 1 : 0, ctx->n_seg);
   } else {
     k_rdm<<<div_up(n, 4), 256, 0, st>>>(R);
   }
@@ -2841,7 +3037,7 @@ int shci_rdm(shci_ctx* ctx, uint32_t n_states, const double* coefs, double* rdm1
 int shci_export_upper(shci_ctx* ctx, int64_t* row_ptr, int64_t* cols, double* vals) {
   if (!ctx || !row_ptr) return SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
-  if (ctx->tiled) return fail(ctx, SHCI_ERR_STATE, "row export needs the CSR layout (build without SHCI_LAYOUT_TILED)");
+  if (ctx->segl) return fail(ctx, SHCI_ERR_STATE, "row export needs the CSR layout (build without SHCI_LAYOUT_SEGMENTS)");
   HIPC(hipSetDevice(ctx->device));
   const uint64_t n = ctx->n;
   const uint64_t m = ctx->nnz / 2 + n;
@@ -2893,14 +3089,18 @@ int shci_clear(shci_ctx* ctx) {
   DevBuf* bufs[] = {&ctx->up_c, &ctx->dn_c, &ctx->up_s, &ctx->dn_s, &ctx->perm, &ctx->iperm, &ctx->up_gid,
                     &ctx->up_gstart, &ctx->dn_gid, &ctx->dn_gstart, &ctx->dnord_idx, &ctx->dnord_up,
                     &ctx->m1a_id, &ctx->m1b_id, &ctx->dm1_vals, &ctx->dm1_rstart, &ctx->dm1_seg, &ctx->diag,
-                    &ctx->it_grp, &ctx->it_cc, &ctx->it_moff, &ctx->tile_first, &ctx->tm, &ctx->tile_poff,
-                    &ctx->tile_pend, &ctx->pidx, &ctx->pv32, &ctx->pv64,
+                    &ctx->mA_id, &ctx->mB_id, &ctx->seg_desc, &ctx->seg_type, &ctx->seg_core, &ctx->seg_mem,
+                    &ctx->sv32, &ctx->sv64,
                     &ctx->cnt, &ctx->row_ptr, &ctx->row_end, &ctx->row_end, &ctx->col, &ctx->v64,
                     &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb, &ctx->va, &ctx->vb, &ctx->xv,
                     &ctx->yv, &ctx->x32, &ctx->y32, &ctx->dav};
   for (DevBuf* b : bufs) release(ctx, *b);
   ctx->n = 0;
   ctx->nnz = 0;
+  ctx->n_seg = ctx->n_mem = ctx->n_slots = ctx->n_pairs = 0;
+  for (auto& c : ctx->seg_cs) c = 0;
+  for (auto& c : ctx->seg_maxm) c = 0;
+  ctx->segl = false;
   ctx->built = false;
   return SHCI_OK;
 }
@@ -2949,10 +3149,11 @@ int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, doubl
   if (bytes) {
     const double F = (double)ctx->nnz;  // off-diagonal full-symmetric entries
     const double vs = fp32 ? 4.0 : 8.0;
-    if (ctx->tiled) {
-      // each pair once: value + 2-byte local index pair; tile member ids;
-      // tile descriptors; FP64 diagonal; x once, y once.
-      *bytes = (double)ctx->n_pairs * (vs + 2.0) + (double)ctx->n_tm * 4.0 + (double)ctx->n_tiles * 32.0 +
+    if (ctx->segl) {
+      // every value slot once, member ids, segment descriptors, FP64
+      // diagonal, x once, y once (the per-member x gathers and y atomics
+      // reuse x and y from the caches and are not counted).
+      *bytes = (double)ctx->n_slots * vs + (double)ctx->n_mem * 4.0 + (double)ctx->n_seg * sizeof(SegDesc) +
                (double)n * 8.0 + 2.0 * (double)n * vs;
     } else {
       // values + columns, row start/end (int64 each), FP64 diagonal, x once, y once.
@@ -2967,11 +3168,11 @@ int shci_layout_stats(shci_ctx* ctx, uint64_t* stats) {
   stats[0] = ctx->n;
   stats[1] = ctx->nnz;
   stats[2] = ctx->nnz_alloc;
-  stats[3] = ctx->tiled ? 1 : 0;
-  stats[4] = ctx->n_items;
-  stats[5] = ctx->n_tiles;
-  stats[6] = ctx->n_tm;
-  stats[7] = ctx->n_pairs;
+  stats[3] = ctx->segl ? 1 : 0;
+  stats[4] = ctx->n_seg;
+  stats[5] = ctx->n_mem;
+  stats[6] = ctx->seg_cs[3];  // segments with more than 64 members
+  stats[7] = ctx->segl ? ctx->n_slots : 0;
   stats[8] = ctx->n_dm1_runs;
   stats[9] = ctx->n_ugroups;
   stats[10] = ctx->n_dgroups;

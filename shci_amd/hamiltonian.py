@@ -168,8 +168,8 @@ class SparseMatrix:
     def layout_stats(self):
         st = (C.c_uint64 * 12)()
         self.ctx.check(self.ctx.lib.shci_layout_stats(self.ctx.handle, st), "shci_layout_stats")
-        names = ["n_dets", "nnz", "nnz_alloc", "tiled", "items", "tiles", "tile_members", "pairs",
-                 "mixed_segments", "alpha_groups", "beta_groups", "lanes"]
+        names = ["n_dets", "nnz", "nnz_alloc", "segments_layout", "segments", "memberships", "large_segments",
+                 "value_slots", "mixed_runs", "alpha_groups", "beta_groups", "lanes"]
         return dict(zip(names, list(st)))
 
     def build_timings(self):
@@ -182,9 +182,9 @@ class SparseMatrix:
 class Hamiltonian:
     """``Hamiltonian<ChemSystem>`` over the device.  ``store`` selects the value
     precision(s) kept: ``"fp64"``, ``"fp32"`` or ``"both"``.  ``layout``:
-    ``"csr"`` (full-symmetric CSR; supports row export) or ``"tiled"`` (each
-    connected pair stored once in tiles of <= 255 dets; faster SpMV, about a
-    third of the memory)."""
+    ``"csr"`` (full-symmetric CSR; supports row export) or ``"segments"``
+    (dets grouped by shared cores, each connected pair stored once as a value
+    slot without column index; faster SpMV, less memory)."""
 
     def __init__(self, n_orb, h1, eri, core=0.0, device=0, store="fp64", keep_zeros=False, layout="csr"):
         self.ctx = Context.get(device)
@@ -193,7 +193,7 @@ class Hamiltonian:
         self.n_dets_prev = 0
         self.store = store
         self.keep_zeros = keep_zeros
-        if layout not in ("csr", "tiled"):
+        if layout not in ("csr", "segments"):
             raise ShciError(f"unknown layout {layout!r}")
         self.layout = layout
         self.set_integrals(n_orb, h1, eri, core)
@@ -215,8 +215,8 @@ class Hamiltonian:
              "both": _abi.SHCI_STORE_FP64 | _abi.SHCI_STORE_FP32}[self.store]
         if self.keep_zeros:
             f |= _abi.SHCI_KEEP_ZEROS
-        if self.layout == "tiled":
-            f |= _abi.SHCI_LAYOUT_TILED
+        if self.layout == "segments":
+            f |= _abi.SHCI_LAYOUT_SEGMENTS
         return f
 
     def update(self, up, dn):

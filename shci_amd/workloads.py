@@ -4,6 +4,7 @@ variational determinant list grown by the reference's selection loop
 Davidson for the coefficients."""
 from __future__ import annotations
 
+import sys
 import time
 
 import numpy as np
@@ -78,7 +79,8 @@ def grow_det_list(cfg: int, target: int | None = None, device: int = 0, verbose:
             w.log.append(rec)
             if verbose:
                 print(f"[workload cfg{cfg}] eps {eps:.1e} dets {len(up):,} new {n_new:,} nnz {nel:,} "
-                      f"E {rec['energy']:.8f} sel {t_sel:.2f}s build {t_build:.2f}s dav {t_dav:.2f}s", flush=True)
+                      f"E {rec['energy']:.8f} sel {t_sel:.2f}s build {t_build:.2f}s dav {t_dav:.2f}s", file=sys.stderr,
+                      flush=True)
             if len(up) >= target or n_new < 0.01 * n_before:
                 break
         if host.n_dets >= target:
@@ -88,5 +90,5 @@ def grow_det_list(cfg: int, target: int | None = None, device: int = 0, verbose:
     ham.clear()
     del ham
     if verbose:
-        print(f"[workload cfg{cfg}] {len(w.up):,} dets in {time.time() - t_start:.1f}s", flush=True)
+        print(f"[workload cfg{cfg}] {len(w.up):,} dets in {time.time() - t_start:.1f}s", file=sys.stderr, flush=True)
     return w

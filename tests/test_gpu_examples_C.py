@@ -17,7 +17,7 @@ from helpers import variation
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("layout", ["csr", "tiled"])
+@pytest.mark.parametrize("layout", ["csr", "segments"])
 def test_examples_C_gpu_iterations(golden_dir, layout):
     from shci_amd.hamiltonian import Davidson, Hamiltonian
     from shci_amd.host import HostSystem

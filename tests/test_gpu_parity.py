@@ -34,11 +34,37 @@ def _canon_upper(rp, cols, vals, n):
     return key[order], vals[order]
 
 
+def _full_space(n, n_up, n_dn, h1, eri):
+    """Every determinant (HF first) with oracle Davidson coefficients."""
+    import itertools
+
+    ups = [sum(1 << k for k in c) for c in itertools.combinations(range(n), n_up)]
+    dns = [sum(1 << k for k in c) for c in itertools.combinations(range(n), n_dn)]
+    hf = ((1 << n_up) - 1, (1 << n_dn) - 1)
+    dets = [hf] + [(u, d) for u in ups for d in dns if (u, d) != hf]
+    up = np.array([d[0] for d in dets], dtype=np.uint64)
+    dn = np.array([d[1] for d in dets], dtype=np.uint64)
+    orc = oracle.System.from_arrays(n, n_up, n_dn, h1, eri, 0.0)
+    orc.update(up, dn)
+    init = np.zeros(len(up))
+    init[0] = 1.0
+    vec = init[None]
+    for _ in range(20):
+        _, vec, conv, _, _ = orc.davidson(vec, 1e-10)
+        if conv:
+            break
+    return up, dn, orc, vec
+
+
 SYSTEMS = [
-    # (label, n_orb, n_up, n_dn, seed, eps schedule)
+    # (label, n_orb, n_up, n_dn, seed, eps schedule or None for the full space)
     ("n12_3_3", 12, 3, 3, 7, [1e-3, 1e-4, 1e-5]),
     ("n16_4_3", 16, 4, 3, 11, [1e-3, 3e-4]),
     ("cfg1_n20_5_5", 20, 5, 5, 1001, [5e-4, 2e-4]),
+    # one alpha electron (A cores are minus-one strings), two beta electrons
+    ("n14_1_2", 14, 1, 2, 13, [1e-3, 1e-5]),
+    # full space: mixed cores of up to 81 members (one-wavefront segments)
+    ("fci_n11_2_2", 11, 2, 2, 17, None),
 ]
 
 
@@ -46,7 +72,10 @@ SYSTEMS = [
 def system(request):
     label, n, nu, nd, seed, eps = request.param
     _, h1, eri, _ = si.synthetic(n, seed=seed)
-    up, dn, orc, coefs = _dets_from_selection(n, nu, nd, h1, eri, eps)
+    if eps is None:
+        up, dn, orc, coefs = _full_space(n, nu, nd, h1, eri)
+    else:
+        up, dn, orc, coefs = _dets_from_selection(n, nu, nd, h1, eri, eps)
     return dict(label=label, n=n, nu=nu, nd=nd, h1=h1, eri=eri, up=up, dn=dn, orc=orc, coefs=coefs)
 
 
@@ -71,7 +100,7 @@ def test_pattern_and_values_bit_exact(system):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("layout", ["csr", "tiled"])
+@pytest.mark.parametrize("layout", ["csr", "segments"])
 def test_spmv_fp64_fp32(system, layout):
     from shci_amd.hamiltonian import Hamiltonian
 
@@ -92,7 +121,7 @@ def test_spmv_fp64_fp32(system, layout):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("layout", ["csr", "tiled"])
+@pytest.mark.parametrize("layout", ["csr", "segments"])
 def test_davidson_fp64_and_mixed(system, layout):
     from shci_amd.hamiltonian import Davidson, Hamiltonian
 
@@ -117,7 +146,7 @@ def test_davidson_fp64_and_mixed(system, layout):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("layout", ["csr", "tiled"])
+@pytest.mark.parametrize("layout", ["csr", "segments"])
 def test_rdm_vs_oracle(system, layout):
     from shci_amd.hamiltonian import Hamiltonian
 
@@ -132,25 +161,35 @@ def test_rdm_vs_oracle(system, layout):
 
 
 @pytest.mark.gpu
-def test_tiled_counts_and_update_existing(system):
-    """The tiled layout stores the same pairs (count) and recomputes values on
-    its fixed pattern like a fresh build."""
+@pytest.mark.parametrize("keep_zeros", [False, True])
+def test_segments_counts_and_update_existing(system, keep_zeros):
+    """The segment layout stores the same pairs (count) as the CSR and
+    recomputes values on its fixed pattern like the CSR does (one context
+    holds one matrix: CSR first, then segments)."""
     from shci_amd.hamiltonian import Hamiltonian
 
     s = system
-    ham = Hamiltonian(s["n"], s["h1"], s["eri"], 0.0, store="fp64", layout="tiled")
-    ham.update(s["up"], s["dn"])
-    assert ham.matrix.count_n_elems() == s["orc"].stats()["nnz"]
     _, h1b, erib, _ = si.synthetic(s["n"], seed=999)
-    ham.set_integrals(s["n"], h1b, erib, 0.0)
-    ham.update_existing_elems()
-    orc_b = oracle.System.from_arrays(s["n"], s["nu"], s["nd"], h1b, erib, 0.0)
-    orc_b.update(s["up"], s["dn"])
     x = np.random.default_rng(3).uniform(-1, 1, len(s["up"]))
-    # pattern is fixed (the new integrals may zero or un-zero a few pairs): compare on
-    # the oracle's pattern for the old integrals recomputed with the new ones
-    s["orc"].set_integrals(h1b, erib, 0.0)
-    s["orc"].update_existing_elems()
-    np.testing.assert_allclose(ham.matrix.mul(x), s["orc"].mul(x), rtol=1e-12, atol=1e-11)
-    s["orc"].set_integrals(s["h1"], s["eri"], 0.0)
-    s["orc"].update_existing_elems()
+    ys = {}
+    for layout in ("csr", "segments"):
+        ham = Hamiltonian(s["n"], s["h1"], s["eri"], 0.0, store="both", layout=layout, keep_zeros=keep_zeros)
+        ham.update(s["up"], s["dn"])
+        ys[layout + "_nnz"] = ham.matrix.count_n_elems()
+        ham.set_integrals(s["n"], h1b, erib, 0.0)
+        ham.update_existing_elems()
+        ys[layout] = ham.matrix.mul(x)
+        ys[layout + "32"] = ham.matrix.mul(x, fp32=True)
+        ham.clear()
+    if not keep_zeros:
+        assert ys["csr_nnz"] == s["orc"].stats()["nnz"]
+        # the oracle's pattern for the first integrals, values recomputed with the second
+        s["orc"].set_integrals(h1b, erib, 0.0)
+        s["orc"].update_existing_elems()
+        y_orc = s["orc"].mul(x)
+        s["orc"].set_integrals(s["h1"], s["eri"], 0.0)
+        s["orc"].update_existing_elems()
+        np.testing.assert_allclose(ys["csr"], y_orc, rtol=1e-12, atol=1e-11)
+    assert ys["segments_nnz"] == ys["csr_nnz"]
+    np.testing.assert_allclose(ys["segments"], ys["csr"], rtol=1e-12, atol=1e-11)
+    np.testing.assert_allclose(ys["segments32"], ys["csr"], rtol=0, atol=2e-4 * np.abs(ys["csr"]).max())
