@@ -46,6 +46,17 @@ def first_within(traj_e, traj_t, e_ref, tol=1e-6):
     return None
 
 
+def measured_traffic(layout, n_dets, kernel):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/), when they were taken
+    on this workload; None otherwise."""
+    for f in sorted((ROOT / "profiles").glob("*_traffic_*.json")):
+        rec = json.loads(f.read_text())
+        if rec.get("layout") == layout and rec.get("n_dets") == n_dets and rec.get("kernel", "") in kernel:
+            return rec["fp32"]["traffic_bytes"]
+    return None
+
+
 def cpu_baseline(w, sample_dets, reps_spmv=2):
     """The oracle (C++/OpenMP restatement of the reference) on the first
     ``sample_dets`` dets of the same list: build + Davidson to the same
@@ -87,6 +98,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--layout", choices=["segments", "csr"], default="segments")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with a 2 GiB elementwise copy (torch.mul by 1) whose HBM counters "
+                         "(FETCH_SIZE, WRITE_SIZE) calibrate the traffic of the kernels above")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU row sharding is "
@@ -174,7 +188,7 @@ def main():
         roof = dict(kernel=kname, bound="hbm", achieved=spmv32_bytes / (spmv32_ms / 1e3) / 1e9,
                     peak=HBM_PEAK_GBS, unit="GB/s")
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"] = measured_traffic(args.layout, n_dets, roof["kernel"])
     log(f"[bench] mixed: step {ms_per_step:.1f} ms, build {np.mean(build_ms):.1f} ms (fill {fill_avg:.1f}), "
         f"Davidson {t_mixed:.3f}s, E {e_mixed:.10f} err {abs(e_mixed - e_ref) * 1e6:.3f} uHa; "
         f"SpMV32 {spmv32_ms:.3f} ms = {spmv32_bytes / spmv32_ms / 1e6:.0f} GB/s")
@@ -239,6 +253,15 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(1 << 29, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: 3 x torch.mul over {a.numel() * 4 / 2**30:.0f} GiB read + written")
 
 
 if __name__ == "__main__":

@@ -46,7 +46,11 @@ def grow_det_list(cfg: int, target: int | None = None, device: int = 0, verbose:
     target = target or spec.target_dets
     schedule = schedule or SCHEDULES[cfg]
     host = HostSystem.from_arrays(w.n, spec.n_up, spec.n_dn, w.h1, w.eri, w.core, n_states=spec.n_states)
-    ham = Hamiltonian(w.n, w.h1, w.eri, w.core, device=device, store="fp32")
+    # The CSR path sums rows in a fixed order, so the grown list is
+    # reproducible; the ~1e7-det configs need the segment layout's memory
+    # (its atomic row sums may move a borderline det between runs).
+    layout = "csr" if cfg in (1, 2, 4) else "segments"
+    ham = Hamiltonian(w.n, w.h1, w.eri, w.core, device=device, store="fp32", layout=layout)
     dav = Davidson(spec.n_states)
     coefs = np.zeros((spec.n_states, 1))
     coefs[0, 0] = 1.0
