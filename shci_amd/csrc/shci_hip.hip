@@ -1545,9 +1545,17 @@ __global__ __launch_bounds__(256) void k_seg_spmv_small(const SegDesc* __restric
     for (int k = 0; k < NV; k++) atomicAdd(&y[(uint64_t)det * NV + k], acc[k]);
 }
 
-// Off-diagonal part, one wavefront per segment (m > 64): member x and the
-// member sums in LDS; every lane updates only its own members (no atomics).
-template <int NV, typename VT, typename XT>
+// Off-diagonal part, one wavefront per (segment, group of 64 T members) for
+// m > 64.  The member x sit in LDS; each lane owns the members
+// j0 + lane + 64 c (c < T) of its group and keeps their sums in registers
+// while the steps d run outermost, so the
+// forward slot (j, j + d) and the backward slot (j - d, j) of a step are read
+// from the same row of the segment within a short window (cache hits for the
+// second read).  Every value load is unconditional at an in-bounds (clamped)
+// slot and the unused half-step term is masked, so 2T loads per step are in
+// flight per lane.
+constexpr int kSegChunks = 4;
+template <int T, int NV, typename VT, typename XT>
 __global__ __launch_bounds__(64) void k_seg_spmv_large(const SegDesc* __restrict__ desc, uint32_t s0, uint32_t s1,
                                                        const uint32_t* __restrict__ mem, const VT* __restrict__ val,
                                                        const XT* __restrict__ x, XT* __restrict__ y) {
@@ -1556,39 +1564,53 @@ __global__ __launch_bounds__(64) void k_seg_spmv_large(const SegDesc* __restrict
   if (s >= s1) return;
   const SegDesc D = desc[s];
   const uint32_t m = D.m, mh = m >> 1, t = threadIdx.x;
+  if (blockIdx.y * (kWave * T) >= m) return;
   XT* sx = reinterpret_cast<XT*>(seg_lds);
-  XT* sa = sx + (size_t)m * NV;
   for (uint32_t j = t; j < m; j += kWave) {
     XT v[NV];
     load_x<NV, XT>(x, mem[D.moff + j], v);
-    for (int k = 0; k < NV; k++) {
-      sx[j * NV + k] = v[k];
-      sa[j * NV + k] = (XT)0;
-    }
+    for (int k = 0; k < NV; k++) sx[j * NV + k] = v[k];
   }
   __syncthreads();
   const VT* __restrict__ vs = val + D.voff;
-  for (uint32_t d = 1; d <= mh; d++) {
-    const VT* __restrict__ vd = vs + (uint64_t)(d - 1) * m;
-    if (2 * d == m) {
-      for (uint32_t j = t; j < m; j += kWave) {
-        const uint32_t vj = j < mh ? j : j - mh, p = j < mh ? j + mh : j - mh;
-        const XT v = (XT)vd[vj];
-        for (int k = 0; k < NV; k++) sa[j * NV + k] += v * sx[p * NV + k];
+  const bool even = (m & 1) == 0;
+  {
+    const uint32_t j0 = blockIdx.y * (kWave * T);
+    XT acc[T][NV];
+    uint32_t jj[T], p[T], q[T];
+#pragma unroll
+    for (int c = 0; c < T; c++) {
+      const uint32_t j = j0 + kWave * c + t;
+      jj[c] = p[c] = q[c] = j < m ? j : 0;
+      for (int k = 0; k < NV; k++) acc[c][k] = (XT)0;
+    }
+    const VT* __restrict__ vd = vs;
+    for (uint32_t d = 1; d <= mh; d++, vd += m) {
+      // last step of an even m: mh slots, the pair (i, i + mh) at slot i
+      const bool half = even && d == mh;
+      XT vf[T], vb[T];
+#pragma unroll
+      for (int c = 0; c < T; c++) {
+        p[c] = p[c] + 1 == m ? 0 : p[c] + 1;
+        q[c] = q[c] == 0 ? m - 1 : q[c] - 1;
+        const uint32_t fi = half && jj[c] >= mh ? jj[c] - mh : jj[c];
+        vf[c] = (XT)vd[fi];
+        vb[c] = (XT)vd[half ? 0 : q[c]];
       }
-    } else {
-      for (uint32_t j = t; j < m; j += kWave) {
-        uint32_t p = j + d;
-        if (p >= m) p -= m;
-        const uint32_t q = j >= d ? j - d : j + m - d;
-        const XT vf = (XT)vd[j], vb = (XT)vd[q];
-        for (int k = 0; k < NV; k++) sa[j * NV + k] += vf * sx[p * NV + k] + vb * sx[q * NV + k];
+#pragma unroll
+      for (int c = 0; c < T; c++) {
+        const XT wb = half ? (XT)0 : vb[c];
+        for (int k = 0; k < NV; k++) acc[c][k] += vf[c] * sx[p[c] * NV + k] + wb * sx[q[c] * NV + k];
       }
     }
-  }
-  for (uint32_t j = t; j < m; j += kWave) {
-    const uint32_t det = mem[D.moff + j];
-    for (int k = 0; k < NV; k++) atomicAdd(&y[(uint64_t)det * NV + k], sa[j * NV + k]);
+#pragma unroll
+    for (int c = 0; c < T; c++) {
+      const uint32_t j = j0 + kWave * c + t;
+      if (j < m) {
+        const uint32_t det = mem[D.moff + j];
+        for (int k = 0; k < NV; k++) atomicAdd(&y[(uint64_t)det * NV + k], acc[c][k]);
+      }
+    }
   }
 }
 
@@ -2099,9 +2121,11 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
     const uint32_t s0 = (uint32_t)ctx->seg_cs[c], s1 = (uint32_t)ctx->seg_cs[c + 1];
     if (s1 == s0) continue;
     if (c < 3) {
-      const size_t lds = (size_t)ctx->seg_maxm[c] * NV * 2 * sizeof(XT);
+      const size_t lds = (size_t)ctx->seg_maxm[c] * NV * sizeof(XT);
       if (lds > 65536) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment too large for the block product");
-      k_seg_spmv_large<NV, VT, XT><<<s1 - s0, kWave, lds, st>>>(desc, s0, s1, mem, val, x, y);
+      // one wavefront per (segment, 256 members)
+      const dim3 grid(s1 - s0, div_up(ctx->seg_maxm[c], kWave * kSegChunks));
+      k_seg_spmv_large<kSegChunks, NV, VT, XT><<<grid, kWave, lds, st>>>(desc, s0, s1, mem, val, x, y);
       continue;
     }
     const int S = 64 >> (c - 3);
@@ -2788,10 +2812,10 @@ int shci_spmm(shci_ctx* ctx, const double* X, double* Y, uint32_t k, uint32_t fl
   HIPC(hipSetDevice(ctx->device));
   const uint64_t n = ctx->n;
   const hipStream_t st = ctx->stream;
-  // The segment layout's block kernels stage 4 vectors of x and sums per
-  // member of a one-wavefront segment in LDS (64 KiB: m <= 2048).
+  // The segment layout's block kernels stage 4 vectors of x per member of a
+  // one-wavefront segment in LDS (64 KiB: m <= 4096).
   const uint32_t seg_big = std::max({ctx->seg_maxm[0], ctx->seg_maxm[1], ctx->seg_maxm[2]});
-  const bool block_ok = ctx->segl ? (size_t)seg_big * 4 * 2 * sizeof(float) <= 65536 : true;
+  const bool block_ok = ctx->segl ? (size_t)seg_big * 4 * sizeof(float) <= 65536 : true;
   if (!(flags & SHCI_FP32) || !has_fp32(ctx) || !block_ok) {
     // FP64 block product as k SpMVs.
     for (uint32_t v = 0; v < k; v++) CHECK_RC(shci_spmv(ctx, X + (size_t)v * n, Y + (size_t)v * n, flags));
