@@ -87,6 +87,57 @@ def cpu_baseline(w, sample_dets, reps_spmv=2):
                 energy=float(ev[0]), cores=cores)
 
 
+def autograd_bench(w, args, n_dirs=20, step=1e-5, seed=2024):
+    """E(h, eri) on the grown det list: forward (build with every excitation
+    <= 2 pair kept + Davidson to a residual <= 1e-9), backward (GPU RDM
+    contraction), and central finite differences along n_dirs seeded random
+    symmetric unit directions in (h, eri)."""
+    import torch
+
+    from shci_amd.autograd import VariationalSolver, variational_energy
+
+    solver = VariationalSolver(w.n, w.up, w.dn, w.core, n_states=w.spec.n_states, layout=args.layout)
+    h = torch.tensor(w.h1, requires_grad=True)
+    v = torch.tensor(w.eri, requires_grad=True)
+    t0 = time.time()
+    e = variational_energy(h, v, solver)
+    t_fwd = time.time() - t0
+    t0 = time.time()
+    e.backward()
+    t_bwd = time.time() - t0
+    gh, gv = h.grad.numpy(), v.grad.numpy()
+    log(f"[autograd] {len(w.up):,} dets: E {float(e):.10f}, forward {t_fwd:.2f}s (residual {solver.residual:.1e}), "
+        f"backward {t_bwd:.2f}s")
+    coefs0 = solver.coefs.copy()
+    rng = np.random.default_rng(seed)
+    worst, t_fd = 0.0, time.time()
+    for k in range(n_dirs):
+        dh = rng.normal(size=w.h1.shape)
+        dh = 0.5 * (dh + dh.T)
+        dv = rng.normal(size=w.eri.shape)
+        nrm = np.sqrt(np.sum(dh * dh) + dv @ dv)
+        dh, dv = dh / nrm, dv / nrm
+        analytic = float(np.sum(gh * dh) + gv @ dv)
+        vals = []
+        for sign in (1, -1):
+            solver.coefs = coefs0.copy()
+            vals.append(float(solver.solve(w.h1 + sign * step * dh, w.eri + sign * step * dv).mean()))
+        fd = (vals[0] - vals[1]) / (2 * step)
+        rel = abs(fd - analytic) / max(1e-2, abs(analytic))
+        worst = max(worst, rel)
+        log(f"[autograd] direction {k}: analytic {analytic:.10f} fd {fd:.10f} rel {rel:.2e}")
+    out = {
+        "metric": "E(h, eri) RDM-adjoint gradient vs central finite differences",
+        "value": worst, "unit": "max relative error", "higher_is_better": False, "n_gpus": 1,
+        "config": {"workload": f"cfg{args.config}: synthetic n_orb={w.n}, n_up=n_dn={w.spec.n_up}, "
+                               f"{len(w.up)} dets, {args.layout} layout, {n_dirs} seeded directions, step {step:g}",
+                   "n_dets": len(w.up)},
+        "forward_s": t_fwd, "backward_s": t_bwd, "fd_s": time.time() - t_fd, "energy": float(e),
+        "residual": solver.residual, "tolerance": 1e-6,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +152,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with a 2 GiB elementwise copy (torch.mul by 1) whose HBM counters "
                          "(FETCH_SIZE, WRITE_SIZE) calibrate the traffic of the kernels above")
+    ap.add_argument("--autograd", action="store_true",
+                    help="time E(h, eri) forward/backward (RDM adjoint) and check 20 finite-difference "
+                         "directions on the --config workload (use --config 4), one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU row sharding is "
@@ -111,9 +165,16 @@ def main():
     from shci_amd.hamiltonian import Davidson, Hamiltonian
 
     w = workloads.grow_det_list(args.config, target=args.target_dets or None)
+    if args.autograd:
+        autograd_bench(w, args)
+        return
     n_dets = len(w.up)
-    init = np.zeros(n_dets)
-    init[0] = 1.0  # the HF determinant is det 0
+    ns = w.spec.n_states
+    # unit vectors on the first n_states dets; det 0 is the HF determinant
+    # (the reference's start for new roots, src/solver/solver.h:479-485)
+    init = np.zeros((ns, n_dets))
+    for s in range(ns):
+        init[s, s] = 1.0
 
     # ---------------- FP64 path ----------------
     ham = Hamiltonian(w.n, w.h1, w.eri, w.core, store="fp64", layout=args.layout)
@@ -126,15 +187,17 @@ def main():
     nnz_full_off = 2 * (stored_upper - n_dets)
     log(f"[bench] {n_dets:,} dets, stored upper {stored_upper:,}, full off-diag {nnz_full_off:,}; "
         f"FP64 build {bt64['total']:.1f} ms (upload+build {t_upload_build64:.2f}s)")
-    d = Davidson(1)
+    d = Davidson(ns)
     t0 = time.time()
     d.diagonalize(ham.matrix, init, 1e-10, mode="fp64", max_calls=60)
-    e_ref = float(d.lowest_eigenvalues[0])
+    e_ref_all = np.array(d.lowest_eigenvalues, dtype=float)
+    e_ref = float(e_ref_all[0])
     log(f"[bench] E_ref (FP64, |dE| <= 1e-10) = {e_ref:.10f} in {time.time() - t0:.2f}s, converged={d.converged}")
     t0 = time.time()
     d.diagonalize(ham.matrix, init, args.tol, mode="fp64", max_calls=60)
     t_fp64 = time.time() - t0
     e_fp64 = float(d.lowest_eigenvalues[0])
+    err64_roots = np.abs(np.array(d.lowest_eigenvalues) - e_ref_all)
     t_fp64_first = first_within(d.trajectory, d.trajectory_t, e_ref)
     spmv64_ms, spmv64_bytes = ham.matrix.time_spmv(fp32=False, reps=10)
     log(f"[bench] FP64 Davidson {t_fp64:.3f}s E {e_fp64:.10f} err {abs(e_fp64 - e_ref) * 1e6:.3f} uHa; "
@@ -145,7 +208,7 @@ def main():
     ham.store = "fp32"
     ham.rebuild()
     step_times, build_ms, dav_times, fill_ms = [], [], [], []
-    d32 = Davidson(1)
+    d32 = Davidson(ns)
     for it in range(args.warmup + args.steps):
         t0 = time.time()
         ham.rebuild()
@@ -161,6 +224,12 @@ def main():
         log(f"[bench] step {it}: {t1 - t0:.3f}s (build {tb - t0:.3f}s, mixed Davidson {t1 - tb:.3f}s)")
     bt32 = ham.matrix.build_timings()
     e_mixed = float(d32.lowest_eigenvalues[0])
+    err32_roots = np.abs(np.array(d32.lowest_eigenvalues) - e_ref_all)
+    spmm = None
+    if ns > 1:
+        # multi-root: 4 FP32 vectors per pass over the stored values vs the single-vector product
+        spmm_ms, spmm_bytes = ham.matrix.time_spmm(4, reps=10)
+        spmm = dict(k=4, ms=spmm_ms, GBps=spmm_bytes / spmm_ms / 1e6, ms_per_root=spmm_ms / 4)
     t_mixed_first = first_within(d32.trajectory, d32.trajectory_t, e_ref)
     spmv32_ms, spmv32_bytes = ham.matrix.time_spmv(fp32=True, reps=10)
     n_spmv_est = len(d32.trajectory)
@@ -252,6 +321,13 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if ns > 1:
+        out["n_states"] = ns
+        out["energies_ref_fp64"] = e_ref_all.tolist()
+        out["mixed_err_uHa_per_root"] = (err32_roots * 1e6).tolist()
+        out["fp64_err_uHa_per_root"] = (err64_roots * 1e6).tolist()
+        out["spmm_fp32"] = dict(spmm, single_vector_ms=spmv32_ms,
+                                speedup_per_root=spmv32_ms / spmm["ms_per_root"])
     print(json.dumps(out))
     if args.copy_calibration:
         import torch

@@ -144,6 +144,13 @@ int shci_get_build_timings(shci_ctx* ctx, float* phases);
  * algorithmic bytes one SpMV moves.  flags: SHCI_FP32 selects the FP32 store. */
 int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, double* bytes);
 
+/* Average kernel time of `reps` FP32 block products with k = 4 device-resident
+ * vectors (the multi-root product of shci_spmm) and the algorithmic bytes of
+ * one product (values read once for the 4 vectors).  No reference
+ * counterpart (the reference declares a multi-vector mul but never defines
+ * it, sparse_matrix.h:28-32). */
+int shci_time_spmm(shci_ctx* ctx, uint32_t k, int reps, float* ms_avg, double* bytes);
+
 /* Layout statistics (no reference counterpart; diagnostics): stats[12] =
  * n_dets, stored off-diagonal entries (both directions), allocated entries
  * (both directions), segment layout?, segments, segment memberships,

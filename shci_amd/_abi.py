@@ -40,6 +40,7 @@ EXPORTS = {
     "shci_export_upper": (C.c_int, [_vp, _i64p, _i64p, _dp]),
     "shci_get_build_timings": (C.c_int, [_vp, _fp]),
     "shci_time_spmv": (C.c_int, [_vp, C.c_uint32, C.c_int, _fp, _dp]),
+    "shci_time_spmm": (C.c_int, [_vp, C.c_uint32, C.c_int, _fp, _dp]),
     "shci_memory": (C.c_int, [_vp, _u64p]),
     "shci_layout_stats": (C.c_int, [_vp, _u64p]),
 }

@@ -34,7 +34,7 @@ class VariationalSolver:
     """Fixed determinant list + device Hamiltonian; warm-started solves."""
 
     def __init__(self, n_orb, up, dn, core=0.0, n_states=1, device=0, tol=1e-13, mode="fp64", max_calls=20,
-                 res_tol=1e-9, max_rounds=40):
+                 res_tol=1e-9, max_rounds=40, layout="csr"):
         self.n = int(n_orb)
         self.up = np.ascontiguousarray(up, dtype=np.uint64)
         self.dn = np.ascontiguousarray(dn, dtype=np.uint64)
@@ -46,6 +46,7 @@ class VariationalSolver:
         self.max_calls = max_calls
         self.res_tol = res_tol
         self.max_rounds = max_rounds
+        self.layout = layout
         self.residual = np.inf
         self.ham = None
         self.coefs = None
@@ -57,7 +58,8 @@ class VariationalSolver:
         h1 = 0.5 * (h1 + h1.T)
         if self.ham is None:
             self.ham = Hamiltonian(self.n, h1, eri, self.core, device=self.device,
-                                   store="both" if self.mode == "mixed" else "fp64", keep_zeros=True)
+                                   store="both" if self.mode == "mixed" else "fp64", keep_zeros=True,
+                                   layout=self.layout)
             self.ham.update(self.up, self.dn)
         else:
             self.ham.set_integrals(self.n, h1, eri, self.core)

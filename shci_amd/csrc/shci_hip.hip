@@ -1497,52 +1497,96 @@ __device__ __forceinline__ void load_x(const XT* __restrict__ x, uint32_t det, X
   }
 }
 
-// Off-diagonal part, lane groups of S lanes (one segment of m <= S members
-// per group).  VT stored value type, XT vector (and accumulation) type.
-template <int S, int NV, typename VT, typename XT>
+// Off-diagonal part, lane groups of S lanes, each group handling NB segments
+// of m <= S members at once.  Per segment the chain descriptor -> member id ->
+// x gather is a few dependent memory latencies; NB independent chains per
+// lane keep that many in flight.  Loads are unconditional at in-bounds
+// (clamped) positions and masked by multiplication; value slots are issued
+// CH = 4 steps at a time.  VT stored value type, XT vector (and accumulation)
+// type.
+template <int S, int NB, int NV, typename VT, typename XT>
 __global__ __launch_bounds__(256) void k_seg_spmv_small(const SegDesc* __restrict__ desc, uint32_t s0, uint32_t s1,
                                                         const uint32_t* __restrict__ mem, const VT* __restrict__ val,
                                                         const XT* __restrict__ x, XT* __restrict__ y) {
-  const uint64_t s = (uint64_t)s0 + (uint64_t)blockIdx.x * (256 / S) + threadIdx.x / S;
+  constexpr int MH = S / 2;
+  constexpr int CH = MH < 4 ? MH : 4;
+  constexpr int GPB = 256 / S;  // segments per block and batch
   const uint32_t j = threadIdx.x & (S - 1);
   const int gbase = (int)(lane_id() & ~(uint32_t)(S - 1));
-  SegDesc D{0u, 0u, 0ull};
-  if (s < s1) D = desc[s];
-  const uint32_t m = D.m, mh = m >> 1;
-  const bool valid = j < m;
-  uint32_t det = 0;
-  XT xj[NV], acc[NV];
-  for (int k = 0; k < NV; k++) {
-    xj[k] = (XT)0;
-    acc[k] = (XT)0;
+  const uint64_t sbase = (uint64_t)s0 + (uint64_t)blockIdx.x * (NB * GPB) + threadIdx.x / S;
+  SegDesc D[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    const uint64_t sb = sbase + (uint64_t)b * GPB;
+    D[b] = SegDesc{0u, 0u, 0ull};
+    if (sb < s1) D[b] = desc[sb];
   }
-  if (valid) {
-    det = mem[D.moff + j];
-    load_x<NV, XT>(x, det, xj);
+  bool valid[NB];
+  uint32_t det[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    valid[b] = j < D[b].m;
+    det[b] = mem[valid[b] ? D[b].moff + j : 0];
   }
-  const VT* __restrict__ vs = val + D.voff;
-  uint32_t p = j, q = j;  // (j + d) mod m, (j - d) mod m
-  for (uint32_t d = 1; d <= mh; d++) {
-    if (valid) {
-      p = p + 1 == m ? 0 : p + 1;
-      q = q == 0 ? m - 1 : q - 1;
+  XT xj[NB][NV], acc[NB][NV];
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    load_x<NV, XT>(x, det[b], xj[b]);
+    for (int k = 0; k < NV; k++) {
+      xj[b][k] = valid[b] ? xj[b][k] : (XT)0;
+      acc[b][k] = (XT)0;
     }
-    const bool half = 2 * d == m;
-    const uint32_t vj = half && j >= mh ? j - mh : j;
-    const XT v = valid ? (XT)__builtin_nontemporal_load(vs + (uint64_t)(d - 1) * m + vj) : (XT)0;
-    const XT vb = __shfl(v, gbase + (int)q, kWave);  // value of the pair (q, j), held by lane q
-    if (half) {
-      for (int k = 0; k < NV; k++) acc[k] += v * __shfl(xj[k], gbase + (int)p, kWave);
-    } else {
-      for (int k = 0; k < NV; k++) {
-        const XT xp = __shfl(xj[k], gbase + (int)p, kWave);
-        const XT xq = __shfl(xj[k], gbase + (int)q, kWave);
-        acc[k] += v * xp + vb * xq;
+  }
+  uint32_t p[NB], q[NB];  // (j + d) mod m, (j - d) mod m for the members
+#pragma unroll
+  for (int b = 0; b < NB; b++) p[b] = q[b] = j;
+#pragma unroll 1
+  for (int c0 = 1; c0 <= MH; c0 += CH) {
+    XT v[NB][CH];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const uint32_t m = D[b].m, mh = m >> 1;
+#pragma unroll
+      for (int i = 0; i < CH; i++) {
+        const uint32_t d = (uint32_t)(c0 + i);
+        const bool live = valid[b] && d <= mh;
+        const uint32_t vj = (2 * d == m && j >= mh) ? j - mh : j;
+        const uint64_t off = live ? (uint64_t)(d - 1) * m + vj : 0;
+        v[b][i] = (XT)__builtin_nontemporal_load(val + D[b].voff + off) * (live ? (XT)1 : (XT)0);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < CH; i++) {
+      const uint32_t d = (uint32_t)(c0 + i);
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        const uint32_t m = D[b].m;
+        const uint32_t pn = p[b] + 1 == m ? 0 : p[b] + 1, qn = q[b] == 0 ? m - 1 : q[b] - 1;
+        p[b] = valid[b] ? pn : p[b];
+        q[b] = valid[b] ? qn : q[b];
+        // Backward value: the slot of the pair (q, j), held by lane q.  In the
+        // last step of an even m both directions share one slot.
+        const XT vbr = __shfl(v[b][i], gbase + (int)q[b], kWave);
+        const XT vb = 2 * d == m ? (XT)0 : vbr;
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+          const XT xp = __shfl(xj[b][k], gbase + (int)p[b], kWave);
+          const XT xq = __shfl(xj[b][k], gbase + (int)q[b], kWave);
+          acc[b][k] += v[b][i] * xp + vb * xq;
+        }
       }
     }
   }
-  if (valid)
-    for (int k = 0; k < NV; k++) atomicAdd(&y[(uint64_t)det * NV + k], acc[k]);
+#pragma unroll
+  for (int b = 0; b < NB; b++)
+    if (valid[b])
+      for (int k = 0; k < NV; k++) atomicAdd(&y[(uint64_t)det[b] * NV + k], acc[b][k]);
+}
+
+// Segments per lane group of the class kernels (see k_seg_spmv_small).
+template <int S>
+constexpr int seg_batches() {
+  return S >= 16 ? 2 : 4;
 }
 
 // Off-diagonal part, one wavefront per (segment, group of 64 T members) for
@@ -1958,6 +2002,17 @@ int build_seg_layout(shci_ctx* ctx) {
   if (M == 0) return SHCI_OK;
   if (M >= (1ull << 31)) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment layout: n_dets * keys per det >= 2^31");
   if (K.tag_shift + 2 > 64) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment keys exceed 64 bits");
+  // The sort needs ~36 B per key; value stores of a previous build are
+  // reallocated after it anyway, so drop them first when memory is short.
+  {
+    size_t free_b = 0, total_b = 0;
+    HIPC(hipMemGetInfo(&free_b, &total_b));
+    if ((double)free_b < 40.0 * (double)M) {
+      DevBuf* big[] = {&ctx->sv64, &ctx->sv32, &ctx->seg_mem, &ctx->seg_desc, &ctx->seg_core, &ctx->seg_type,
+                       &ctx->dav, &ctx->tmp};
+      for (DevBuf* b : big) release(ctx, *b);
+    }
+  }
   // Keys of every (det, core) membership, one radix sort.
   TmpBuf keys(ctx), keys_s(ctx), vals(ctx), vals_s(ctx);
   CHECK_RC(ensure(ctx, keys.b, M * 8));
@@ -2129,15 +2184,18 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
       continue;
     }
     const int S = 64 >> (c - 3);
-    const uint32_t grid = div_up((uint64_t)(s1 - s0) * S, 256);
+#define SHCI_SEG_SMALL(SS)                                                                           \
+  k_seg_spmv_small<SS, seg_batches<SS>(), NV, VT, XT>                                                \
+      <<<div_up((uint64_t)(s1 - s0) * SS, 256 * seg_batches<SS>()), 256, 0, st>>>(desc, s0, s1, mem, val, x, y)
     switch (S) {
-      case 64: k_seg_spmv_small<64, NV, VT, XT><<<grid, 256, 0, st>>>(desc, s0, s1, mem, val, x, y); break;
-      case 32: k_seg_spmv_small<32, NV, VT, XT><<<grid, 256, 0, st>>>(desc, s0, s1, mem, val, x, y); break;
-      case 16: k_seg_spmv_small<16, NV, VT, XT><<<grid, 256, 0, st>>>(desc, s0, s1, mem, val, x, y); break;
-      case 8: k_seg_spmv_small<8, NV, VT, XT><<<grid, 256, 0, st>>>(desc, s0, s1, mem, val, x, y); break;
-      case 4: k_seg_spmv_small<4, NV, VT, XT><<<grid, 256, 0, st>>>(desc, s0, s1, mem, val, x, y); break;
-      default: k_seg_spmv_small<2, NV, VT, XT><<<grid, 256, 0, st>>>(desc, s0, s1, mem, val, x, y); break;
+      case 64: SHCI_SEG_SMALL(64); break;
+      case 32: SHCI_SEG_SMALL(32); break;
+      case 16: SHCI_SEG_SMALL(16); break;
+      case 8: SHCI_SEG_SMALL(8); break;
+      case 4: SHCI_SEG_SMALL(4); break;
+      default: SHCI_SEG_SMALL(2); break;
     }
+#undef SHCI_SEG_SMALL
   }
   HIPC(hipGetLastError());
   return SHCI_OK;
@@ -3183,6 +3241,44 @@ int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, doubl
       // values + columns, row start/end (int64 each), FP64 diagonal, x once, y once.
       *bytes = F * (vs + 4.0) + (double)n * 16.0 + (double)n * 8.0 + 2.0 * (double)n * vs;
     }
+  }
+  return SHCI_OK;
+}
+
+int shci_time_spmm(shci_ctx* ctx, uint32_t k, int reps, float* ms_avg, double* bytes) {
+  if (!ctx || reps <= 0 || k != 4) return ctx ? fail(ctx, SHCI_ERR_ARG, "shci_time_spmm: k must be 4") : SHCI_ERR_ARG;
+  if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
+  if (!has_fp32(ctx)) return fail(ctx, SHCI_ERR_STATE, "FP32 store not built");
+  HIPC(hipSetDevice(ctx->device));
+  const uint64_t n = ctx->n;
+  const hipStream_t st = ctx->stream;
+  TmpBuf xi(ctx), yi(ctx);
+  CHECK_RC(ensure(ctx, xi.b, (size_t)4 * n * 4));
+  CHECK_RC(ensure(ctx, yi.b, (size_t)4 * n * 4));
+  {
+    std::vector<float> h((size_t)4 * n, (float)(1.0 / std::sqrt((double)n)));
+    HIPC(hipMemcpyAsync(xi.b.p, h.data(), (size_t)4 * n * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipStreamSynchronize(st));
+  }
+  auto once = [&]() -> int {
+    return ctx->segl ? launch_seg_spmv<4, float, float>(ctx, 0.0, xi.p<float>(), yi.p<float>())
+                     : launch_spmm4(ctx, 0.0, xi.p<float>(), yi.p<float>());
+  };
+  CHECK_RC(once());  // warm-up
+  HIPC(hipEventRecord(ctx->ev[0], st));
+  for (int r = 0; r < reps; r++) CHECK_RC(once());
+  HIPC(hipEventRecord(ctx->ev[1], st));
+  HIPC(hipEventSynchronize(ctx->ev[1]));
+  float ms = 0.f;
+  HIPC(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  if (ms_avg) *ms_avg = ms / reps;
+  if (bytes) {
+    // values (and columns or member ids) once for all 4 vectors; 4 x and 4 y values per det
+    const double vec = 2.0 * 4.0 * 4.0 * (double)n + 8.0 * (double)n;
+    if (ctx->segl)
+      *bytes = (double)ctx->n_slots * 4.0 + (double)ctx->n_mem * 4.0 + (double)ctx->n_seg * sizeof(SegDesc) + vec;
+    else
+      *bytes = (double)ctx->nnz * 8.0 + (double)n * 16.0 + vec;
   }
   return SHCI_OK;
 }

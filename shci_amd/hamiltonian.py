@@ -165,6 +165,14 @@ class SparseMatrix:
                                                    C.byref(ms), C.byref(by)), "shci_time_spmv")
         return ms.value, by.value
 
+    def time_spmm(self, k: int = 4, reps: int = 10):
+        """Average time (ms) and algorithmic bytes of an FP32 block product with k vectors."""
+        ms = C.c_float()
+        by = C.c_double()
+        self.ctx.check(self.ctx.lib.shci_time_spmm(self.ctx.handle, k, reps, C.byref(ms), C.byref(by)),
+                       "shci_time_spmm")
+        return ms.value, by.value
+
     def layout_stats(self):
         st = (C.c_uint64 * 12)()
         self.ctx.check(self.ctx.lib.shci_layout_stats(self.ctx.handle, st), "shci_layout_stats")

@@ -23,9 +23,9 @@ __all__ = ["Workload", "grow_det_list", "SCHEDULES"]
 SCHEDULES = {
     1: (5e-4, 2e-4, 1e-4),
     2: (1e-3, 1e-4, 1e-5, 3e-6, 1e-6, 6e-7, 4e-7),
-    3: (1e-3, 1e-4, 1e-5, 3e-6, 1e-6, 5e-7, 3e-7, 2e-7),
+    3: (1e-3, 1e-4, 1e-5, 3e-6, 1e-6, 5e-7, 3e-7, 2e-7, 1.6e-7),
     4: (1e-3, 1e-4, 1e-5, 3e-6, 1e-6, 5e-7),
-    5: (1e-3, 1e-4, 1e-5, 3e-6, 1e-6, 5e-7, 3e-7, 2e-7),
+    5: (1e-3, 1e-4, 1e-5, 3e-6, 1e-6, 8e-7),
 }
 
 
