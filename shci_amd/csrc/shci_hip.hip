@@ -1292,33 +1292,75 @@ __global__ void k_seg_keys(SegKeyParams K, uint64_t total, uint64_t* __restrict_
   vals[t] = (uint32_t)t;
 }
 
-// Runs of equal keys -> kept segments (m >= 2).
-__global__ void k_seg_keep(const uint32_t* __restrict__ rstart, uint64_t R, uint32_t* __restrict__ keep,
-                           unsigned int* __restrict__ bad) {
+// A member is kept when the run owns at least one pair of it with another
+// member: members whose pairs all belong to other segments (a C core whose
+// other members share its alpha or beta string, an alpha single owned by a
+// different core) would only cost an x gather and a y update per product.
+__global__ void k_seg_useful(SegKeyParams K, const uint32_t* __restrict__ vals_s, const uint32_t* __restrict__ incl,
+                             const uint32_t* __restrict__ rstart, uint64_t M, uint32_t* __restrict__ use) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M) return;
+  const uint32_t r = incl[p] - 1;
+  const uint32_t a = rstart[r], b = rstart[r + 1];
+  uint32_t useful = 0;
+  if (b - a >= 2) {
+    uint32_t ty, k, ty0, k0;
+    uint64_t i, i0;
+    seg_decode(K, vals_s[p], ty, i, k);
+    seg_decode(K, vals_s[a], ty0, i0, k0);
+    const uint64_t core = ty == kSegC ? 0ull : ty == kSegA ? drop_electrons(K.up[i0], k0, K.ra)
+                                                           : drop_electrons(K.dn[i0], k0, K.rb);
+    const uint32_t nrm = ty == kSegA ? K.ra : K.rb;
+    const uint64_t ui = K.up[i], di = K.dn[i];
+    for (uint32_t q = a; q < b && !useful; q++) {
+      if (q == p) continue;
+      uint32_t tq, kq;
+      uint64_t l;
+      seg_decode(K, vals_s[q], tq, l, kq);
+      useful = seg_owns(ty, core, nrm, ui, di, K.up[l], K.dn[l]) ? 1u : 0u;
+    }
+  }
+  use[p] = useful;
+}
+
+// Kept members of run r (from the inclusive scan of the member flags) and the
+// scan value before the run.
+__device__ __forceinline__ uint32_t run_kept(const uint32_t* __restrict__ useinc, const uint32_t* __restrict__ rstart,
+                                             uint32_t r, uint32_t& before) {
+  const uint32_t a = rstart[r], b = rstart[r + 1];
+  before = a ? useinc[a - 1] : 0u;
+  return useinc[b - 1] - before;
+}
+
+// Runs of equal keys -> kept segments (>= 2 kept members).
+__global__ void k_seg_keep(const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ useinc, uint64_t R,
+                           uint32_t* __restrict__ keep, unsigned int* __restrict__ bad) {
   const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r > R) return;
   if (r == R) {
     keep[r] = 0;
     return;
   }
-  const uint32_t m = rstart[r + 1] - rstart[r];
-  keep[r] = m >= 2 ? 1u : 0u;
-  if (m > kSegMaxM) atomicOr(bad, 1u);
+  if (rstart[r + 1] - rstart[r] > kSegMaxM) atomicOr(bad, 1u);
+  uint32_t before;
+  keep[r] = run_kept(useinc, rstart, (uint32_t)r, before) >= 2 ? 1u : 0u;
 }
 
-__global__ void k_seg_compact(const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ keep,
-                              const uint32_t* __restrict__ sidx, uint64_t R, uint32_t* __restrict__ skey,
-                              uint32_t* __restrict__ srun) {
+__global__ void k_seg_compact(const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ useinc,
+                              const uint32_t* __restrict__ keep, const uint32_t* __restrict__ sidx, uint64_t R,
+                              uint32_t* __restrict__ skey, uint32_t* __restrict__ srun) {
   const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R || !keep[r]) return;
-  skey[sidx[r]] = kSegMaxM - (rstart[r + 1] - rstart[r]);
+  uint32_t before;
+  skey[sidx[r]] = kSegMaxM - run_kept(useinc, rstart, (uint32_t)r, before);
   srun[sidx[r]] = (uint32_t)r;
 }
 
 // Ordered segment f (largest first): sizes, slot counts, run -> segment map,
 // class histogram.
-__global__ void k_seg_sizes(const uint32_t* __restrict__ srun, const uint32_t* __restrict__ rstart, uint64_t S,
-                            uint64_t* __restrict__ msz, uint64_t* __restrict__ nslot, uint32_t* __restrict__ finv,
+__global__ void k_seg_sizes(const uint32_t* __restrict__ srun, const uint32_t* __restrict__ rstart,
+                            const uint32_t* __restrict__ useinc, uint64_t S, uint64_t* __restrict__ msz,
+                            uint64_t* __restrict__ nslot, uint32_t* __restrict__ finv,
                             unsigned int* __restrict__ hist) {
   __shared__ unsigned int h[kSegClasses];
   if (threadIdx.x < kSegClasses) h[threadIdx.x] = 0;
@@ -1326,7 +1368,8 @@ __global__ void k_seg_sizes(const uint32_t* __restrict__ srun, const uint32_t* _
   const uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f < S) {
     const uint32_t r = srun[f];
-    const uint64_t m = rstart[r + 1] - rstart[r];
+    uint32_t before;
+    const uint64_t m = run_kept(useinc, rstart, r, before);
     msz[f] = m;
     nslot[f] = m * (m - 1) / 2;
     finv[r] = (uint32_t)f;
@@ -1337,15 +1380,17 @@ __global__ void k_seg_sizes(const uint32_t* __restrict__ srun, const uint32_t* _
 }
 
 __global__ void k_seg_desc(SegKeyParams K, const uint32_t* __restrict__ srun, const uint32_t* __restrict__ rstart,
-                           const uint32_t* __restrict__ vals_s, const uint64_t* __restrict__ moff,
+                           const uint32_t* __restrict__ useinc, const uint32_t* __restrict__ vals_s,
+                           const uint64_t* __restrict__ moff,
                            const uint64_t* __restrict__ voff, uint64_t S, SegDesc* __restrict__ desc,
                            uint8_t* __restrict__ type, uint64_t* __restrict__ core) {
   const uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= S) return;
   const uint32_t r = srun[f];
   SegDesc D;
+  uint32_t before;
   D.moff = (uint32_t)moff[f];
-  D.m = rstart[r + 1] - rstart[r];
+  D.m = run_kept(useinc, rstart, r, before);
   D.voff = voff[f];
   desc[f] = D;
   uint32_t ty, k;
@@ -1356,17 +1401,20 @@ __global__ void k_seg_desc(SegKeyParams K, const uint32_t* __restrict__ srun, co
 }
 
 __global__ void k_seg_members(SegKeyParams K, const uint32_t* __restrict__ vals_s, const uint32_t* __restrict__ incl,
-                              const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ finv,
+                              const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ use,
+                              const uint32_t* __restrict__ useinc, const uint32_t* __restrict__ finv,
                               const SegDesc* __restrict__ desc, uint64_t M, uint32_t* __restrict__ mem) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= M) return;
+  if (p >= M || !use[p]) return;
   const uint32_t r = incl[p] - 1;
   const uint32_t f = finv[r];
   if (f == 0xFFFFFFFFu) return;
   uint32_t ty, k;
   uint64_t i;
   seg_decode(K, vals_s[p], ty, i, k);
-  mem[(uint64_t)desc[f].moff + (p - rstart[r])] = (uint32_t)i;
+  const uint32_t a = rstart[r];
+  const uint32_t before = a ? useinc[a - 1] : 0u;
+  mem[(uint64_t)desc[f].moff + (useinc[p] - 1 - before)] = (uint32_t)i;
 }
 
 // ---- value fill / update ----
@@ -2033,15 +2081,21 @@ int build_seg_layout(shci_ctx* ctx) {
   CHECK_RC(incl_scan(ctx, flag, incl, M));
   uint32_t R = 0;
   CHECK_RC(copy_back(ctx, &R, incl + (M - 1)));
-  TmpBuf rstart(ctx), keep(ctx), sidx(ctx);
+  TmpBuf rstart(ctx), keep(ctx), sidx(ctx), useinc(ctx);
   CHECK_RC(ensure(ctx, rstart.b, ((uint64_t)R + 1) * 4));
   CHECK_RC(ensure(ctx, keep.b, ((uint64_t)R + 1) * 4));
   CHECK_RC(ensure(ctx, sidx.b, ((uint64_t)R + 1) * 4));
+  CHECK_RC(ensure(ctx, useinc.b, M * 4));
   k_run_starts<<<div_up(M, 256), 256, 0, st>>>(flag, incl, rstart.p<uint32_t>(), M);
+  // Member flags (reusing the head-flag array) and their inclusive scan.
+  uint32_t* use = flag;
+  k_seg_useful<<<div_up(M, 256), 256, 0, st>>>(K, vals_s.p<uint32_t>(), incl, rstart.p<uint32_t>(), M, use);
+  CHECK_RC(incl_scan(ctx, use, useinc.p<uint32_t>(), M));
   CHECK_RC(ensure(ctx, ctx->scal, 64 * 8));
   unsigned int* bad = reinterpret_cast<unsigned int*>(ctx->scal.p);
   HIPC(hipMemsetAsync(bad, 0, 4, st));
-  k_seg_keep<<<div_up((uint64_t)R + 1, 256), 256, 0, st>>>(rstart.p<uint32_t>(), R, keep.p<uint32_t>(), bad);
+  k_seg_keep<<<div_up((uint64_t)R + 1, 256), 256, 0, st>>>(rstart.p<uint32_t>(), useinc.p<uint32_t>(), R,
+                                                          keep.p<uint32_t>(), bad);
   CHECK_RC(excl_scan(ctx, keep.p<uint32_t>(), sidx.p<uint32_t>(), (uint64_t)R + 1));
   uint32_t S = 0, hbad = 0;
   CHECK_RC(copy_back(ctx, &S, sidx.p<uint32_t>() + R));
@@ -2054,8 +2108,8 @@ int build_seg_layout(shci_ctx* ctx) {
   CHECK_RC(ensure(ctx, srun.b, (uint64_t)S * 4));
   CHECK_RC(ensure(ctx, skey_s.b, (uint64_t)S * 4));
   CHECK_RC(ensure(ctx, srun_s.b, (uint64_t)S * 4));
-  k_seg_compact<<<div_up(R, 256), 256, 0, st>>>(rstart.p<uint32_t>(), keep.p<uint32_t>(), sidx.p<uint32_t>(), R,
-                                                skey.p<uint32_t>(), srun.p<uint32_t>());
+  k_seg_compact<<<div_up(R, 256), 256, 0, st>>>(rstart.p<uint32_t>(), useinc.p<uint32_t>(), keep.p<uint32_t>(),
+                                                sidx.p<uint32_t>(), R, skey.p<uint32_t>(), srun.p<uint32_t>());
   release(ctx, keep.b);
   release(ctx, sidx.b);
   CHECK_RC(radix_sort_pairs(ctx, skey.p<uint32_t>(), skey_s.p<uint32_t>(), srun.p<uint32_t>(), srun_s.p<uint32_t>(), S,
@@ -2072,8 +2126,8 @@ int build_seg_layout(shci_ctx* ctx) {
   HIPC(hipMemsetAsync(finv.b.p, 0xFF, (uint64_t)R * 4, st));
   unsigned int* hist = bad + 4;
   HIPC(hipMemsetAsync(hist, 0, kSegClasses * 4, st));
-  k_seg_sizes<<<div_up(S, 256), 256, 0, st>>>(srun_s.p<uint32_t>(), rstart.p<uint32_t>(), S, msz.p<uint64_t>(),
-                                             nslot.p<uint64_t>(), finv.p<uint32_t>(), hist);
+  k_seg_sizes<<<div_up(S, 256), 256, 0, st>>>(srun_s.p<uint32_t>(), rstart.p<uint32_t>(), useinc.p<uint32_t>(), S,
+                                             msz.p<uint64_t>(), nslot.p<uint64_t>(), finv.p<uint32_t>(), hist);
   CHECK_RC(excl_scan(ctx, msz.p<uint64_t>(), moff.p<uint64_t>(), S));
   CHECK_RC(excl_scan(ctx, nslot.p<uint64_t>(), voff.p<uint64_t>(), S));
   uint64_t last[4];
@@ -2090,11 +2144,13 @@ int build_seg_layout(shci_ctx* ctx) {
   CHECK_RC(ensure(ctx, ctx->seg_core, (uint64_t)S * 8));
   CHECK_RC(ensure(ctx, ctx->seg_mem, n_mem * 4));
   SegDesc* desc = P<SegDesc>(ctx->seg_desc);
-  k_seg_desc<<<div_up(S, 256), 256, 0, st>>>(K, srun_s.p<uint32_t>(), rstart.p<uint32_t>(), vals_s.p<uint32_t>(),
+  k_seg_desc<<<div_up(S, 256), 256, 0, st>>>(K, srun_s.p<uint32_t>(), rstart.p<uint32_t>(), useinc.p<uint32_t>(),
+                                            vals_s.p<uint32_t>(),
                                             moff.p<uint64_t>(), voff.p<uint64_t>(), S, desc,
                                             P<uint8_t>(ctx->seg_type), P<uint64_t>(ctx->seg_core));
-  k_seg_members<<<div_up(M, 256), 256, 0, st>>>(K, vals_s.p<uint32_t>(), incl, rstart.p<uint32_t>(),
-                                               finv.p<uint32_t>(), desc, M, P<uint32_t>(ctx->seg_mem));
+  k_seg_members<<<div_up(M, 256), 256, 0, st>>>(K, vals_s.p<uint32_t>(), incl, rstart.p<uint32_t>(), use,
+                                               useinc.p<uint32_t>(), finv.p<uint32_t>(), desc, M,
+                                               P<uint32_t>(ctx->seg_mem));
   HIPC(hipGetLastError());
   ctx->n_seg = S;
   ctx->n_mem = n_mem;
