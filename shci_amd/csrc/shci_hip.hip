@@ -106,9 +106,11 @@ struct shci_ctx {
   uint32_t r2_alpha = 0, r2_beta = 0;   // distinct minus-two strings
   DevBuf mA_id, mB_id;                  // [group * C(n_e, 2) + pair] -> minus-two string id
   uint64_t n_seg = 0, n_mem = 0, n_slots = 0, n_pairs = 0;
-  uint64_t seg_cs[10] = {};             // segment ranges of the size classes
-  uint32_t seg_maxm[3] = {};            // largest segment of each one-wavefront class
+  uint64_t seg_cs[28] = {};             // segment ranges: size class x type (C, A, B)
+  uint32_t seg_maxm[27] = {};           // largest segment of each range
   DevBuf seg_desc, seg_type, seg_core, seg_mem, sv32, sv64;
+  DevBuf dnpos;                         // internal det -> position in beta order
+  DevBuf xb, yb;                        // beta-order x and partial y of the A segments
 
   hipEvent_t ev[9] = {};
   float phase_ms[8] = {};
@@ -1166,6 +1168,7 @@ __global__ __launch_bounds__(256) void k_export_fill(uint64_t n, const int64_t* 
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSegMaxM = 4096;  // C: (n_orb-n_up+1)(n_orb-n_dn+1) <= 63*63; A, B: C(n_orb-n+2, 2) <= 2016
 constexpr int kSegClasses = 9;       // m > 1024, > 256, > 64 (one wavefront each); lane groups of 64 .. 2
+constexpr int kSegRanges = kSegClasses * 3;  // x segment type; range index = class * 3 + type
 
 enum : uint32_t { kSegC = 0, kSegA = 1, kSegB = 2 };
 
@@ -1248,6 +1251,7 @@ struct SegKeyParams {
   const uint32_t* __restrict__ mB;
   const uint64_t* __restrict__ up;
   const uint64_t* __restrict__ dn;
+  const uint32_t* __restrict__ dnpos;  // internal det -> position in beta order
   int bC, bA, bB;  // widths of the low key fields
   int tag_shift;
 };
@@ -1346,24 +1350,36 @@ __global__ void k_seg_keep(const uint32_t* __restrict__ rstart, const uint32_t* 
   keep[r] = run_kept(useinc, rstart, (uint32_t)r, before) >= 2 ? 1u : 0u;
 }
 
-__global__ void k_seg_compact(const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ useinc,
+// Order key of a kept run: size class, type, then (lane-group classes) the
+// alpha group (C, B) or beta group (A) of its first member, so that the
+// segments sharing a wavefront touch neighbouring x and y entries; within the
+// one-wavefront classes, largest first.
+__global__ void k_seg_compact(SegKeyParams K, const uint32_t* __restrict__ vals_s,
+                              const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ useinc,
                               const uint32_t* __restrict__ keep, const uint32_t* __restrict__ sidx, uint64_t R,
-                              uint32_t* __restrict__ skey, uint32_t* __restrict__ srun) {
+                              uint64_t* __restrict__ skey, uint32_t* __restrict__ srun) {
   const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R || !keep[r]) return;
   uint32_t before;
-  skey[sidx[r]] = kSegMaxM - run_kept(useinc, rstart, (uint32_t)r, before);
+  const uint32_t m = run_kept(useinc, rstart, (uint32_t)r, before);
+  uint32_t ty, k;
+  uint64_t i;
+  seg_decode(K, vals_s[rstart[r]], ty, i, k);
+  const uint64_t cls = seg_class(m);
+  const uint64_t grp = ty == kSegA ? K.dn_gid[i] : K.up_gid[i];
+  const uint64_t low = cls < 3 ? (uint64_t)(kSegMaxM - m) : ((grp << 13) | (uint64_t)(kSegMaxM - m));
+  skey[sidx[r]] = (cls << 60) | ((uint64_t)ty << 58) | low;
   srun[sidx[r]] = (uint32_t)r;
 }
 
 // Ordered segment f (largest first): sizes, slot counts, run -> segment map,
 // class histogram.
-__global__ void k_seg_sizes(const uint32_t* __restrict__ srun, const uint32_t* __restrict__ rstart,
-                            const uint32_t* __restrict__ useinc, uint64_t S, uint64_t* __restrict__ msz,
-                            uint64_t* __restrict__ nslot, uint32_t* __restrict__ finv,
+__global__ void k_seg_sizes(const uint32_t* __restrict__ srun, const uint64_t* __restrict__ skey,
+                            const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ useinc, uint64_t S,
+                            uint64_t* __restrict__ msz, uint64_t* __restrict__ nslot, uint32_t* __restrict__ finv,
                             unsigned int* __restrict__ hist) {
-  __shared__ unsigned int h[kSegClasses];
-  if (threadIdx.x < kSegClasses) h[threadIdx.x] = 0;
+  __shared__ unsigned int h[kSegRanges];
+  if (threadIdx.x < kSegRanges) h[threadIdx.x] = 0;
   __syncthreads();
   const uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f < S) {
@@ -1373,10 +1389,11 @@ __global__ void k_seg_sizes(const uint32_t* __restrict__ srun, const uint32_t* _
     msz[f] = m;
     nslot[f] = m * (m - 1) / 2;
     finv[r] = (uint32_t)f;
-    atomicAdd(&h[seg_class((uint32_t)m)], 1u);
+    const uint64_t key = skey[f];
+    atomicAdd(&h[(uint32_t)(key >> 60) * 3 + (uint32_t)((key >> 58) & 3)], 1u);
   }
   __syncthreads();
-  if (threadIdx.x < kSegClasses && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+  if (threadIdx.x < kSegRanges && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
 
 __global__ void k_seg_desc(SegKeyParams K, const uint32_t* __restrict__ srun, const uint32_t* __restrict__ rstart,
@@ -1414,7 +1431,7 @@ __global__ void k_seg_members(SegKeyParams K, const uint32_t* __restrict__ vals_
   seg_decode(K, vals_s[p], ty, i, k);
   const uint32_t a = rstart[r];
   const uint32_t before = a ? useinc[a - 1] : 0u;
-  mem[(uint64_t)desc[f].moff + (useinc[p] - 1 - before)] = (uint32_t)i;
+  mem[(uint64_t)desc[f].moff + (useinc[p] - 1 - before)] = ty == kSegA ? K.dnpos[i] : (uint32_t)i;
 }
 
 // ---- value fill / update ----
@@ -1425,6 +1442,7 @@ struct SegFillParams {
   const uint64_t* __restrict__ core;
   const uint64_t* __restrict__ up;
   const uint64_t* __restrict__ dn;
+  const uint32_t* __restrict__ dnord;  // beta-order position -> internal det (A members)
   Ints I;
   double* __restrict__ v64;
   float* __restrict__ v32;
@@ -1478,7 +1496,8 @@ __global__ __launch_bounds__(256) void k_seg_fill_small(SegFillParams F) {
   const bool valid = j < m;
   uint64_t uj = 0, dj = 0;
   if (valid) {
-    const uint32_t det = F.mem[D.moff + j];
+    const uint32_t mj = F.mem[D.moff + j];
+    const uint32_t det = ty == kSegA ? F.dnord[mj] : mj;
     uj = F.up[det];
     dj = F.dn[det];
   }
@@ -1504,7 +1523,8 @@ __global__ __launch_bounds__(64) void k_seg_fill_large(SegFillParams F) {
   uint64_t* su = reinterpret_cast<uint64_t*>(seg_lds);
   uint64_t* sd = su + m;
   for (uint32_t j = t; j < m; j += kWave) {
-    const uint32_t det = F.mem[D.moff + j];
+    const uint32_t mj = F.mem[D.moff + j];
+    const uint32_t det = ty == kSegA ? F.dnord[mj] : mj;
     su[j] = F.up[det];
     sd[j] = F.dn[det];
   }
@@ -1710,8 +1730,8 @@ __global__ __launch_bounds__(64) void k_seg_spmv_large(const SegDesc* __restrict
 __global__ __launch_bounds__(256) void k_seg_rdm(RdmParams R, const SegDesc* __restrict__ desc,
                                                  const uint32_t* __restrict__ mem, const uint8_t* __restrict__ type,
                                                  const uint64_t* __restrict__ core, const double* __restrict__ v64,
-                                                 const float* __restrict__ v32, uint32_t ra, uint32_t rb,
-                                                 int keep_zeros, uint64_t n_seg) {
+                                                 const float* __restrict__ v32, const uint32_t* __restrict__ dnord,
+                                                 uint32_t ra, uint32_t rb, int keep_zeros, uint64_t n_seg) {
   const uint64_t s = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (s >= n_seg) return;
   const SegDesc D = desc[s];
@@ -1722,7 +1742,11 @@ __global__ __launch_bounds__(256) void k_seg_rdm(RdmParams R, const SegDesc* __r
     const uint32_t d = (uint32_t)(k / D.m) + 1, j = (uint32_t)(k % D.m);
     uint32_t p = j + d;
     if (p >= D.m) p -= D.m;
-    const uint32_t a = mem[D.moff + j], b = mem[D.moff + p];
+    uint32_t a = mem[D.moff + j], b = mem[D.moff + p];
+    if (ty == kSegA) {
+      a = dnord[a];
+      b = dnord[b];
+    }
     const uint64_t ua = R.up[a], da = R.dn[a], ub = R.up[b], db = R.dn[b];
     if (!seg_owns(ty, cs, ty == kSegA ? ra : rb, ua, da, ub, db)) continue;
     if (!keep_zeros && !(v64 ? v64[D.voff + k] != 0.0 : v32[D.voff + k] != 0.0f)) continue;
@@ -2031,6 +2055,7 @@ SegKeyParams seg_key_params(shci_ctx* ctx) {
   K.mB = nd >= 2 ? P<uint32_t>(ctx->mB_id) : P<uint32_t>(ctx->m1b_id);
   K.up = P<uint64_t>(ctx->up_s);
   K.dn = P<uint64_t>(ctx->dn_s);
+  K.dnpos = P<uint32_t>(ctx->dnpos);
   const uint32_t rA = nu >= 2 ? ctx->r2_alpha : ctx->r_alpha, rB = nd >= 2 ? ctx->r2_beta : ctx->r_beta;
   K.bC = bits_for(ctx->r_beta);
   K.bA = bits_for(ctx->n_dgroups);
@@ -2042,6 +2067,8 @@ SegKeyParams seg_key_params(shci_ctx* ctx) {
 // Segments (sorted by size, largest first), their members and slot offsets.
 int build_seg_layout(shci_ctx* ctx) {
   const hipStream_t st = ctx->stream;
+  CHECK_RC(ensure(ctx, ctx->dnpos, ctx->n * 4));
+  k_inverse_perm<<<div_up(ctx->n, 256), 256, 0, st>>>(P<uint32_t>(ctx->dnord_idx), P<uint32_t>(ctx->dnpos), ctx->n);
   const SegKeyParams K = seg_key_params(ctx);
   const uint64_t M = K.n * ((uint64_t)K.PC + K.PA + K.PB);
   ctx->n_seg = ctx->n_mem = ctx->n_slots = 0;
@@ -2102,21 +2129,21 @@ int build_seg_layout(shci_ctx* ctx) {
   CHECK_RC(copy_back(ctx, &hbad, bad));
   if (hbad) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment with more than 4096 members");
   if (S == 0) return SHCI_OK;
-  // Order the segments by size, largest first (stable: key order within a size).
+  // Order the segments: size class, type, group (k_seg_compact).
   TmpBuf skey(ctx), srun(ctx), skey_s(ctx), srun_s(ctx);
-  CHECK_RC(ensure(ctx, skey.b, (uint64_t)S * 4));
+  CHECK_RC(ensure(ctx, skey.b, (uint64_t)S * 8));
   CHECK_RC(ensure(ctx, srun.b, (uint64_t)S * 4));
-  CHECK_RC(ensure(ctx, skey_s.b, (uint64_t)S * 4));
+  CHECK_RC(ensure(ctx, skey_s.b, (uint64_t)S * 8));
   CHECK_RC(ensure(ctx, srun_s.b, (uint64_t)S * 4));
-  k_seg_compact<<<div_up(R, 256), 256, 0, st>>>(rstart.p<uint32_t>(), useinc.p<uint32_t>(), keep.p<uint32_t>(),
-                                                sidx.p<uint32_t>(), R, skey.p<uint32_t>(), srun.p<uint32_t>());
+  k_seg_compact<<<div_up(R, 256), 256, 0, st>>>(K, vals_s.p<uint32_t>(), rstart.p<uint32_t>(), useinc.p<uint32_t>(),
+                                                keep.p<uint32_t>(), sidx.p<uint32_t>(), R, skey.p<uint64_t>(),
+                                                srun.p<uint32_t>());
   release(ctx, keep.b);
   release(ctx, sidx.b);
-  CHECK_RC(radix_sort_pairs(ctx, skey.p<uint32_t>(), skey_s.p<uint32_t>(), srun.p<uint32_t>(), srun_s.p<uint32_t>(), S,
-                            13));
+  CHECK_RC(radix_sort_pairs(ctx, skey.p<uint64_t>(), skey_s.p<uint64_t>(), srun.p<uint32_t>(), srun_s.p<uint32_t>(), S,
+                            64));  // class in bits 60-63, type 58-59
   release(ctx, skey.b);
   release(ctx, srun.b);
-  release(ctx, skey_s.b);
   TmpBuf msz(ctx), nslot(ctx), finv(ctx), moff(ctx), voff(ctx);
   CHECK_RC(ensure(ctx, msz.b, (uint64_t)S * 8));
   CHECK_RC(ensure(ctx, nslot.b, (uint64_t)S * 8));
@@ -2124,10 +2151,12 @@ int build_seg_layout(shci_ctx* ctx) {
   CHECK_RC(ensure(ctx, moff.b, (uint64_t)S * 8));
   CHECK_RC(ensure(ctx, voff.b, (uint64_t)S * 8));
   HIPC(hipMemsetAsync(finv.b.p, 0xFF, (uint64_t)R * 4, st));
-  unsigned int* hist = bad + 4;
-  HIPC(hipMemsetAsync(hist, 0, kSegClasses * 4, st));
-  k_seg_sizes<<<div_up(S, 256), 256, 0, st>>>(srun_s.p<uint32_t>(), rstart.p<uint32_t>(), useinc.p<uint32_t>(), S,
-                                             msz.p<uint64_t>(), nslot.p<uint64_t>(), finv.p<uint32_t>(), hist);
+  unsigned int* hist = bad + 32;  // bytes 128..235 of the 512-byte status block
+  HIPC(hipMemsetAsync(hist, 0, kSegRanges * 4, st));
+  k_seg_sizes<<<div_up(S, 256), 256, 0, st>>>(srun_s.p<uint32_t>(), skey_s.p<uint64_t>(), rstart.p<uint32_t>(),
+                                             useinc.p<uint32_t>(), S, msz.p<uint64_t>(), nslot.p<uint64_t>(),
+                                             finv.p<uint32_t>(), hist);
+  release(ctx, skey_s.b);
   CHECK_RC(excl_scan(ctx, msz.p<uint64_t>(), moff.p<uint64_t>(), S));
   CHECK_RC(excl_scan(ctx, nslot.p<uint64_t>(), voff.p<uint64_t>(), S));
   uint64_t last[4];
@@ -2137,8 +2166,8 @@ int build_seg_layout(shci_ctx* ctx) {
   CHECK_RC(copy_back(ctx, &last[3], nslot.p<uint64_t>() + (S - 1)));
   const uint64_t n_mem = last[0] + last[1], n_slots = last[2] + last[3];
   if (n_mem >= (1ull << 32)) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment memberships >= 2^32");
-  unsigned int h[kSegClasses];
-  CHECK_RC(copy_back(ctx, h, hist, kSegClasses));
+  unsigned int h[kSegRanges];
+  CHECK_RC(copy_back(ctx, h, hist, kSegRanges));
   CHECK_RC(ensure(ctx, ctx->seg_desc, (uint64_t)S * sizeof(SegDesc)));
   CHECK_RC(ensure(ctx, ctx->seg_type, (uint64_t)S));
   CHECK_RC(ensure(ctx, ctx->seg_core, (uint64_t)S * 8));
@@ -2156,12 +2185,12 @@ int build_seg_layout(shci_ctx* ctx) {
   ctx->n_mem = n_mem;
   ctx->n_slots = n_slots;
   ctx->seg_cs[0] = 0;
-  for (int c = 0; c < kSegClasses; c++) ctx->seg_cs[c + 1] = ctx->seg_cs[c] + h[c];
-  for (int c = 0; c < 3; c++) {
-    if (h[c] == 0) continue;
+  for (int r = 0; r < kSegRanges; r++) ctx->seg_cs[r + 1] = ctx->seg_cs[r] + h[r];
+  for (int r = 0; r < 9; r++) {  // one-wavefront classes: largest segment first in each range
+    if (h[r] == 0) continue;
     SegDesc d0;
-    CHECK_RC(copy_back(ctx, &d0, desc + ctx->seg_cs[c]));
-    ctx->seg_maxm[c] = d0.m;
+    CHECK_RC(copy_back(ctx, &d0, desc + ctx->seg_cs[r]));
+    ctx->seg_maxm[r] = d0.m;
   }
   return SHCI_OK;
 }
@@ -2174,6 +2203,7 @@ SegFillParams seg_fill_params(shci_ctx* ctx, int update) {
   F.core = P<uint64_t>(ctx->seg_core);
   F.up = P<uint64_t>(ctx->up_s);
   F.dn = P<uint64_t>(ctx->dn_s);
+  F.dnord = P<uint32_t>(ctx->dnord_idx);
   F.I = make_ints(ctx);
   F.v64 = P<double>(ctx->sv64);
   F.v32 = P<float>(ctx->sv32);
@@ -2192,13 +2222,14 @@ int seg_fill(shci_ctx* ctx, int update) {
   CHECK_RC(ensure(ctx, ctx->scal, 64 * 8));
   SegFillParams F = seg_fill_params(ctx, update);
   HIPC(hipMemsetAsync(F.kept, 0, 8, st));
-  for (int c = 0; c < kSegClasses; c++) {
-    F.s0 = (uint32_t)ctx->seg_cs[c];
-    F.s1 = (uint32_t)ctx->seg_cs[c + 1];
+  for (int r = 0; r < kSegRanges; r++) {
+    const int c = r / 3;
+    F.s0 = (uint32_t)ctx->seg_cs[r];
+    F.s1 = (uint32_t)ctx->seg_cs[r + 1];
     const uint32_t cnt = F.s1 - F.s0;
     if (cnt == 0) continue;
     if (c < 3) {
-      k_seg_fill_large<<<cnt, kWave, ctx->seg_maxm[c] * 16, st>>>(F);
+      k_seg_fill_large<<<cnt, kWave, ctx->seg_maxm[r] * 16, st>>>(F);
       continue;
     }
     const int S = 64 >> (c - 3);
@@ -2220,29 +2251,70 @@ int seg_fill(shci_ctx* ctx, int update) {
 }
 
 // y = (H - shift) x on internal-order device vectors, NV interleaved vectors.
+// Beta-order copy of x for the A segments (xb[k] = x[dnord[k]]) and a zeroed
+// partial product yb; afterwards y[dnord[k]] += yb[k].
+template <int NV, typename XT>
+__global__ void k_perm_in(const uint32_t* __restrict__ dnord, const XT* __restrict__ x, XT* __restrict__ xb,
+                          XT* __restrict__ yb, uint64_t n) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint64_t i = dnord[k];
+  for (int v = 0; v < NV; v++) {
+    xb[k * NV + v] = x[i * NV + v];
+    yb[k * NV + v] = (XT)0;
+  }
+}
+
+template <int NV, typename XT>
+__global__ void k_perm_add(const uint32_t* __restrict__ dnord, const XT* __restrict__ yb, XT* __restrict__ y,
+                           uint64_t n) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint64_t i = dnord[k];
+  for (int v = 0; v < NV; v++) y[i * NV + v] += yb[k * NV + v];
+}
+
+// y = (H - shift) x on internal-order device vectors, NV interleaved vectors.
+// A segments (members in beta order) run on xb/yb, C and B segments on x/y.
 template <int NV, typename VT, typename XT>
 int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
   const hipStream_t st = ctx->stream;
+  const uint64_t n = ctx->n;
   const VT* val = sizeof(VT) == 4 ? reinterpret_cast<const VT*>(ctx->sv32.p) : reinterpret_cast<const VT*>(ctx->sv64.p);
   if (!val && ctx->n_slots) return fail(ctx, SHCI_ERR_STATE, "requested precision not stored");
-  k_diag_apply<NV, XT><<<div_up(ctx->n, 256), 256, 0, st>>>(P<double>(ctx->diag), shift, x, y, ctx->n);
+  k_diag_apply<NV, XT><<<div_up(n, 256), 256, 0, st>>>(P<double>(ctx->diag), shift, x, y, n);
+  bool have_a = false;
+  for (int c = 0; c < kSegClasses; c++) have_a |= ctx->seg_cs[3 * c + kSegA + 1] > ctx->seg_cs[3 * c + kSegA];
+  XT* xb = nullptr;
+  XT* yb = nullptr;
+  const uint32_t* dnord = P<uint32_t>(ctx->dnord_idx);
+  if (have_a) {
+    CHECK_RC(ensure(ctx, ctx->xb, n * NV * sizeof(XT)));
+    CHECK_RC(ensure(ctx, ctx->yb, n * NV * sizeof(XT)));
+    xb = static_cast<XT*>(ctx->xb.p);
+    yb = static_cast<XT*>(ctx->yb.p);
+    k_perm_in<NV, XT><<<div_up(n, 256), 256, 0, st>>>(dnord, x, xb, yb, n);
+  }
   const SegDesc* desc = P<SegDesc>(ctx->seg_desc);
   const uint32_t* mem = P<uint32_t>(ctx->seg_mem);
-  for (int c = 0; c < kSegClasses; c++) {
-    const uint32_t s0 = (uint32_t)ctx->seg_cs[c], s1 = (uint32_t)ctx->seg_cs[c + 1];
+  for (int r = 0; r < kSegRanges; r++) {
+    const int c = r / 3;
+    const uint32_t s0 = (uint32_t)ctx->seg_cs[r], s1 = (uint32_t)ctx->seg_cs[r + 1];
     if (s1 == s0) continue;
+    const XT* xr = (r % 3 == (int)kSegA) ? xb : x;
+    XT* yr = (r % 3 == (int)kSegA) ? yb : y;
     if (c < 3) {
-      const size_t lds = (size_t)ctx->seg_maxm[c] * NV * sizeof(XT);
+      const size_t lds = (size_t)ctx->seg_maxm[r] * NV * sizeof(XT);
       if (lds > 65536) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment too large for the block product");
       // one wavefront per (segment, 256 members)
-      const dim3 grid(s1 - s0, div_up(ctx->seg_maxm[c], kWave * kSegChunks));
-      k_seg_spmv_large<kSegChunks, NV, VT, XT><<<grid, kWave, lds, st>>>(desc, s0, s1, mem, val, x, y);
+      const dim3 grid(s1 - s0, div_up(ctx->seg_maxm[r], kWave * kSegChunks));
+      k_seg_spmv_large<kSegChunks, NV, VT, XT><<<grid, kWave, lds, st>>>(desc, s0, s1, mem, val, xr, yr);
       continue;
     }
     const int S = 64 >> (c - 3);
 #define SHCI_SEG_SMALL(SS)                                                                           \
   k_seg_spmv_small<SS, seg_batches<SS>(), NV, VT, XT>                                                \
-      <<<div_up((uint64_t)(s1 - s0) * SS, 256 * seg_batches<SS>()), 256, 0, st>>>(desc, s0, s1, mem, val, x, y)
+      <<<div_up((uint64_t)(s1 - s0) * SS, 256 * seg_batches<SS>()), 256, 0, st>>>(desc, s0, s1, mem, val, xr, yr)
     switch (S) {
       case 64: SHCI_SEG_SMALL(64); break;
       case 32: SHCI_SEG_SMALL(32); break;
@@ -2253,6 +2325,7 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
     }
 #undef SHCI_SEG_SMALL
   }
+  if (have_a) k_perm_add<NV, XT><<<div_up(n, 256), 256, 0, st>>>(dnord, yb, y, n);
   HIPC(hipGetLastError());
   return SHCI_OK;
 }
@@ -2354,7 +2427,7 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
   }
   {
     DevBuf* seg[] = {&ctx->seg_desc, &ctx->seg_type, &ctx->seg_core, &ctx->seg_mem, &ctx->sv32, &ctx->sv64,
-                     &ctx->mA_id, &ctx->mB_id};
+                     &ctx->mA_id, &ctx->mB_id, &ctx->dnpos, &ctx->xb, &ctx->yb};
     for (DevBuf* b : seg) release(ctx, *b);
     ctx->n_seg = ctx->n_mem = ctx->n_slots = ctx->n_pairs = 0;
   }
@@ -2740,7 +2813,8 @@ int shci_ctx_destroy(shci_ctx* ctx) {
                     &ctx->perm, &ctx->iperm, &ctx->up_gid, &ctx->up_gstart, &ctx->dn_gid, &ctx->dn_gstart,
                     &ctx->dnord_idx, &ctx->dnord_up, &ctx->m1a_id, &ctx->m1b_id, &ctx->dm1_vals, &ctx->dm1_rstart,
                     &ctx->dm1_seg, &ctx->flag, &ctx->mA_id, &ctx->mB_id, &ctx->seg_desc, &ctx->seg_type,
-                    &ctx->seg_core, &ctx->seg_mem, &ctx->sv32, &ctx->sv64, &ctx->diag,
+                    &ctx->seg_core, &ctx->seg_mem, &ctx->sv32, &ctx->sv64, &ctx->dnpos, &ctx->xb, &ctx->yb,
+                    &ctx->diag,
                     &ctx->cnt,
                     &ctx->row_ptr, &ctx->row_end, &ctx->col, &ctx->v64, &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb,
                     &ctx->va, &ctx->vb, &ctx->scal, &ctx->part, &ctx->xv, &ctx->yv, &ctx->x32, &ctx->y32,
@@ -2928,7 +3002,7 @@ int shci_spmm(shci_ctx* ctx, const double* X, double* Y, uint32_t k, uint32_t fl
   const hipStream_t st = ctx->stream;
   // The segment layout's block kernels stage 4 vectors of x per member of a
   // one-wavefront segment in LDS (64 KiB: m <= 4096).
-  const uint32_t seg_big = std::max({ctx->seg_maxm[0], ctx->seg_maxm[1], ctx->seg_maxm[2]});
+  const uint32_t seg_big = *std::max_element(ctx->seg_maxm, ctx->seg_maxm + 9);
   const bool block_ok = ctx->segl ? (size_t)seg_big * 4 * sizeof(float) <= 65536 : true;
   if (!(flags & SHCI_FP32) || !has_fp32(ctx) || !block_ok) {
     // FP64 block product as k SpMVs.
@@ -3157,7 +3231,8 @@ int shci_rdm(shci_ctx* ctx, uint32_t n_states, const double* coefs, double* rdm1
     if (ctx->n_seg)
       k_seg_rdm<<<div_up(ctx->n_seg, 4), 256, 0, st>>>(
           R, P<SegDesc>(ctx->seg_desc), P<uint32_t>(ctx->seg_mem), P<uint8_t>(ctx->seg_type),
-          P<uint64_t>(ctx->seg_core), P<double>(ctx->sv64), P<float>(ctx->sv32), ctx->ra, ctx->rb,
+          P<uint64_t>(ctx->seg_core), P<double>(ctx->sv64), P<float>(ctx->sv32), P<uint32_t>(ctx->dnord_idx),
+          ctx->ra, ctx->rb,
           (ctx->build_flags & SHCI_KEEP_ZEROS) ? 1 : 0, ctx->n_seg);
   } else {
     k_rdm<<<div_up(n, 4), 256, 0, st>>>(R);
@@ -3228,7 +3303,7 @@ int shci_clear(shci_ctx* ctx) {
                     &ctx->up_gstart, &ctx->dn_gid, &ctx->dn_gstart, &ctx->dnord_idx, &ctx->dnord_up,
                     &ctx->m1a_id, &ctx->m1b_id, &ctx->dm1_vals, &ctx->dm1_rstart, &ctx->dm1_seg, &ctx->diag,
                     &ctx->mA_id, &ctx->mB_id, &ctx->seg_desc, &ctx->seg_type, &ctx->seg_core, &ctx->seg_mem,
-                    &ctx->sv32, &ctx->sv64,
+                    &ctx->sv32, &ctx->sv64, &ctx->dnpos, &ctx->xb, &ctx->yb,
                     &ctx->cnt, &ctx->row_ptr, &ctx->row_end, &ctx->row_end, &ctx->col, &ctx->v64,
                     &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb, &ctx->va, &ctx->vb, &ctx->xv,
                     &ctx->yv, &ctx->x32, &ctx->y32, &ctx->dav};
@@ -3347,7 +3422,7 @@ int shci_layout_stats(shci_ctx* ctx, uint64_t* stats) {
   stats[3] = ctx->segl ? 1 : 0;
   stats[4] = ctx->n_seg;
   stats[5] = ctx->n_mem;
-  stats[6] = ctx->seg_cs[3];  // segments with more than 64 members
+  stats[6] = ctx->seg_cs[9];  // segments with more than 64 members
   stats[7] = ctx->segl ? ctx->n_slots : 0;
   stats[8] = ctx->n_dm1_runs;
   stats[9] = ctx->n_ugroups;
