@@ -213,24 +213,52 @@ __device__ double single_elem(const Ints& I, uint64_t si, uint64_t sj, uint64_t 
   const double* D = I.tD + (size_t)(p * n + q) * n;
   const double* X = I.tX + (size_t)(p * n + q) * n;
   double e = 0.0;
-  uint64_t m = si & ~(1ull << p);
-  uint64_t op = oppo;
-  while (m) {
-    const int k = ctz64(m);
-    m &= m - 1;
-    e -= X[k];
-    const double direct = D[k];
-    if ((op >> k) & 1ull) {
-      op &= ~(1ull << k);
-      e += 2 * direct;
-    } else {
-      e += direct;
+  // Same-spin orbitals k != p in ascending order, then the opposite-spin
+  // orbitals not among them, four at a time: the table loads of a group are
+  // issued before its additions, which keep the reference's order.
+  const uint64_t m0 = si & ~(1ull << p);
+  for (uint64_t m = m0; m;) {
+    int k[4];
+    bool v[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      v[t] = m != 0;
+      k[t] = v[t] ? ctz64(m) : 0;
+      m &= m - 1;
+    }
+    double xv[4], dv[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      xv[t] = X[k[t]];
+      dv[t] = D[k[t]];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      if (v[t]) {
+        e -= xv[t];
+        if ((oppo >> k[t]) & 1ull) {
+          e += 2 * dv[t];
+        } else {
+          e += dv[t];
+        }
+      }
     }
   }
-  while (op) {
-    const int k = ctz64(op);
-    op &= op - 1;
-    e += D[k];
+  for (uint64_t op = oppo & ~m0; op;) {
+    int k[4];
+    bool v[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      v[t] = op != 0;
+      k[t] = v[t] ? ctz64(op) : 0;
+      op &= op - 1;
+    }
+    double dv[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) dv[t] = D[k[t]];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+      if (v[t]) e += dv[t];
   }
   e *= pf;
   return one + e;
