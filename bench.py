@@ -106,7 +106,7 @@ def autograd_bench(w, args, n_dirs=20, step=1e-5, seed=2024):
     e.backward()
     t_bwd = time.time() - t0
     gh, gv = h.grad.numpy(), v.grad.numpy()
-    log(f"[autograd] {len(w.up):,} dets: E {float(e):.10f}, forward {t_fwd:.2f}s (residual {solver.residual:.1e}), "
+    log(f"[autograd] {len(w.up):,} dets: E {e.item():.10f}, forward {t_fwd:.2f}s (residual {solver.residual:.1e}), "
         f"backward {t_bwd:.2f}s")
     coefs0 = solver.coefs.copy()
     rng = np.random.default_rng(seed)
@@ -132,7 +132,7 @@ def autograd_bench(w, args, n_dirs=20, step=1e-5, seed=2024):
         "config": {"workload": f"cfg{args.config}: synthetic n_orb={w.n}, n_up=n_dn={w.spec.n_up}, "
                                f"{len(w.up)} dets, {args.layout} layout, {n_dirs} seeded directions, step {step:g}",
                    "n_dets": len(w.up)},
-        "forward_s": t_fwd, "backward_s": t_bwd, "fd_s": time.time() - t_fd, "energy": float(e),
+        "forward_s": t_fwd, "backward_s": t_bwd, "fd_s": time.time() - t_fd, "energy": e.item(),
         "residual": solver.residual, "tolerance": 1e-6,
     }
     print(json.dumps(out))

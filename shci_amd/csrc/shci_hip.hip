@@ -1754,6 +1754,99 @@ __global__ __launch_bounds__(64) void k_seg_spmv_large(const SegDesc* __restrict
   }
 }
 
+// ---- Block product: 4 interleaved FP32 vectors (x[det * 4 + v]) ----
+// Segments of m <= S members (S <= 16): a group of 4S lanes per segment, lane
+// 4j + v holding vector v of member j.  The four lanes of a member read the
+// same value slot, and their four sums leave in one atomic instruction that
+// touches one cache line.
+template <int S>
+__global__ __launch_bounds__(256) void k_seg_spmm_small(const SegDesc* __restrict__ desc, uint32_t s0, uint32_t s1,
+                                                        const uint32_t* __restrict__ mem, const float* __restrict__ val,
+                                                        const float* __restrict__ x, float* __restrict__ y) {
+  constexpr int L = 4 * S;
+  constexpr int MH = S / 2;
+  const uint64_t s = (uint64_t)s0 + (uint64_t)blockIdx.x * (256 / L) + threadIdx.x / L;
+  const uint32_t l = threadIdx.x & (L - 1), j = l >> 2, v = l & 3;
+  const int gbase = (int)(lane_id() & ~(uint32_t)(L - 1));
+  SegDesc D{0u, 0u, 0ull};
+  if (s < s1) D = desc[s];
+  const uint32_t m = D.m, mh = m >> 1;
+  const bool valid = j < m;
+  const uint32_t det = mem[valid ? D.moff + j : 0];
+  const float xj = x[(uint64_t)det * 4 + v] * (valid ? 1.f : 0.f);
+  float acc = 0.f;
+  uint32_t p = j, q = j;
+#pragma unroll
+  for (int d = 1; d <= MH; d++) {
+    const bool live = valid && (uint32_t)d <= mh;
+    const uint32_t vj = (2 * (uint32_t)d == m && j >= mh) ? j - mh : j;
+    const uint64_t off = live ? (uint64_t)(d - 1) * m + vj : 0;
+    const float w = val[D.voff + off] * (live ? 1.f : 0.f);
+    const uint32_t pn = p + 1 == m ? 0 : p + 1, qn = q == 0 ? m - 1 : q - 1;
+    p = valid ? pn : p;
+    q = valid ? qn : q;
+    const float wbr = __shfl(w, gbase + (int)(4 * q + v), kWave);
+    const float wb = 2 * (uint32_t)d == m ? 0.f : wbr;
+    const float xp = __shfl(xj, gbase + (int)(4 * p + v), kWave);
+    const float xq = __shfl(xj, gbase + (int)(4 * q + v), kWave);
+    acc += w * xp + wb * xq;
+  }
+  if (valid) atomicAdd(&y[(uint64_t)det * 4 + v], acc);
+}
+
+// Segments of m > 16 members: one wavefront per (segment, 64 members), x of
+// all members and vectors in LDS, lane 4 jl + v holding vector v of the
+// members j0 + jl + 16 c (c < 4); steps outermost as in k_seg_spmv_large.
+__global__ __launch_bounds__(64) void k_seg_spmm_wave(const SegDesc* __restrict__ desc, uint32_t s0, uint32_t s1,
+                                                      const uint32_t* __restrict__ mem, const float* __restrict__ val,
+                                                      const float* __restrict__ x, float* __restrict__ y) {
+  constexpr int T = 4;
+  extern __shared__ __align__(16) unsigned char seg_lds[];
+  const uint32_t s = s0 + blockIdx.x;
+  if (s >= s1) return;
+  const SegDesc D = desc[s];
+  const uint32_t m = D.m, mh = m >> 1, t = threadIdx.x;
+  const uint32_t j0 = blockIdx.y * 16 * T;
+  if (j0 >= m) return;
+  float* sx = reinterpret_cast<float*>(seg_lds);
+  for (uint32_t e = t; e < 4 * m; e += kWave) sx[e] = x[(uint64_t)mem[D.moff + (e >> 2)] * 4 + (e & 3)];
+  __syncthreads();
+  const uint32_t v = t & 3, jl = t >> 2;
+  const float* __restrict__ vs = val + D.voff;
+  const bool even = (m & 1) == 0;
+  float acc[T];
+  uint32_t jj[T], p[T], q[T];
+#pragma unroll
+  for (int c = 0; c < T; c++) {
+    const uint32_t j = j0 + jl + 16 * c;
+    jj[c] = p[c] = q[c] = j < m ? j : 0;
+    acc[c] = 0.f;
+  }
+  const float* __restrict__ vd = vs;
+  for (uint32_t d = 1; d <= mh; d++, vd += m) {
+    const bool half = even && d == mh;
+    float vf[T], vb[T];
+#pragma unroll
+    for (int c = 0; c < T; c++) {
+      p[c] = p[c] + 1 == m ? 0 : p[c] + 1;
+      q[c] = q[c] == 0 ? m - 1 : q[c] - 1;
+      const uint32_t fi = half && jj[c] >= mh ? jj[c] - mh : jj[c];
+      vf[c] = vd[fi];
+      vb[c] = vd[half ? 0 : q[c]];
+    }
+#pragma unroll
+    for (int c = 0; c < T; c++) {
+      const float wb = half ? 0.f : vb[c];
+      acc[c] += vf[c] * sx[p[c] * 4 + v] + wb * sx[q[c] * 4 + v];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < T; c++) {
+    const uint32_t j = j0 + jl + 16 * c;
+    if (j < m) atomicAdd(&y[(uint64_t)mem[D.moff + j] * 4 + v], acc[c]);
+  }
+}
+
 // ---- RDM over segments: every stored pair once (one wavefront per segment) ----
 __global__ __launch_bounds__(256) void k_seg_rdm(RdmParams R, const SegDesc* __restrict__ desc,
                                                  const uint32_t* __restrict__ mem, const uint8_t* __restrict__ type,
@@ -2331,27 +2424,46 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
     if (s1 == s0) continue;
     const XT* xr = (r % 3 == (int)kSegA) ? xb : x;
     XT* yr = (r % 3 == (int)kSegA) ? yb : y;
-    if (c < 3) {
-      const size_t lds = (size_t)ctx->seg_maxm[r] * NV * sizeof(XT);
-      if (lds > 65536) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment too large for the block product");
-      // one wavefront per (segment, 256 members)
-      const dim3 grid(s1 - s0, div_up(ctx->seg_maxm[r], kWave * kSegChunks));
-      k_seg_spmv_large<kSegChunks, NV, VT, XT><<<grid, kWave, lds, st>>>(desc, s0, s1, mem, val, xr, yr);
-      continue;
-    }
-    const int S = 64 >> (c - 3);
+    if constexpr (NV == 4) {
+      // block product: one lane per (member, vector)
+      const float* vf = reinterpret_cast<const float*>(val);
+      if (c <= 4) {  // m > 16: one wavefront per (segment, 64 members)
+        const uint32_t mx = c < 3 ? ctx->seg_maxm[r] : (c == 3 ? 64u : 32u);
+        const dim3 grid(s1 - s0, div_up(mx, 64));
+        k_seg_spmm_wave<<<grid, kWave, (size_t)mx * 4 * sizeof(float), st>>>(desc, s0, s1, mem, vf, xr, yr);
+      } else {
+        const int S = 64 >> (c - 3);
+        const uint32_t grid = div_up((uint64_t)(s1 - s0) * 4 * S, 256);
+        switch (S) {
+          case 16: k_seg_spmm_small<16><<<grid, 256, 0, st>>>(desc, s0, s1, mem, vf, xr, yr); break;
+          case 8: k_seg_spmm_small<8><<<grid, 256, 0, st>>>(desc, s0, s1, mem, vf, xr, yr); break;
+          case 4: k_seg_spmm_small<4><<<grid, 256, 0, st>>>(desc, s0, s1, mem, vf, xr, yr); break;
+          default: k_seg_spmm_small<2><<<grid, 256, 0, st>>>(desc, s0, s1, mem, vf, xr, yr); break;
+        }
+      }
+    } else {
+      if (c < 3) {
+        const size_t lds = (size_t)ctx->seg_maxm[r] * NV * sizeof(XT);
+        if (lds > 65536) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment too large for the one-wavefront kernel");
+        // one wavefront per (segment, 256 members)
+        const dim3 grid(s1 - s0, div_up(ctx->seg_maxm[r], kWave * kSegChunks));
+        k_seg_spmv_large<kSegChunks, NV, VT, XT><<<grid, kWave, lds, st>>>(desc, s0, s1, mem, val, xr, yr);
+        continue;
+      }
+      const int S = 64 >> (c - 3);
 #define SHCI_SEG_SMALL(SS)                                                                           \
   k_seg_spmv_small<SS, seg_batches<SS>(), NV, VT, XT>                                                \
       <<<div_up((uint64_t)(s1 - s0) * SS, 256 * seg_batches<SS>()), 256, 0, st>>>(desc, s0, s1, mem, val, xr, yr)
-    switch (S) {
-      case 64: SHCI_SEG_SMALL(64); break;
-      case 32: SHCI_SEG_SMALL(32); break;
-      case 16: SHCI_SEG_SMALL(16); break;
-      case 8: SHCI_SEG_SMALL(8); break;
-      case 4: SHCI_SEG_SMALL(4); break;
-      default: SHCI_SEG_SMALL(2); break;
-    }
+      switch (S) {
+        case 64: SHCI_SEG_SMALL(64); break;
+        case 32: SHCI_SEG_SMALL(32); break;
+        case 16: SHCI_SEG_SMALL(16); break;
+        case 8: SHCI_SEG_SMALL(8); break;
+        case 4: SHCI_SEG_SMALL(4); break;
+        default: SHCI_SEG_SMALL(2); break;
+      }
 #undef SHCI_SEG_SMALL
+    }
   }
   if (have_a) k_perm_add<NV, XT><<<div_up(n, 256), 256, 0, st>>>(dnord, yb, y, n);
   HIPC(hipGetLastError());
@@ -3028,7 +3140,7 @@ int shci_spmm(shci_ctx* ctx, const double* X, double* Y, uint32_t k, uint32_t fl
   HIPC(hipSetDevice(ctx->device));
   const uint64_t n = ctx->n;
   const hipStream_t st = ctx->stream;
-  // The segment layout's block kernels stage 4 vectors of x per member of a
+  // The segment layout's block kernels stage the 4 vectors of x of a
   // one-wavefront segment in LDS (64 KiB: m <= 4096).
   const uint32_t seg_big = *std::max_element(ctx->seg_maxm, ctx->seg_maxm + 9);
   const bool block_ok = ctx->segl ? (size_t)seg_big * 4 * sizeof(float) <= 65536 : true;
