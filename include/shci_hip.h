@@ -158,6 +158,13 @@ int shci_time_spmm(shci_ctx* ctx, uint32_t k, int reps, float* ms_avg, double* b
  * search, alpha groups, beta groups, CSR SpMV lanes per row. */
 int shci_layout_stats(shci_ctx* ctx, uint64_t* stats);
 
+/* Segment-layout ranges (no reference counterpart; diagnostics): for each of
+ * the 27 (size class, type) ranges r = 3 * class + type (classes m > 1024,
+ * > 256, > 64, 33-64, 17-32, 9-16, 5-8, 3-4, 2; types C, A, B),
+ * out[3 * r] = segments, out[3 * r + 1] = memberships, out[3 * r + 2] =
+ * value slots.  SHCI_ERR_STATE unless the segment layout is built. */
+int shci_segment_ranges(shci_ctx* ctx, uint64_t* out);
+
 /* Device memory in use by the context (bytes). */
 int shci_memory(shci_ctx* ctx, uint64_t* bytes);
 

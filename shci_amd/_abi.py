@@ -43,6 +43,7 @@ EXPORTS = {
     "shci_time_spmm": (C.c_int, [_vp, C.c_uint32, C.c_int, _fp, _dp]),
     "shci_memory": (C.c_int, [_vp, _u64p]),
     "shci_layout_stats": (C.c_int, [_vp, _u64p]),
+    "shci_segment_ranges": (C.c_int, [_vp, _u64p]),
 }
 
 

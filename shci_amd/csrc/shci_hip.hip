@@ -1539,6 +1539,76 @@ __global__ __launch_bounds__(256) void k_seg_fill_small(SegFillParams F) {
   seg_count(F, kept);
 }
 
+// Mixed-core (C) segments at build time: every owned pair is an opposite-spin
+// double whose value is sign * (pu qu | pd qd) (chem_system.cc:599-607, as in
+// double_mixed).  Four steps per iteration: the partner dets, ownership tests,
+// signs and integral indices first, then four unconditional integral loads.
+template <int S>
+__global__ __launch_bounds__(256) void k_seg_fill_c(SegFillParams F) {
+  constexpr int U = 4;
+  const uint64_t s = (uint64_t)F.s0 + (uint64_t)blockIdx.x * (256 / S) + threadIdx.x / S;
+  const uint32_t j = threadIdx.x & (S - 1);
+  const int gbase = (int)(lane_id() & ~(uint32_t)(S - 1));
+  SegDesc D{0u, 0u, 0ull};
+  if (s < F.s1) D = F.desc[s];
+  const uint32_t m = D.m, mh = m >> 1;
+  const bool valid = j < m;
+  uint64_t uj = 0, dj = 0;
+  if (valid) {
+    const uint32_t det = F.mem[D.moff + j];
+    uj = F.up[det];
+    dj = F.dn[det];
+  }
+  // Steps of the wave's largest segment (segments of a range are in group order).
+  uint32_t steps = mh;
+  for (int o = 32; o > 0; o >>= 1) steps = max(steps, (uint32_t)__shfl_xor((int)steps, o, kWave));
+  uint32_t kept = 0, p = j;
+  for (uint32_t d0 = 1; d0 <= steps; d0 += U) {
+    uint64_t idx[U];
+    double sgn[U];
+    bool live[U], own[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t d = d0 + u;
+      if (valid) p = p + 1 == m ? 0 : p + 1;
+      const uint64_t ul = shfl64(uj, gbase + (int)p), dl = shfl64(dj, gbase + (int)p);
+      live[u] = valid && d <= mh && (2 * d != m || j < mh);
+      own[u] = live[u] && uj != ul && dj != dl;
+      idx[u] = 0;
+      sgn[u] = 1.0;
+      if (own[u]) {
+        const uint64_t lu = uj & ~ul, ru = ul & ~uj, ld = dj & ~dl, rd = dl & ~dj;
+        const int pu = ctz64(lu), qu = ctz64(ru), pd = ctz64(ld), qd = ctz64(rd);
+        const int cu = below(uj, pu) + below(ul, qu);
+        const int cd = below(dj, pd) + below(dl, qd);
+        const int pf = ((cu & 1) ? -1 : 1) * ((cd & 1) ? -1 : 1);
+        idx[u] = c2(c2(pu, qu), c2(pd, qd));
+        sgn[u] = (double)pf;
+      }
+    }
+    double e[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) e[u] = F.I.eri[idx[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (!live[u]) continue;
+      double v = 0.0;
+      bool keep = false;
+      if (own[u]) {
+        v = e[u];
+        v *= sgn[u];
+        keep = F.keep_zeros || fabs(v) >= kEps;
+        if (!keep) v = 0.0;
+      }
+      const uint64_t slot = D.voff + (uint64_t)(d0 + u - 1) * m + j;
+      if (F.v64) F.v64[slot] = v;
+      if (F.v32) F.v32[slot] = (float)v;
+      kept += keep ? 1u : 0u;
+    }
+  }
+  seg_count(F, kept);
+}
+
 // One wavefront per segment (m > 64), member dets staged in LDS.
 __global__ __launch_bounds__(64) void k_seg_fill_large(SegFillParams F) {
   extern __shared__ __align__(16) unsigned char seg_lds[];
@@ -1593,90 +1663,108 @@ __device__ __forceinline__ void load_x(const XT* __restrict__ x, uint32_t det, X
   }
 }
 
+// Value of the lane whose byte address (4 * lane) is a (ds_bpermute).  All
+// lanes of the wave must be active.
+__device__ __forceinline__ float lane_read(float v, int a) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(a, __float_as_int(v)));
+}
+__device__ __forceinline__ double lane_read(double v, int a) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)(uint32_t)u);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)(uint32_t)(u >> 32));
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // Off-diagonal part, lane groups of S lanes, each group handling NB segments
-// of m <= S members at once.  Per segment the chain descriptor -> member id ->
-// x gather is a few dependent memory latencies; NB independent chains per
-// lane keep that many in flight.  Loads are unconditional at in-bounds
-// (clamped) positions and masked by multiplication; value slots are issued
-// CH = 4 steps at a time.  VT stored value type, XT vector (and accumulation)
-// type.
-template <int S, int NB, int NV, typename VT, typename XT>
+// of m <= S members at once (one vector).  Lane j holds member j.  At step d
+// it loads the value v of the pair (j, j + d mod m), adds v * x_{j+d} to its
+// own sum and v * x_j to a travelling sum R.  x_{j+d} and R both move one lane
+// down the segment per step (lane j reads lane j + 1 mod m, a fixed address),
+// so after T steps lane j's R holds what it owes member j + T mod m; one last
+// cross-lane read delivers it.  The last step of an even m (m/2 slots, pair
+// (j, j + m/2) read by both of its lanes) adds the forward term only.  The
+// step loop runs to the wave's longest segment; lanes past their own steps
+// re-read their last slot and add zero.  VT stored value type, XT vector and
+// accumulation type.
+template <int S, int NB, typename VT, typename XT>
 __global__ __launch_bounds__(256) void k_seg_spmv_small(const SegDesc* __restrict__ desc, uint32_t s0, uint32_t s1,
                                                         const uint32_t* __restrict__ mem, const VT* __restrict__ val,
                                                         const XT* __restrict__ x, XT* __restrict__ y) {
-  constexpr int MH = S / 2;
-  constexpr int CH = MH < 4 ? MH : 4;
-  constexpr int GPB = 256 / S;  // segments per block and batch
+  constexpr int CH = S / 2 < 4 ? S / 2 : 4;  // steps per batch of value loads
+  constexpr int GPB = 256 / S;                // segments per block and batch
   const uint32_t j = threadIdx.x & (S - 1);
-  const int gbase = (int)(lane_id() & ~(uint32_t)(S - 1));
+  const uint32_t gbase = lane_id() & ~(uint32_t)(S - 1);
   const uint64_t sbase = (uint64_t)s0 + (uint64_t)blockIdx.x * (NB * GPB) + threadIdx.x / S;
-  SegDesc D[NB];
+  using gptr = const __attribute__((address_space(1))) VT*;  // global loads, not flat
+  uint32_t m[NB], full[NB], det[NB];
+  uint64_t voff[NB];
+  int nxt[NB];
+  gptr vp[NB];
+  XT xj[NB], xp[NB], acc[NB], R[NB];
+  uint32_t steps = 0;
 #pragma unroll
   for (int b = 0; b < NB; b++) {
     const uint64_t sb = sbase + (uint64_t)b * GPB;
-    D[b] = SegDesc{0u, 0u, 0ull};
-    if (sb < s1) D[b] = desc[sb];
+    SegDesc D{0u, 0u, 0ull};
+    if (sb < s1) D = desc[sb];
+    m[b] = D.m;
+    voff[b] = D.voff;
+    const bool valid = j < D.m;
+    det[b] = mem[valid ? D.moff + j : 0];
+    const XT xv = x[det[b]];
+    xj[b] = valid ? xv : (XT)0;
+    full[b] = valid ? (D.m - 1) / 2 : 0;  // steps with m slots
+    const uint32_t jn = j + 1 == D.m ? 0 : j + 1;
+    nxt[b] = (int)((gbase + (valid ? jn : j)) << 2);
+    vp[b] = (gptr)(val + D.voff + (full[b] ? j : 0));
+    xp[b] = xj[b];
+    acc[b] = (XT)0;
+    R[b] = (XT)0;
+    steps = max(steps, full[b]);
   }
-  bool valid[NB];
-  uint32_t det[NB];
-#pragma unroll
-  for (int b = 0; b < NB; b++) {
-    valid[b] = j < D[b].m;
-    det[b] = mem[valid[b] ? D[b].moff + j : 0];
-  }
-  XT xj[NB][NV], acc[NB][NV];
-#pragma unroll
-  for (int b = 0; b < NB; b++) {
-    load_x<NV, XT>(x, det[b], xj[b]);
-    for (int k = 0; k < NV; k++) {
-      xj[b][k] = valid[b] ? xj[b][k] : (XT)0;
-      acc[b][k] = (XT)0;
-    }
-  }
-  uint32_t p[NB], q[NB];  // (j + d) mod m, (j - d) mod m for the members
-#pragma unroll
-  for (int b = 0; b < NB; b++) p[b] = q[b] = j;
+  for (int o = 32; o > 0; o >>= 1) steps = max(steps, (uint32_t)__shfl_xor((int)steps, o, kWave));
+  steps = (uint32_t)__builtin_amdgcn_readfirstlane((int)steps);
+  uint32_t T = 0;  // rotations done
 #pragma unroll 1
-  for (int c0 = 1; c0 <= MH; c0 += CH) {
+  for (uint32_t c0 = 1; c0 <= steps; c0 += CH, T += CH) {
     XT v[NB][CH];
 #pragma unroll
     for (int b = 0; b < NB; b++) {
-      const uint32_t m = D[b].m, mh = m >> 1;
+      bool live = c0 <= full[b];
 #pragma unroll
       for (int i = 0; i < CH; i++) {
-        const uint32_t d = (uint32_t)(c0 + i);
-        const bool live = valid[b] && d <= mh;
-        const uint32_t vj = (2 * d == m && j >= mh) ? j - mh : j;
-        const uint64_t off = live ? (uint64_t)(d - 1) * m + vj : 0;
-        v[b][i] = (XT)__builtin_nontemporal_load(val + D[b].voff + off) * (live ? (XT)1 : (XT)0);
+        const VT w = __builtin_nontemporal_load(vp[b]);
+        const bool next = c0 + i + 1 <= full[b];
+        v[b][i] = live ? (XT)w : (XT)0;
+        vp[b] += next ? m[b] : 0u;  // stays on the last slot read
+        live = next;
       }
     }
 #pragma unroll
     for (int i = 0; i < CH; i++) {
-      const uint32_t d = (uint32_t)(c0 + i);
 #pragma unroll
       for (int b = 0; b < NB; b++) {
-        const uint32_t m = D[b].m;
-        const uint32_t pn = p[b] + 1 == m ? 0 : p[b] + 1, qn = q[b] == 0 ? m - 1 : q[b] - 1;
-        p[b] = valid[b] ? pn : p[b];
-        q[b] = valid[b] ? qn : q[b];
-        // Backward value: the slot of the pair (q, j), held by lane q.  In the
-        // last step of an even m both directions share one slot.
-        const XT vbr = __shfl(v[b][i], gbase + (int)q[b], kWave);
-        const XT vb = 2 * d == m ? (XT)0 : vbr;
-#pragma unroll
-        for (int k = 0; k < NV; k++) {
-          const XT xp = __shfl(xj[b][k], gbase + (int)p[b], kWave);
-          const XT xq = __shfl(xj[b][k], gbase + (int)q[b], kWave);
-          acc[b][k] += v[b][i] * xp + vb * xq;
-        }
+        xp[b] = lane_read(xp[b], nxt[b]);
+        R[b] = lane_read(R[b], nxt[b]);
+        acc[b] = fma(v[b][i], xp[b], acc[b]);
+        R[b] = fma(v[b][i], xj[b], R[b]);
       }
     }
   }
 #pragma unroll
-  for (int b = 0; b < NB; b++)
-    if (valid[b])
-      for (int k = 0; k < NV; k++) atomicAdd(&y[(uint64_t)det[b] * NV + k], acc[b][k]);
+  for (int b = 0; b < NB; b++) {
+    const uint32_t mb = m[b], mh = mb >> 1, mm = mb ? mb : 1u;
+    const bool valid = j < mb;
+    const bool half = valid && (mb & 1u) == 0;
+    const uint32_t hj = j < mh ? j : j - mh;
+    const VT w = val[voff[b] + (half ? (uint64_t)(mh - 1) * mb + hj : 0ull)];
+    const uint32_t pj = j + mh >= mb ? j + mh - mb : j + mh;
+    const XT xh = lane_read(xj[b], (int)((gbase + (valid ? pj : j)) << 2));
+    acc[b] = fma(half ? (XT)w : (XT)0, xh, acc[b]);
+    const uint32_t src = valid ? (j + mm - T % mm) % mm : j;
+    const XT rb = lane_read(R[b], (int)((gbase + src) << 2));
+    if (valid) atomicAdd(&y[det[b]], acc[b] + rb);
+  }
 }
 
 // Segments per lane group of the class kernels (see k_seg_spmv_small).
@@ -2355,6 +2443,17 @@ int seg_fill(shci_ctx* ctx, int update) {
     }
     const int S = 64 >> (c - 3);
     const uint32_t grid = div_up((uint64_t)cnt * S, 256);
+    if (!update && r % 3 == (int)kSegC) {
+      switch (S) {
+        case 64: k_seg_fill_c<64><<<grid, 256, 0, st>>>(F); break;
+        case 32: k_seg_fill_c<32><<<grid, 256, 0, st>>>(F); break;
+        case 16: k_seg_fill_c<16><<<grid, 256, 0, st>>>(F); break;
+        case 8: k_seg_fill_c<8><<<grid, 256, 0, st>>>(F); break;
+        case 4: k_seg_fill_c<4><<<grid, 256, 0, st>>>(F); break;
+        default: k_seg_fill_c<2><<<grid, 256, 0, st>>>(F); break;
+      }
+      continue;
+    }
     switch (S) {
       case 64: k_seg_fill_small<64><<<grid, 256, 0, st>>>(F); break;
       case 32: k_seg_fill_small<32><<<grid, 256, 0, st>>>(F); break;
@@ -2452,7 +2551,7 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
       }
       const int S = 64 >> (c - 3);
 #define SHCI_SEG_SMALL(SS)                                                                           \
-  k_seg_spmv_small<SS, seg_batches<SS>(), NV, VT, XT>                                                \
+  k_seg_spmv_small<SS, seg_batches<SS>(), VT, XT>                                                    \
       <<<div_up((uint64_t)(s1 - s0) * SS, 256 * seg_batches<SS>()), 256, 0, st>>>(desc, s0, s1, mem, val, xr, yr)
       switch (S) {
         case 64: SHCI_SEG_SMALL(64); break;
@@ -3568,6 +3667,32 @@ int shci_layout_stats(shci_ctx* ctx, uint64_t* stats) {
   stats[9] = ctx->n_ugroups;
   stats[10] = ctx->n_dgroups;
   stats[11] = (uint64_t)ctx->lanes;
+  return SHCI_OK;
+}
+
+int shci_segment_ranges(shci_ctx* ctx, uint64_t* out) {
+  if (!ctx || !out) return SHCI_ERR_ARG;
+  if (!ctx->segl || !ctx->seg_desc.p) return fail(ctx, SHCI_ERR_STATE, "segment layout not built");
+  // members and slots are laid out in segment order: range totals are
+  // differences of the first descriptors of consecutive ranges
+  uint64_t moff[kSegRanges + 1], voff[kSegRanges + 1];
+  for (int r = 0; r <= kSegRanges; r++) {
+    if (ctx->seg_cs[r] >= ctx->n_seg) {
+      moff[r] = ctx->n_mem;
+      voff[r] = ctx->n_slots;
+      continue;
+    }
+    SegDesc D;
+    HIPC(hipMemcpyAsync(&D, P<SegDesc>(ctx->seg_desc) + ctx->seg_cs[r], sizeof(D), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    moff[r] = D.moff;
+    voff[r] = D.voff;
+  }
+  for (int r = 0; r < kSegRanges; r++) {
+    out[3 * r] = ctx->seg_cs[r + 1] - ctx->seg_cs[r];
+    out[3 * r + 1] = moff[r + 1] - moff[r];
+    out[3 * r + 2] = voff[r + 1] - voff[r];
+  }
   return SHCI_OK;
 }
 

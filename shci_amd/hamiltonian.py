@@ -180,6 +180,15 @@ class SparseMatrix:
                  "value_slots", "mixed_runs", "alpha_groups", "beta_groups", "lanes"]
         return dict(zip(names, list(st)))
 
+    def segment_ranges(self):
+        """Per (size class, type) range of the segment layout: a list of 27
+        dicts (class, type, segments, memberships, slots); class 0..8 = m > 1024,
+        > 256, > 64, 33-64, 17-32, 9-16, 5-8, 3-4, 2; type C, A, B."""
+        out = (C.c_uint64 * 81)()
+        self.ctx.check(self.ctx.lib.shci_segment_ranges(self.ctx.handle, out), "shci_segment_ranges")
+        return [{"class": r // 3, "type": "CAB"[r % 3], "segments": out[3 * r], "memberships": out[3 * r + 1],
+                 "slots": out[3 * r + 2]} for r in range(27)]
+
     def build_timings(self):
         t = (C.c_float * 8)()
         self.ctx.check(self.ctx.lib.shci_get_build_timings(self.ctx.handle, t), "shci_get_build_timings")

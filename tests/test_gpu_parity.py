@@ -180,6 +180,20 @@ def test_segments_counts_and_update_existing(system, keep_zeros):
         ham.update_existing_elems()
         ys[layout] = ham.matrix.mul(x)
         ys[layout + "32"] = ham.matrix.mul(x, fp32=True)
+        if layout == "segments":
+            st = ham.matrix.layout_stats()
+            rg = ham.matrix.segment_ranges()
+            # the ranges partition the segments, memberships and slots; a
+            # segment of m members keeps m(m-1)/2 slots and sits in its class
+            assert sum(r["segments"] for r in rg) == st["segments"]
+            assert sum(r["memberships"] for r in rg) == st["memberships"]
+            assert sum(r["slots"] for r in rg) == st["value_slots"]
+            lo = [1025, 257, 65, 33, 17, 9, 5, 3, 2]
+            hi = [4096, 1024, 256, 64, 32, 16, 8, 4, 2]
+            for r in rg:
+                c, ns, nm = r["class"], r["segments"], r["memberships"]
+                assert lo[c] * ns <= nm <= hi[c] * ns
+                assert (nm * (lo[c] - 1)) // 2 <= r["slots"] <= (nm * (hi[c] - 1)) // 2
         ham.clear()
     if not keep_zeros:
         assert ys["csr_nnz"] == s["orc"].stats()["nnz"]
