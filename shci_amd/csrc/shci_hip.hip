@@ -111,6 +111,14 @@ struct shci_ctx {
   DevBuf seg_desc, seg_type, seg_core, seg_mem, sv32, sv64;
   DevBuf dnpos;                         // internal det -> position in beta order
   DevBuf xb, yb;                        // beta-order x and partial y of the A segments
+  // Tiles of the segments with m <= 64: runs of one spin group's segments
+  // whose x and y a workgroup keeps in LDS (see k_seg_spmv_tiles).
+  uint64_t n_tiles = 0;
+  DevBuf tile_rb, tile_re;              // [tile * 6 + class - 3] -> segment range of the class run
+  DevBuf tile_u0, tile_nu;              // first entry and size of the tile's det table
+  DevBuf tile_tab;                      // tile det tables (internal det ids)
+  DevBuf seg_loc;                       // membership -> index in its tile's table (m <= 64 only)
+  DevBuf kept;                          // kept-pair counters of the segment fill
 
   hipEvent_t ev[9] = {};
   float phase_ms[8] = {};
@@ -1315,10 +1323,10 @@ __global__ void k_seg_keys(SegKeyParams K, uint64_t total, uint64_t* __restrict_
   if (type == kSegC) {
     const uint32_t k1 = k / K.nd, k2 = k - k1 * K.nd;
     payload = ((uint64_t)K.m1a[(uint64_t)K.up_gid[i] * K.nu + k1] << K.bC) | K.m1b[(uint64_t)K.dn_gid[i] * K.nd + k2];
-  } else if (type == kSegA) {
-    payload = ((uint64_t)K.mA[(uint64_t)K.up_gid[i] * K.PA + k] << K.bA) | K.dn_gid[i];
+  } else if (type == kSegA) {  // spin group in the high field: runs of one beta group are adjacent
+    payload = ((uint64_t)K.dn_gid[i] << K.bA) | K.mA[(uint64_t)K.up_gid[i] * K.PA + k];
   } else {
-    payload = ((uint64_t)K.mB[(uint64_t)K.dn_gid[i] * K.PB + k] << K.bB) | K.up_gid[i];
+    payload = ((uint64_t)K.up_gid[i] << K.bB) | K.mB[(uint64_t)K.dn_gid[i] * K.PB + k];
   }
   keys[t] = ((uint64_t)type << K.tag_shift) | payload;
   vals[t] = (uint32_t)t;
@@ -1378,14 +1386,80 @@ __global__ void k_seg_keep(const uint32_t* __restrict__ rstart, const uint32_t* 
   keep[r] = run_kept(useinc, rstart, (uint32_t)r, before) >= 2 ? 1u : 0u;
 }
 
+// Tiles.  The kept runs come out of the key sort in (type, spin group, core)
+// order: C runs grouped by their alpha - 1 core, A runs by their beta string,
+// B runs by their alpha string.  The segments of one group share members (a
+// det sits in up to n_beta C segments of one alpha - 1 core and in up to
+// C(n_alpha, 2) A segments of its beta string), so a workgroup that keeps
+// the group's x and y in LDS gathers and updates each det once instead of
+// once per segment.  A group's segments with m <= 64 are cut into tiles of
+// fewer than kTileMax memberships: a segment starts a new tile when the
+// memberships before it in its group reach the next multiple of kTileCap.
+constexpr uint32_t kTileCap = 1984;
+constexpr uint32_t kTileMax = 2048;   // > kTileCap + 64: memberships (and dets) of one tile
+constexpr uint32_t kTileHash = 4096;  // LDS hash slots of the table build
+
+// Per kept run (index f in run order): spin-group key and the memberships it
+// brings to its tile (m <= 64) or 0 (larger segments have no tile).
+__global__ void k_seg_groups(SegKeyParams K, const uint32_t* __restrict__ vals_s, const uint32_t* __restrict__ rstart,
+                             const uint32_t* __restrict__ useinc, const uint32_t* __restrict__ keep,
+                             const uint32_t* __restrict__ sidx, uint64_t R, uint64_t* __restrict__ gkey,
+                             uint32_t* __restrict__ w) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R || !keep[r]) return;
+  uint32_t before;
+  const uint32_t m = run_kept(useinc, rstart, (uint32_t)r, before);
+  uint32_t ty, k;
+  uint64_t i;
+  seg_decode(K, vals_s[rstart[r]], ty, i, k);
+  uint64_t g;
+  if (ty == kSegC) g = K.m1a[(uint64_t)K.up_gid[i] * K.nu + k / K.nd];
+  else if (ty == kSegA) g = K.dn_gid[i];
+  else g = K.up_gid[i];
+  const uint32_t f = sidx[r];
+  gkey[f] = ((uint64_t)ty << 32) | g;
+  w[f] = m <= 64 ? m : 0u;
+}
+
+// gflag[f] = run f opens a spin group.
+__global__ void k_seg_gflags(const uint64_t* __restrict__ gkey, uint64_t S, uint32_t* __restrict__ gflag) {
+  const uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= S) return;
+  gflag[f] = (f == 0 || gkey[f] != gkey[f - 1]) ? 1u : 0u;
+}
+
+// gW[group] = memberships of the tiles before the group (W = inclusive scan of w).
+__global__ void k_seg_gstart(const uint32_t* __restrict__ gflag, const uint32_t* __restrict__ gord,
+                             const uint32_t* __restrict__ W, const uint32_t* __restrict__ w, uint64_t S,
+                             uint32_t* __restrict__ gW) {
+  const uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= S || !gflag[f]) return;
+  gW[gord[f] - 1] = W[f] - w[f];
+}
+
+// tflag[f] = run f opens a tile.
+__global__ void k_seg_tflags(const uint32_t* __restrict__ gflag, const uint32_t* __restrict__ gord,
+                             const uint32_t* __restrict__ W, const uint32_t* __restrict__ w,
+                             const uint32_t* __restrict__ gW, uint64_t S, uint32_t* __restrict__ tflag) {
+  const uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= S) return;
+  const uint32_t piece = (W[f] - w[f] - gW[gord[f] - 1]) / kTileCap;
+  uint32_t open = gflag[f];
+  if (!open) {
+    const uint32_t prev = (W[f - 1] - w[f - 1] - gW[gord[f - 1] - 1]) / kTileCap;
+    open = piece != prev ? 1u : 0u;
+  }
+  tflag[f] = open;
+}
+
 // Order key of a kept run: size class, type, then (lane-group classes) the
-// alpha group (C, B) or beta group (A) of its first member, so that the
-// segments sharing a wavefront touch neighbouring x and y entries; within the
-// one-wavefront classes, largest first.
+// tile, so that each class range lists the tiles' runs in tile order; largest
+// first within a tile and within the one-wavefront classes.
 __global__ void k_seg_compact(SegKeyParams K, const uint32_t* __restrict__ vals_s,
                               const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ useinc,
-                              const uint32_t* __restrict__ keep, const uint32_t* __restrict__ sidx, uint64_t R,
-                              uint64_t* __restrict__ skey, uint32_t* __restrict__ srun) {
+                              const uint32_t* __restrict__ keep, const uint32_t* __restrict__ sidx,
+                              const uint32_t* __restrict__ tileinc, uint64_t R, uint64_t* __restrict__ skey,
+                              uint32_t* __restrict__ srun) {
   const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R || !keep[r]) return;
   uint32_t before;
@@ -1394,34 +1468,147 @@ __global__ void k_seg_compact(SegKeyParams K, const uint32_t* __restrict__ vals_
   uint64_t i;
   seg_decode(K, vals_s[rstart[r]], ty, i, k);
   const uint64_t cls = seg_class(m);
-  const uint64_t grp = ty == kSegA ? K.dn_gid[i] : K.up_gid[i];
-  const uint64_t low = cls < 3 ? (uint64_t)(kSegMaxM - m) : ((grp << 13) | (uint64_t)(kSegMaxM - m));
-  skey[sidx[r]] = (cls << 60) | ((uint64_t)ty << 58) | low;
-  srun[sidx[r]] = (uint32_t)r;
+  const uint32_t f = sidx[r];
+  const uint64_t tile = tileinc[f] - 1;
+  const uint64_t low = cls < 3 ? (uint64_t)(kSegMaxM - m) : ((tile << 13) | (uint64_t)(kSegMaxM - m));
+  skey[f] = (cls << 60) | ((uint64_t)ty << 58) | low;
+  srun[f] = (uint32_t)r;
 }
 
-// Ordered segment f (largest first): sizes, slot counts, run -> segment map,
-// class histogram.
-__global__ void k_seg_sizes(const uint32_t* __restrict__ srun, const uint64_t* __restrict__ skey,
-                            const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ useinc, uint64_t S,
-                            uint64_t* __restrict__ msz, uint64_t* __restrict__ nslot, uint32_t* __restrict__ finv,
-                            unsigned int* __restrict__ hist) {
-  __shared__ unsigned int h[kSegRanges];
-  if (threadIdx.x < kSegRanges) h[threadIdx.x] = 0;
-  __syncthreads();
-  const uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f < S) {
-    const uint32_t r = srun[f];
-    uint32_t before;
-    const uint64_t m = run_kept(useinc, rstart, r, before);
-    msz[f] = m;
-    nslot[f] = m * (m - 1) / 2;
-    finv[r] = (uint32_t)f;
-    const uint64_t key = skey[f];
-    atomicAdd(&h[(uint32_t)(key >> 60) * 3 + (uint32_t)((key >> 58) & 3)], 1u);
+// Class runs of the tiles: rb/re[tile * 6 + class - 3] (zero-initialised;
+// f0 = first segment of the lane-group classes).
+__global__ void k_tile_runs(const uint64_t* __restrict__ skey, uint64_t f0, uint64_t S, uint32_t* __restrict__ rb,
+                            uint32_t* __restrict__ re) {
+  const uint64_t f = f0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= S) return;
+  const uint64_t key = skey[f];
+  const uint64_t run = key >> 13;  // class, type, tile
+  const uint64_t slot = ((key >> 13) & ((1ull << 45) - 1)) * 6 + ((key >> 60) - 3);
+  if (f == f0 || (skey[f - 1] >> 13) != run) rb[slot] = (uint32_t)f;
+  if (f + 1 == S || (skey[f + 1] >> 13) != run) re[slot] = (uint32_t)(f + 1);
+}
+
+// Memberships of each tile.
+__global__ void k_tile_count(const uint32_t* __restrict__ rb, const uint32_t* __restrict__ re,
+                             const SegDesc* __restrict__ desc, uint64_t S, uint64_t n_mem, uint64_t T,
+                             uint32_t* __restrict__ cnt) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  uint32_t c = 0;
+  for (int k = 0; k < 6; k++) {
+    const uint32_t b = rb[t * 6 + k], e = re[t * 6 + k];
+    if (b < e) c += (uint32_t)((e < S ? (uint64_t)desc[e].moff : n_mem) - desc[b].moff);
+  }
+  cnt[t] = c;
+}
+
+__global__ void k_tile_check(const uint32_t* __restrict__ cnt, uint64_t T, unsigned int* __restrict__ bad) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < T && cnt[t] >= kTileMax) atomicOr(bad, 1u);
+}
+
+// One workgroup per tile: the distinct dets of its memberships (LDS hash
+// table with linear probing; at most kTileMax keys in kTileHash slots), their
+// table tab[u0 + i] in slot order and each membership's table index.
+__global__ __launch_bounds__(256) void k_tile_tables(const uint32_t* __restrict__ rb, const uint32_t* __restrict__ re,
+                                                     const uint32_t* __restrict__ u0, const SegDesc* __restrict__ desc,
+                                                     uint64_t S, uint64_t n_mem, const uint32_t* __restrict__ mem,
+                                                     const uint8_t* __restrict__ type,
+                                                     const uint32_t* __restrict__ dnord, uint32_t* __restrict__ tab,
+                                                     uint32_t* __restrict__ nu, uint16_t* __restrict__ loc) {
+  __shared__ uint32_t hk[kTileHash];
+  __shared__ uint16_t cid[kTileHash];
+  __shared__ uint16_t sl[kTileMax];
+  __shared__ uint32_t wtot[4];
+  const uint32_t t = blockIdx.x, tid = threadIdx.x;
+  for (uint32_t h = tid; h < kTileHash; h += 256) hk[h] = 0xFFFFFFFFu;
+  uint64_t mb[6], me[6];
+  bool isA = false;
+  for (int k = 0; k < 6; k++) {
+    const uint32_t b = rb[(uint64_t)t * 6 + k], e = re[(uint64_t)t * 6 + k];
+    mb[k] = me[k] = 0;
+    if (b < e) {
+      mb[k] = desc[b].moff;
+      me[k] = e < S ? (uint64_t)desc[e].moff : n_mem;
+      isA = type[b] == kSegA;
+    }
   }
   __syncthreads();
-  if (threadIdx.x < kSegRanges && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+  uint32_t k0 = 0;
+  for (int k = 0; k < 6; k++) {
+    const uint32_t len = (uint32_t)(me[k] - mb[k]);
+    for (uint32_t i = tid; i < len; i += 256) {
+      uint32_t d = mem[mb[k] + i];
+      if (isA) d = dnord[d];
+      uint32_t h = (d * 2654435761u) >> 20;
+      while (true) {
+        const uint32_t old = atomicCAS(&hk[h], 0xFFFFFFFFu, d);
+        if (old == 0xFFFFFFFFu || old == d) break;
+        h = (h + 1) & (kTileHash - 1);
+      }
+      sl[k0 + i] = (uint16_t)h;
+    }
+    k0 += len;
+  }
+  __syncthreads();
+  // slot order -> table index: each thread counts 16 consecutive slots,
+  // then a block-wide exclusive scan of the counts
+  constexpr int PER = kTileHash / 256;
+  uint32_t c = 0;
+  for (int q = 0; q < PER; q++) c += hk[tid * PER + q] != 0xFFFFFFFFu ? 1u : 0u;
+  uint32_t inc = c;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t v = (uint32_t)__shfl_up((int)inc, o, kWave);
+    if (lane_id() >= (uint32_t)o) inc += v;
+  }
+  if (lane_id() == kWave - 1) wtot[tid / kWave] = inc;
+  __syncthreads();
+  uint32_t base = inc - c;
+  for (uint32_t wv = 0; wv < tid / kWave; wv++) base += wtot[wv];
+  const uint32_t total = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+  for (int q = 0; q < PER; q++) {
+    const uint32_t h = tid * PER + q;
+    if (hk[h] != 0xFFFFFFFFu) {
+      cid[h] = (uint16_t)base;
+      tab[(uint64_t)u0[t] + base] = hk[h];
+      base++;
+    }
+  }
+  if (tid == 0) nu[t] = total;
+  __syncthreads();
+  k0 = 0;
+  for (int k = 0; k < 6; k++) {
+    const uint32_t len = (uint32_t)(me[k] - mb[k]);
+    for (uint32_t i = tid; i < len; i += 256) loc[mb[k] + i] = cid[sl[k0 + i]];
+    k0 += len;
+  }
+}
+
+// Ordered segment f: sizes, slot counts, run -> segment map.
+__global__ void k_seg_sizes(const uint32_t* __restrict__ srun, const uint32_t* __restrict__ rstart,
+                            const uint32_t* __restrict__ useinc, uint64_t S, uint64_t* __restrict__ msz,
+                            uint64_t* __restrict__ nslot, uint32_t* __restrict__ finv) {
+  const uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= S) return;
+  const uint32_t r = srun[f];
+  uint32_t before;
+  const uint64_t m = run_kept(useinc, rstart, r, before);
+  msz[f] = m;
+  nslot[f] = m * (m - 1) / 2;
+  finv[r] = (uint32_t)f;
+}
+
+// First segment of each (class, type) range in the sorted order: bnd[r] for
+// r = 0..kSegRanges (bnd[kSegRanges] = S).  Each entry is written once.
+__global__ void k_seg_bounds(const uint64_t* __restrict__ skey, uint64_t S, uint64_t* __restrict__ bnd) {
+  const uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= S) return;
+  auto range = [](uint64_t key) { return (int)(key >> 60) * 3 + (int)((key >> 58) & 3); };
+  const int rf = range(skey[f]);
+  const int rp = f == 0 ? -1 : range(skey[f - 1]);
+  for (int r = rp + 1; r <= rf; r++) bnd[r] = f;
+  if (f + 1 == S)
+    for (int r = rf + 1; r <= kSegRanges; r++) bnd[r] = S;
 }
 
 __global__ void k_seg_desc(SegKeyParams K, const uint32_t* __restrict__ srun, const uint32_t* __restrict__ rstart,
@@ -1496,14 +1683,21 @@ __device__ __forceinline__ uint32_t seg_slot(const SegFillParams& F, uint32_t ty
       if (!keep) v = 0.0;
     }
   }
-  if (F.v64) F.v64[slot] = v;
-  if (F.v32) F.v32[slot] = (float)v;
+  // streaming stores: the slots are not read again by the fill, and keeping
+  // them out of L2 leaves it to the integral tables
+  if (F.v64) __builtin_nontemporal_store(v, F.v64 + slot);
+  if (F.v32) __builtin_nontemporal_store((float)v, F.v32 + slot);
   return keep ? 1u : 0u;
 }
 
+// Kept-pair counters: one per 128-byte line, a wavefront adds to the line of
+// its workgroup (modulo kKeptLines), so that no single address takes an atomic
+// from every wavefront of the fill.
+constexpr uint32_t kKeptLines = 1024;
+
 __device__ __forceinline__ void seg_count(const SegFillParams& F, uint32_t kept) {
   for (int o = 32; o > 0; o >>= 1) kept += __shfl_xor(kept, o, kWave);
-  if (lane_id() == 0 && kept) atomicAdd(F.kept, (unsigned long long)kept);
+  if (lane_id() == 0 && kept) atomicAdd(F.kept + (blockIdx.x % kKeptLines) * 16, (unsigned long long)kept);
 }
 
 // Lane groups of S lanes, one segment (m <= S) per group.
@@ -1601,8 +1795,8 @@ __global__ __launch_bounds__(256) void k_seg_fill_c(SegFillParams F) {
         if (!keep) v = 0.0;
       }
       const uint64_t slot = D.voff + (uint64_t)(d0 + u - 1) * m + j;
-      if (F.v64) F.v64[slot] = v;
-      if (F.v32) F.v32[slot] = (float)v;
+      if (F.v64) __builtin_nontemporal_store(v, F.v64 + slot);
+      if (F.v32) __builtin_nontemporal_store((float)v, F.v32 + slot);
       kept += keep ? 1u : 0u;
     }
   }
@@ -1634,6 +1828,84 @@ __global__ __launch_bounds__(64) void k_seg_fill_large(SegFillParams F) {
       uint32_t p = j + d;
       if (p >= m) p -= m;
       kept += seg_slot(F, ty, core, D.voff + (uint64_t)(d - 1) * m + j, su[j], sd[j], su[p], sd[p]);
+    }
+  }
+  seg_count(F, kept);
+}
+
+// Mixed-core (C) segments of m > 64 at build time: one wavefront per segment,
+// member strings in LDS, each lane walking its slots (d, j), (d, j + 64), ...
+// four at a time so that four integral loads are in flight.  Values as in
+// k_seg_fill_c.
+__global__ __launch_bounds__(64) void k_seg_fill_large_c(SegFillParams F) {
+  extern __shared__ __align__(16) unsigned char seg_lds[];
+  constexpr int U = 4;
+  const uint32_t s = F.s0 + blockIdx.x;
+  if (s >= F.s1) return;
+  const SegDesc D = F.desc[s];
+  const uint32_t m = D.m, mh = m >> 1, t = threadIdx.x;
+  uint64_t* su = reinterpret_cast<uint64_t*>(seg_lds);
+  uint64_t* sd = su + m;
+  for (uint32_t j = t; j < m; j += kWave) {
+    const uint32_t det = F.mem[D.moff + j];
+    su[j] = F.up[det];
+    sd[j] = F.dn[det];
+  }
+  __syncthreads();
+  uint32_t kept = 0;
+  uint32_t d = 1, j = t, lim = m;  // slots of step d (m > 64, so step 1 has m); j < 64 < lim
+  while (d <= mh) {
+    uint64_t idx[U], slot[U];
+    double sgn[U];
+    bool live[U], own[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      live[u] = d <= mh;
+      own[u] = false;
+      idx[u] = 0;
+      sgn[u] = 1.0;
+      slot[u] = 0;
+      if (live[u]) {
+        uint32_t p = j + d;
+        if (p >= m) p -= m;
+        const uint64_t uj = su[j], dj = sd[j], ul = su[p], dl = sd[p];
+        slot[u] = D.voff + (uint64_t)(d - 1) * m + j;
+        own[u] = uj != ul && dj != dl;
+        if (own[u]) {
+          const uint64_t lu = uj & ~ul, ru = ul & ~uj, ld = dj & ~dl, rd = dl & ~dj;
+          const int pu = ctz64(lu), qu = ctz64(ru), pd = ctz64(ld), qd = ctz64(rd);
+          const int cu = below(uj, pu) + below(ul, qu);
+          const int cd = below(dj, pd) + below(dl, qd);
+          const int pf = ((cu & 1) ? -1 : 1) * ((cd & 1) ? -1 : 1);
+          idx[u] = c2(c2(pu, qu), c2(pd, qd));
+          sgn[u] = (double)pf;
+        }
+        // next slot of this lane: 64 further in the rotation order
+        j += kWave;
+        while (d <= mh && j >= lim) {
+          j -= lim;
+          d++;
+          lim = 2 * d == m ? mh : m;
+        }
+      }
+    }
+    double e[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) e[u] = F.I.eri[idx[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (!live[u]) continue;
+      double v = 0.0;
+      bool keep = false;
+      if (own[u]) {
+        v = e[u];
+        v *= sgn[u];
+        keep = F.keep_zeros || fabs(v) >= kEps;
+        if (!keep) v = 0.0;
+      }
+      if (F.v64) __builtin_nontemporal_store(v, F.v64 + slot[u]);
+      if (F.v32) __builtin_nontemporal_store((float)v, F.v32 + slot[u]);
+      kept += keep ? 1u : 0u;
     }
   }
   seg_count(F, kept);
@@ -1675,55 +1947,40 @@ __device__ __forceinline__ double lane_read(double v, int a) {
   return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-// Off-diagonal part, lane groups of S lanes, each group handling NB segments
-// of m <= S members at once (one vector).  Lane j holds member j.  At step d
-// it loads the value v of the pair (j, j + d mod m), adds v * x_{j+d} to its
-// own sum and v * x_j to a travelling sum R.  x_{j+d} and R both move one lane
-// down the segment per step (lane j reads lane j + 1 mod m, a fixed address),
-// so after T steps lane j's R holds what it owes member j + T mod m; one last
-// cross-lane read delivers it.  The last step of an even m (m/2 slots, pair
-// (j, j + m/2) read by both of its lanes) adds the forward term only.  The
-// step loop runs to the wave's longest segment; lanes past their own steps
-// re-read their last slot and add zero.  VT stored value type, XT vector and
-// accumulation type.
+// Member sums of NB segments of m <= S members, one lane group of S lanes per
+// segment and lane j holding member j (x_j given; zero past m).  At step d lane
+// j loads the value v of the pair (j, j + d mod m), adds v * x_{j+d} to its own
+// sum and v * x_j to a travelling sum R.  x_{j+d} and R both move one lane down
+// the segment per step (lane j reads lane j + 1 mod m, a fixed address), so
+// after T steps lane j's R holds what it owes member j + T mod m; one last
+// cross-lane read delivers it.  The last step of an even m (m/2 slots, the pair
+// (j, j + m/2) read by both of its lanes) adds the forward term only.  The step
+// loop runs for `steps` steps, the most of any segment of the wave (wave
+// uniform); lanes past their own steps re-read their last slot and add zero.
+// All lanes of the wave must call this.
 template <int S, int NB, typename VT, typename XT>
-__global__ __launch_bounds__(256) void k_seg_spmv_small(const SegDesc* __restrict__ desc, uint32_t s0, uint32_t s1,
-                                                        const uint32_t* __restrict__ mem, const VT* __restrict__ val,
-                                                        const XT* __restrict__ x, XT* __restrict__ y) {
+__device__ __forceinline__ void seg_group_sums(const SegDesc (&D)[NB], const XT (&xj)[NB], const VT* __restrict__ val,
+                                               uint32_t steps, XT (&out)[NB]) {
   constexpr int CH = S / 2 < 4 ? S / 2 : 4;  // steps per batch of value loads
-  constexpr int GPB = 256 / S;                // segments per block and batch
+  using gptr = const __attribute__((address_space(1))) VT*;  // global loads, not flat
   const uint32_t j = threadIdx.x & (S - 1);
   const uint32_t gbase = lane_id() & ~(uint32_t)(S - 1);
-  const uint64_t sbase = (uint64_t)s0 + (uint64_t)blockIdx.x * (NB * GPB) + threadIdx.x / S;
-  using gptr = const __attribute__((address_space(1))) VT*;  // global loads, not flat
-  uint32_t m[NB], full[NB], det[NB];
-  uint64_t voff[NB];
+  uint32_t full[NB];
   int nxt[NB];
   gptr vp[NB];
-  XT xj[NB], xp[NB], acc[NB], R[NB];
-  uint32_t steps = 0;
+  XT xp[NB], acc[NB], R[NB];
 #pragma unroll
   for (int b = 0; b < NB; b++) {
-    const uint64_t sb = sbase + (uint64_t)b * GPB;
-    SegDesc D{0u, 0u, 0ull};
-    if (sb < s1) D = desc[sb];
-    m[b] = D.m;
-    voff[b] = D.voff;
-    const bool valid = j < D.m;
-    det[b] = mem[valid ? D.moff + j : 0];
-    const XT xv = x[det[b]];
-    xj[b] = valid ? xv : (XT)0;
-    full[b] = valid ? (D.m - 1) / 2 : 0;  // steps with m slots
-    const uint32_t jn = j + 1 == D.m ? 0 : j + 1;
+    const uint32_t m = D[b].m;
+    const bool valid = j < m;
+    full[b] = valid ? (m - 1) / 2 : 0;  // steps with m slots
+    const uint32_t jn = j + 1 == m ? 0 : j + 1;
     nxt[b] = (int)((gbase + (valid ? jn : j)) << 2);
-    vp[b] = (gptr)(val + D.voff + (full[b] ? j : 0));
+    vp[b] = (gptr)(val + D[b].voff + (full[b] ? j : 0));
     xp[b] = xj[b];
     acc[b] = (XT)0;
     R[b] = (XT)0;
-    steps = max(steps, full[b]);
   }
-  for (int o = 32; o > 0; o >>= 1) steps = max(steps, (uint32_t)__shfl_xor((int)steps, o, kWave));
-  steps = (uint32_t)__builtin_amdgcn_readfirstlane((int)steps);
   uint32_t T = 0;  // rotations done
 #pragma unroll 1
   for (uint32_t c0 = 1; c0 <= steps; c0 += CH, T += CH) {
@@ -1736,7 +1993,7 @@ __global__ __launch_bounds__(256) void k_seg_spmv_small(const SegDesc* __restric
         const VT w = __builtin_nontemporal_load(vp[b]);
         const bool next = c0 + i + 1 <= full[b];
         v[b][i] = live ? (XT)w : (XT)0;
-        vp[b] += next ? m[b] : 0u;  // stays on the last slot read
+        vp[b] += next ? D[b].m : 0u;  // stays on the last slot read
         live = next;
       }
     }
@@ -1753,25 +2010,90 @@ __global__ __launch_bounds__(256) void k_seg_spmv_small(const SegDesc* __restric
   }
 #pragma unroll
   for (int b = 0; b < NB; b++) {
-    const uint32_t mb = m[b], mh = mb >> 1, mm = mb ? mb : 1u;
+    const uint32_t mb = D[b].m, mh = mb >> 1;
     const bool valid = j < mb;
     const bool half = valid && (mb & 1u) == 0;
     const uint32_t hj = j < mh ? j : j - mh;
-    const VT w = val[voff[b] + (half ? (uint64_t)(mh - 1) * mb + hj : 0ull)];
+    const VT w = val[D[b].voff + (half ? (uint64_t)(mh - 1) * mb + hj : 0ull)];
     const uint32_t pj = j + mh >= mb ? j + mh - mb : j + mh;
     const XT xh = lane_read(xj[b], (int)((gbase + (valid ? pj : j)) << 2));
     acc[b] = fma(half ? (XT)w : (XT)0, xh, acc[b]);
-    const uint32_t src = valid ? (j + mm - T % mm) % mm : j;
-    const XT rb = lane_read(R[b], (int)((gbase + src) << 2));
-    if (valid) atomicAdd(&y[det[b]], acc[b] + rb);
+    // (j - T) mod m; T < 2m for the segments of a lane-group class (T is at
+    // most (S/2 - 1) rounded up to a multiple of CH, m > S/2 or m = S = 2)
+    uint32_t src = j + 2 * mb - T;
+    src = src >= mb ? src - mb : src;
+    src = src >= mb ? src - mb : src;
+    out[b] = acc[b] + lane_read(R[b], (int)((gbase + (valid ? src : j)) << 2));
   }
 }
 
-// Segments per lane group of the class kernels (see k_seg_spmv_small).
-template <int S>
-constexpr int seg_batches() {
-  return S >= 16 ? 2 : 4;
+// The segments [b, e) of one class run of a tile (m <= S), lane groups of S
+// lanes with NB segments each per round; member x from the tile's LDS copy xs,
+// member sums added to its LDS y (ys).
+template <int S, int NB, typename VT, typename XT>
+__device__ __forceinline__ void tile_class_run(uint32_t b, uint32_t e, const SegDesc* __restrict__ desc,
+                                               const uint16_t* __restrict__ loc, const VT* __restrict__ val,
+                                               const XT* xs, XT* ys) {
+  constexpr uint32_t GW = kWave / S;  // lane groups per wave
+  constexpr uint32_t PW = GW * NB;    // segments per wave and round
+  const uint32_t j = threadIdx.x & (S - 1), gl = (threadIdx.x & (kWave - 1)) / S, wv = threadIdx.x / kWave;
+  for (uint32_t r0 = b; r0 < e; r0 += 4 * PW) {
+    // the wave's segments are consecutive, the first the largest (a class
+    // run lists its segments largest first); waves past the run idle
+    const uint32_t sw = r0 + wv * PW;
+    if (sw >= e) break;
+    const uint32_t steps = (desc[sw].m - 1) / 2;
+    SegDesc D[NB];
+    XT xj[NB], out[NB];
+    uint32_t l[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+      const uint32_t sk = sw + k * GW + gl;
+      D[k] = SegDesc{0u, 0u, 0ull};
+      if (sk < e) D[k] = desc[sk];
+      const bool valid = j < D[k].m;
+      l[k] = valid ? loc[(uint64_t)D[k].moff + j] : 0u;
+      xj[k] = valid ? xs[l[k]] : (XT)0;
+    }
+    seg_group_sums<S, NB, VT, XT>(D, xj, val, steps, out);
+#pragma unroll
+    for (int k = 0; k < NB; k++)
+      if (j < D[k].m) atomicAdd(&ys[l[k]], out[k]);
+  }
 }
+
+// Off-diagonal part of the segments with m <= 64, one workgroup per tile: the
+// tile's x in LDS (one gather per distinct det), its class runs with lane
+// groups of 64, 32, 16, 8, 4, 2 lanes, member sums into an LDS y, then one
+// global atomic per distinct det.  VT stored value type, XT vector type.
+template <typename VT, typename XT>
+__global__ __launch_bounds__(256) void k_seg_spmv_tiles(const uint32_t* __restrict__ rb, const uint32_t* __restrict__ re,
+                                                        const uint32_t* __restrict__ u0, const uint32_t* __restrict__ nu,
+                                                        const uint32_t* __restrict__ tab, const SegDesc* __restrict__ desc,
+                                                        const uint16_t* __restrict__ loc, const VT* __restrict__ val,
+                                                        const XT* __restrict__ x, XT* __restrict__ y) {
+  __shared__ XT xs[kTileMax];
+  __shared__ XT ys[kTileMax];
+  const uint64_t t = blockIdx.x;
+  const uint32_t U = nu[t];
+  const uint32_t* __restrict__ tb = tab + u0[t];
+  for (uint32_t u = threadIdx.x; u < U; u += 256) {
+    xs[u] = x[tb[u]];
+    ys[u] = (XT)0;
+  }
+  __syncthreads();
+  const uint32_t* rbt = rb + t * 6;
+  const uint32_t* ret = re + t * 6;
+  tile_class_run<64, 2, VT, XT>(rbt[0], ret[0], desc, loc, val, xs, ys);
+  tile_class_run<32, 2, VT, XT>(rbt[1], ret[1], desc, loc, val, xs, ys);
+  tile_class_run<16, 2, VT, XT>(rbt[2], ret[2], desc, loc, val, xs, ys);
+  tile_class_run<8, 4, VT, XT>(rbt[3], ret[3], desc, loc, val, xs, ys);
+  tile_class_run<4, 4, VT, XT>(rbt[4], ret[4], desc, loc, val, xs, ys);
+  tile_class_run<2, 4, VT, XT>(rbt[5], ret[5], desc, loc, val, xs, ys);
+  __syncthreads();
+  for (uint32_t u = threadIdx.x; u < U; u += 256) atomicAdd(&y[tb[u]], ys[u]);
+}
+
 
 // Off-diagonal part, one wavefront per (segment, group of 64 T members) for
 // m > 64.  The member x sit in LDS; each lane owns the members
@@ -2267,9 +2589,10 @@ SegKeyParams seg_key_params(shci_ctx* ctx) {
   K.dnpos = P<uint32_t>(ctx->dnpos);
   const uint32_t rA = nu >= 2 ? ctx->r2_alpha : ctx->r_alpha, rB = nd >= 2 ? ctx->r2_beta : ctx->r_beta;
   K.bC = bits_for(ctx->r_beta);
-  K.bA = bits_for(ctx->n_dgroups);
-  K.bB = bits_for(ctx->n_ugroups);
-  K.tag_shift = std::max({bits_for(ctx->r_alpha) + K.bC, bits_for(rA) + K.bA, bits_for(rB) + K.bB});
+  K.bA = bits_for(rA);
+  K.bB = bits_for(rB);
+  K.tag_shift = std::max({bits_for(ctx->r_alpha) + K.bC, bits_for(ctx->n_dgroups) + K.bA,
+                          bits_for(ctx->n_ugroups) + K.bB});
   return K;
 }
 
@@ -2280,7 +2603,7 @@ int build_seg_layout(shci_ctx* ctx) {
   k_inverse_perm<<<div_up(ctx->n, 256), 256, 0, st>>>(P<uint32_t>(ctx->dnord_idx), P<uint32_t>(ctx->dnpos), ctx->n);
   const SegKeyParams K = seg_key_params(ctx);
   const uint64_t M = K.n * ((uint64_t)K.PC + K.PA + K.PB);
-  ctx->n_seg = ctx->n_mem = ctx->n_slots = 0;
+  ctx->n_seg = ctx->n_mem = ctx->n_slots = ctx->n_tiles = 0;
   for (auto& c : ctx->seg_cs) c = 0;
   for (auto& c : ctx->seg_maxm) c = 0;
   if (M == 0) return SHCI_OK;
@@ -2293,7 +2616,7 @@ int build_seg_layout(shci_ctx* ctx) {
     HIPC(hipMemGetInfo(&free_b, &total_b));
     if ((double)free_b < 40.0 * (double)M) {
       DevBuf* big[] = {&ctx->sv64, &ctx->sv32, &ctx->seg_mem, &ctx->seg_desc, &ctx->seg_core, &ctx->seg_type,
-                       &ctx->dav, &ctx->tmp};
+                       &ctx->seg_loc, &ctx->tile_tab, &ctx->dav, &ctx->tmp};
       for (DevBuf* b : big) release(ctx, *b);
     }
   }
@@ -2338,17 +2661,47 @@ int build_seg_layout(shci_ctx* ctx) {
   CHECK_RC(copy_back(ctx, &hbad, bad));
   if (hbad) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment with more than 4096 members");
   if (S == 0) return SHCI_OK;
-  // Order the segments: size class, type, group (k_seg_compact).
+  // Tiles: the kept runs' spin groups cut at kTileCap memberships.
+  TmpBuf tileinc(ctx);
+  uint32_t n_tiles = 0;
+  {
+    TmpBuf gkey(ctx), tw(ctx), gflag(ctx), gord(ctx), W(ctx), gW(ctx);
+    CHECK_RC(ensure(ctx, gkey.b, (uint64_t)S * 8));
+    CHECK_RC(ensure(ctx, tw.b, (uint64_t)S * 4));
+    CHECK_RC(ensure(ctx, gflag.b, (uint64_t)S * 4));
+    k_seg_groups<<<div_up(R, 256), 256, 0, st>>>(K, vals_s.p<uint32_t>(), rstart.p<uint32_t>(), useinc.p<uint32_t>(),
+                                                 keep.p<uint32_t>(), sidx.p<uint32_t>(), R, gkey.p<uint64_t>(),
+                                                 tw.p<uint32_t>());
+    k_seg_gflags<<<div_up(S, 256), 256, 0, st>>>(gkey.p<uint64_t>(), S, gflag.p<uint32_t>());
+    release(ctx, gkey.b);
+    CHECK_RC(ensure(ctx, gord.b, (uint64_t)S * 4));
+    CHECK_RC(ensure(ctx, W.b, (uint64_t)S * 4));
+    CHECK_RC(ensure(ctx, gW.b, (uint64_t)S * 4));
+    CHECK_RC(incl_scan(ctx, gflag.p<uint32_t>(), gord.p<uint32_t>(), S));
+    CHECK_RC(incl_scan(ctx, tw.p<uint32_t>(), W.p<uint32_t>(), S));
+    k_seg_gstart<<<div_up(S, 256), 256, 0, st>>>(gflag.p<uint32_t>(), gord.p<uint32_t>(), W.p<uint32_t>(),
+                                                 tw.p<uint32_t>(), S, gW.p<uint32_t>());
+    CHECK_RC(ensure(ctx, gkey.b, (uint64_t)S * 4));
+    uint32_t* tflag = gkey.p<uint32_t>();
+    k_seg_tflags<<<div_up(S, 256), 256, 0, st>>>(gflag.p<uint32_t>(), gord.p<uint32_t>(), W.p<uint32_t>(),
+                                                 tw.p<uint32_t>(), gW.p<uint32_t>(), S, tflag);
+    HIPC(hipGetLastError());
+    CHECK_RC(ensure(ctx, tileinc.b, (uint64_t)S * 4));
+    CHECK_RC(incl_scan(ctx, tflag, tileinc.p<uint32_t>(), S));
+    CHECK_RC(copy_back(ctx, &n_tiles, tileinc.p<uint32_t>() + (S - 1)));
+  }
+  // Order the segments: size class, type, tile (k_seg_compact).
   TmpBuf skey(ctx), srun(ctx), skey_s(ctx), srun_s(ctx);
   CHECK_RC(ensure(ctx, skey.b, (uint64_t)S * 8));
   CHECK_RC(ensure(ctx, srun.b, (uint64_t)S * 4));
   CHECK_RC(ensure(ctx, skey_s.b, (uint64_t)S * 8));
   CHECK_RC(ensure(ctx, srun_s.b, (uint64_t)S * 4));
   k_seg_compact<<<div_up(R, 256), 256, 0, st>>>(K, vals_s.p<uint32_t>(), rstart.p<uint32_t>(), useinc.p<uint32_t>(),
-                                                keep.p<uint32_t>(), sidx.p<uint32_t>(), R, skey.p<uint64_t>(),
-                                                srun.p<uint32_t>());
+                                                keep.p<uint32_t>(), sidx.p<uint32_t>(), tileinc.p<uint32_t>(), R,
+                                                skey.p<uint64_t>(), srun.p<uint32_t>());
   release(ctx, keep.b);
   release(ctx, sidx.b);
+  release(ctx, tileinc.b);
   CHECK_RC(radix_sort_pairs(ctx, skey.p<uint64_t>(), skey_s.p<uint64_t>(), srun.p<uint32_t>(), srun_s.p<uint32_t>(), S,
                             64));  // class in bits 60-63, type 58-59
   release(ctx, skey.b);
@@ -2360,12 +2713,10 @@ int build_seg_layout(shci_ctx* ctx) {
   CHECK_RC(ensure(ctx, moff.b, (uint64_t)S * 8));
   CHECK_RC(ensure(ctx, voff.b, (uint64_t)S * 8));
   HIPC(hipMemsetAsync(finv.b.p, 0xFF, (uint64_t)R * 4, st));
-  unsigned int* hist = bad + 32;  // bytes 128..235 of the 512-byte status block
-  HIPC(hipMemsetAsync(hist, 0, kSegRanges * 4, st));
-  k_seg_sizes<<<div_up(S, 256), 256, 0, st>>>(srun_s.p<uint32_t>(), skey_s.p<uint64_t>(), rstart.p<uint32_t>(),
-                                             useinc.p<uint32_t>(), S, msz.p<uint64_t>(), nslot.p<uint64_t>(),
-                                             finv.p<uint32_t>(), hist);
-  release(ctx, skey_s.b);
+  uint64_t* bnd = reinterpret_cast<uint64_t*>(ctx->scal.p) + 32;  // bytes 256..479 of the status block
+  k_seg_sizes<<<div_up(S, 256), 256, 0, st>>>(srun_s.p<uint32_t>(), rstart.p<uint32_t>(), useinc.p<uint32_t>(), S,
+                                             msz.p<uint64_t>(), nslot.p<uint64_t>(), finv.p<uint32_t>());
+  k_seg_bounds<<<div_up(S, 256), 256, 0, st>>>(skey_s.p<uint64_t>(), S, bnd);
   CHECK_RC(excl_scan(ctx, msz.p<uint64_t>(), moff.p<uint64_t>(), S));
   CHECK_RC(excl_scan(ctx, nslot.p<uint64_t>(), voff.p<uint64_t>(), S));
   uint64_t last[4];
@@ -2375,8 +2726,8 @@ int build_seg_layout(shci_ctx* ctx) {
   CHECK_RC(copy_back(ctx, &last[3], nslot.p<uint64_t>() + (S - 1)));
   const uint64_t n_mem = last[0] + last[1], n_slots = last[2] + last[3];
   if (n_mem >= (1ull << 32)) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment memberships >= 2^32");
-  unsigned int h[kSegRanges];
-  CHECK_RC(copy_back(ctx, h, hist, kSegRanges));
+  uint64_t hb[kSegRanges + 1];
+  CHECK_RC(copy_back(ctx, hb, bnd, kSegRanges + 1));
   CHECK_RC(ensure(ctx, ctx->seg_desc, (uint64_t)S * sizeof(SegDesc)));
   CHECK_RC(ensure(ctx, ctx->seg_type, (uint64_t)S));
   CHECK_RC(ensure(ctx, ctx->seg_core, (uint64_t)S * 8));
@@ -2393,14 +2744,48 @@ int build_seg_layout(shci_ctx* ctx) {
   ctx->n_seg = S;
   ctx->n_mem = n_mem;
   ctx->n_slots = n_slots;
-  ctx->seg_cs[0] = 0;
-  for (int r = 0; r < kSegRanges; r++) ctx->seg_cs[r + 1] = ctx->seg_cs[r] + h[r];
+  for (int r = 0; r <= kSegRanges; r++) ctx->seg_cs[r] = hb[r];
   for (int r = 0; r < 9; r++) {  // one-wavefront classes: largest segment first in each range
-    if (h[r] == 0) continue;
+    if (hb[r + 1] == hb[r]) continue;
     SegDesc d0;
     CHECK_RC(copy_back(ctx, &d0, desc + ctx->seg_cs[r]));
     ctx->seg_maxm[r] = d0.m;
   }
+  // Tile class runs, det tables and the memberships' table indices.
+  const uint64_t T = n_tiles, f0 = ctx->seg_cs[9];
+  CHECK_RC(ensure(ctx, ctx->tile_rb, T * 6 * 4));
+  CHECK_RC(ensure(ctx, ctx->tile_re, T * 6 * 4));
+  CHECK_RC(ensure(ctx, ctx->tile_u0, T * 4));
+  CHECK_RC(ensure(ctx, ctx->tile_nu, T * 4));
+  HIPC(hipMemsetAsync(ctx->tile_rb.p, 0, T * 6 * 4, st));
+  HIPC(hipMemsetAsync(ctx->tile_re.p, 0, T * 6 * 4, st));
+  if (S > f0)
+    k_tile_runs<<<div_up(S - f0, 256), 256, 0, st>>>(skey_s.p<uint64_t>(), f0, S, P<uint32_t>(ctx->tile_rb),
+                                                     P<uint32_t>(ctx->tile_re));
+  release(ctx, skey_s.b);
+  {
+    TmpBuf tcnt(ctx);
+    CHECK_RC(ensure(ctx, tcnt.b, T * 4));
+    k_tile_count<<<div_up(T, 256), 256, 0, st>>>(P<uint32_t>(ctx->tile_rb), P<uint32_t>(ctx->tile_re), desc, S,
+                                                 n_mem, T, tcnt.p<uint32_t>());
+    CHECK_RC(excl_scan(ctx, tcnt.p<uint32_t>(), P<uint32_t>(ctx->tile_u0), T));
+    uint32_t lastc = 0, lastu = 0;
+    CHECK_RC(copy_back(ctx, &lastc, tcnt.p<uint32_t>() + (T - 1)));
+    CHECK_RC(copy_back(ctx, &lastu, P<uint32_t>(ctx->tile_u0) + (T - 1)));
+    const uint64_t n_tab = (uint64_t)lastu + lastc;
+    CHECK_RC(ensure(ctx, ctx->tile_tab, std::max<uint64_t>(n_tab, 1) * 4));
+    CHECK_RC(ensure(ctx, ctx->seg_loc, n_mem * 2));
+    HIPC(hipMemsetAsync(bad, 0, 4, st));
+    k_tile_check<<<div_up(T, 256), 256, 0, st>>>(tcnt.p<uint32_t>(), T, bad);
+    CHECK_RC(copy_back(ctx, &hbad, bad));
+    if (hbad) return fail(ctx, SHCI_ERR_STATE, "internal: tile with too many memberships");
+  }
+  k_tile_tables<<<T, 256, 0, st>>>(P<uint32_t>(ctx->tile_rb), P<uint32_t>(ctx->tile_re), P<uint32_t>(ctx->tile_u0),
+                                   desc, S, n_mem, P<uint32_t>(ctx->seg_mem), P<uint8_t>(ctx->seg_type),
+                                   P<uint32_t>(ctx->dnord_idx), P<uint32_t>(ctx->tile_tab),
+                                   P<uint32_t>(ctx->tile_nu), P<uint16_t>(ctx->seg_loc));
+  HIPC(hipGetLastError());
+  ctx->n_tiles = T;
   return SHCI_OK;
 }
 
@@ -2416,7 +2801,7 @@ SegFillParams seg_fill_params(shci_ctx* ctx, int update) {
   F.I = make_ints(ctx);
   F.v64 = P<double>(ctx->sv64);
   F.v32 = P<float>(ctx->sv32);
-  F.kept = reinterpret_cast<unsigned long long*>(ctx->scal.p) + 8;
+  F.kept = P<unsigned long long>(ctx->kept);
   F.ra = ctx->ra;
   F.rb = ctx->rb;
   F.s0 = F.s1 = 0;
@@ -2429,8 +2814,9 @@ SegFillParams seg_fill_params(shci_ctx* ctx, int update) {
 int seg_fill(shci_ctx* ctx, int update) {
   const hipStream_t st = ctx->stream;
   CHECK_RC(ensure(ctx, ctx->scal, 64 * 8));
+  CHECK_RC(ensure(ctx, ctx->kept, kKeptLines * 128));
   SegFillParams F = seg_fill_params(ctx, update);
-  HIPC(hipMemsetAsync(F.kept, 0, 8, st));
+  HIPC(hipMemsetAsync(F.kept, 0, kKeptLines * 128, st));
   for (int r = 0; r < kSegRanges; r++) {
     const int c = r / 3;
     F.s0 = (uint32_t)ctx->seg_cs[r];
@@ -2438,7 +2824,10 @@ int seg_fill(shci_ctx* ctx, int update) {
     const uint32_t cnt = F.s1 - F.s0;
     if (cnt == 0) continue;
     if (c < 3) {
-      k_seg_fill_large<<<cnt, kWave, ctx->seg_maxm[r] * 16, st>>>(F);
+      if (!update && r % 3 == (int)kSegC)
+        k_seg_fill_large_c<<<cnt, kWave, ctx->seg_maxm[r] * 16, st>>>(F);
+      else
+        k_seg_fill_large<<<cnt, kWave, ctx->seg_maxm[r] * 16, st>>>(F);
       continue;
     }
     const int S = 64 >> (c - 3);
@@ -2464,8 +2853,10 @@ int seg_fill(shci_ctx* ctx, int update) {
     }
   }
   HIPC(hipGetLastError());
+  std::vector<unsigned long long> lines((size_t)kKeptLines * 16);
+  CHECK_RC(copy_back(ctx, lines.data(), F.kept, lines.size()));
   unsigned long long kept = 0;
-  CHECK_RC(copy_back(ctx, &kept, F.kept));
+  for (uint32_t i = 0; i < kKeptLines; i++) kept += lines[(size_t)i * 16];
   if (!update) ctx->n_pairs = kept;
   return SHCI_OK;
 }
@@ -2503,8 +2894,12 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
   const VT* val = sizeof(VT) == 4 ? reinterpret_cast<const VT*>(ctx->sv32.p) : reinterpret_cast<const VT*>(ctx->sv64.p);
   if (!val && ctx->n_slots) return fail(ctx, SHCI_ERR_STATE, "requested precision not stored");
   k_diag_apply<NV, XT><<<div_up(n, 256), 256, 0, st>>>(P<double>(ctx->diag), shift, x, y, n);
+  // A segments address beta-order copies of x and y: all of them in the block
+  // product, only the one-wavefront classes (m > 64) in the single product
+  // (the tiles hold internal det ids).
+  const int a_classes = NV == 4 ? kSegClasses : 3;
   bool have_a = false;
-  for (int c = 0; c < kSegClasses; c++) have_a |= ctx->seg_cs[3 * c + kSegA + 1] > ctx->seg_cs[3 * c + kSegA];
+  for (int c = 0; c < a_classes; c++) have_a |= ctx->seg_cs[3 * c + kSegA + 1] > ctx->seg_cs[3 * c + kSegA];
   XT* xb = nullptr;
   XT* yb = nullptr;
   const uint32_t* dnord = P<uint32_t>(ctx->dnord_idx);
@@ -2540,29 +2935,20 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
           default: k_seg_spmm_small<2><<<grid, 256, 0, st>>>(desc, s0, s1, mem, vf, xr, yr); break;
         }
       }
-    } else {
-      if (c < 3) {
-        const size_t lds = (size_t)ctx->seg_maxm[r] * NV * sizeof(XT);
-        if (lds > 65536) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment too large for the one-wavefront kernel");
-        // one wavefront per (segment, 256 members)
-        const dim3 grid(s1 - s0, div_up(ctx->seg_maxm[r], kWave * kSegChunks));
-        k_seg_spmv_large<kSegChunks, NV, VT, XT><<<grid, kWave, lds, st>>>(desc, s0, s1, mem, val, xr, yr);
-        continue;
-      }
-      const int S = 64 >> (c - 3);
-#define SHCI_SEG_SMALL(SS)                                                                           \
-  k_seg_spmv_small<SS, seg_batches<SS>(), VT, XT>                                                    \
-      <<<div_up((uint64_t)(s1 - s0) * SS, 256 * seg_batches<SS>()), 256, 0, st>>>(desc, s0, s1, mem, val, xr, yr)
-      switch (S) {
-        case 64: SHCI_SEG_SMALL(64); break;
-        case 32: SHCI_SEG_SMALL(32); break;
-        case 16: SHCI_SEG_SMALL(16); break;
-        case 8: SHCI_SEG_SMALL(8); break;
-        case 4: SHCI_SEG_SMALL(4); break;
-        default: SHCI_SEG_SMALL(2); break;
-      }
-#undef SHCI_SEG_SMALL
+    } else if (c < 3) {
+      const size_t lds = (size_t)ctx->seg_maxm[r] * NV * sizeof(XT);
+      if (lds > 65536) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment too large for the one-wavefront kernel");
+      // one wavefront per (segment, 256 members)
+      const dim3 grid(s1 - s0, div_up(ctx->seg_maxm[r], kWave * kSegChunks));
+      k_seg_spmv_large<kSegChunks, NV, VT, XT><<<grid, kWave, lds, st>>>(desc, s0, s1, mem, val, xr, yr);
     }
+  }
+  if constexpr (NV == 1) {
+    // m <= 64: one workgroup per tile
+    if (ctx->n_tiles)
+      k_seg_spmv_tiles<VT, XT><<<ctx->n_tiles, 256, 0, st>>>(
+          P<uint32_t>(ctx->tile_rb), P<uint32_t>(ctx->tile_re), P<uint32_t>(ctx->tile_u0), P<uint32_t>(ctx->tile_nu),
+          P<uint32_t>(ctx->tile_tab), desc, P<uint16_t>(ctx->seg_loc), val, x, y);
   }
   if (have_a) k_perm_add<NV, XT><<<div_up(n, 256), 256, 0, st>>>(dnord, yb, y, n);
   HIPC(hipGetLastError());
@@ -2666,9 +3052,10 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
   }
   {
     DevBuf* seg[] = {&ctx->seg_desc, &ctx->seg_type, &ctx->seg_core, &ctx->seg_mem, &ctx->sv32, &ctx->sv64,
-                     &ctx->mA_id, &ctx->mB_id, &ctx->dnpos, &ctx->xb, &ctx->yb};
+                     &ctx->mA_id, &ctx->mB_id, &ctx->dnpos, &ctx->xb, &ctx->yb, &ctx->tile_rb, &ctx->tile_re,
+                     &ctx->tile_u0, &ctx->tile_nu, &ctx->tile_tab, &ctx->seg_loc};
     for (DevBuf* b : seg) release(ctx, *b);
-    ctx->n_seg = ctx->n_mem = ctx->n_slots = ctx->n_pairs = 0;
+    ctx->n_seg = ctx->n_mem = ctx->n_slots = ctx->n_pairs = ctx->n_tiles = 0;
   }
   // Pass 1: count candidates per row.
   CHECK_RC(ensure(ctx, ctx->cnt, n * 4));
@@ -3048,11 +3435,12 @@ int shci_ctx_destroy(shci_ctx* ctx) {
   if (!ctx) return SHCI_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  DevBuf* bufs[] = {&ctx->h1, &ctx->eri, &ctx->tD, &ctx->tX, &ctx->up_c, &ctx->dn_c, &ctx->up_s, &ctx->dn_s,
+  DevBuf* bufs[] = {&ctx->h1, &ctx->eri, &ctx->tD, &ctx->tX, &ctx->kept, &ctx->up_c, &ctx->dn_c, &ctx->up_s, &ctx->dn_s,
                     &ctx->perm, &ctx->iperm, &ctx->up_gid, &ctx->up_gstart, &ctx->dn_gid, &ctx->dn_gstart,
                     &ctx->dnord_idx, &ctx->dnord_up, &ctx->m1a_id, &ctx->m1b_id, &ctx->dm1_vals, &ctx->dm1_rstart,
                     &ctx->dm1_seg, &ctx->flag, &ctx->mA_id, &ctx->mB_id, &ctx->seg_desc, &ctx->seg_type,
                     &ctx->seg_core, &ctx->seg_mem, &ctx->sv32, &ctx->sv64, &ctx->dnpos, &ctx->xb, &ctx->yb,
+                    &ctx->tile_rb, &ctx->tile_re, &ctx->tile_u0, &ctx->tile_nu, &ctx->tile_tab, &ctx->seg_loc,
                     &ctx->diag,
                     &ctx->cnt,
                     &ctx->row_ptr, &ctx->row_end, &ctx->col, &ctx->v64, &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb,
@@ -3542,14 +3930,15 @@ int shci_clear(shci_ctx* ctx) {
                     &ctx->up_gstart, &ctx->dn_gid, &ctx->dn_gstart, &ctx->dnord_idx, &ctx->dnord_up,
                     &ctx->m1a_id, &ctx->m1b_id, &ctx->dm1_vals, &ctx->dm1_rstart, &ctx->dm1_seg, &ctx->diag,
                     &ctx->mA_id, &ctx->mB_id, &ctx->seg_desc, &ctx->seg_type, &ctx->seg_core, &ctx->seg_mem,
-                    &ctx->sv32, &ctx->sv64, &ctx->dnpos, &ctx->xb, &ctx->yb,
+                    &ctx->sv32, &ctx->sv64, &ctx->dnpos, &ctx->xb, &ctx->yb, &ctx->tile_rb, &ctx->tile_re,
+                    &ctx->tile_u0, &ctx->tile_nu, &ctx->tile_tab, &ctx->seg_loc,
                     &ctx->cnt, &ctx->row_ptr, &ctx->row_end, &ctx->row_end, &ctx->col, &ctx->v64,
                     &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb, &ctx->va, &ctx->vb, &ctx->xv,
                     &ctx->yv, &ctx->x32, &ctx->y32, &ctx->dav};
   for (DevBuf* b : bufs) release(ctx, *b);
   ctx->n = 0;
   ctx->nnz = 0;
-  ctx->n_seg = ctx->n_mem = ctx->n_slots = ctx->n_pairs = 0;
+  ctx->n_seg = ctx->n_mem = ctx->n_slots = ctx->n_pairs = ctx->n_tiles = 0;
   for (auto& c : ctx->seg_cs) c = 0;
   for (auto& c : ctx->seg_maxm) c = 0;
   ctx->segl = false;
@@ -3667,6 +4056,7 @@ int shci_layout_stats(shci_ctx* ctx, uint64_t* stats) {
   stats[9] = ctx->n_ugroups;
   stats[10] = ctx->n_dgroups;
   stats[11] = (uint64_t)ctx->lanes;
+  stats[12] = ctx->segl ? ctx->n_tiles : 0;
   return SHCI_OK;
 }
 

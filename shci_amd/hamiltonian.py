@@ -174,10 +174,10 @@ class SparseMatrix:
         return ms.value, by.value
 
     def layout_stats(self):
-        st = (C.c_uint64 * 12)()
+        st = (C.c_uint64 * 13)()
         self.ctx.check(self.ctx.lib.shci_layout_stats(self.ctx.handle, st), "shci_layout_stats")
         names = ["n_dets", "nnz", "nnz_alloc", "segments_layout", "segments", "memberships", "large_segments",
-                 "value_slots", "mixed_runs", "alpha_groups", "beta_groups", "lanes"]
+                 "value_slots", "mixed_runs", "alpha_groups", "beta_groups", "lanes", "tiles"]
         return dict(zip(names, list(st)))
 
     def segment_ranges(self):

@@ -210,6 +210,49 @@ def test_segments_counts_and_update_existing(system, keep_zeros):
 
 
 @pytest.mark.gpu
+def test_segments_with_split_tiles_vs_csr():
+    """Full space of 12 orbitals, 3+3 electrons (48,400 dets): the spin groups
+    hold thousands of memberships, so the segment SpMV runs on groups cut into
+    several tiles.  FP64 and FP32 products, and the FP64 Davidson energy,
+    against the CSR layout (bit-exact against the oracle on the smaller
+    systems)."""
+    import itertools
+
+    from shci_amd.hamiltonian import Davidson, Hamiltonian
+
+    n, nu, nd = 12, 3, 3
+    _, h1, eri, _ = si.synthetic(n, seed=21)
+    strs = [sum(1 << k for k in c) for c in itertools.combinations(range(n), nu)]
+    hf = (1 << nu) - 1
+    dets = [(hf, hf)] + [(u, d) for u in strs for d in strs if (u, d) != (hf, hf)]
+    up = np.array([d[0] for d in dets], dtype=np.uint64)
+    dn = np.array([d[1] for d in dets], dtype=np.uint64)
+    x = np.random.default_rng(5).uniform(-1, 1, len(up))
+    init = np.zeros(len(up))
+    init[0] = 1.0
+    res = {}
+    for layout in ("csr", "segments"):
+        ham = Hamiltonian(n, h1, eri, 0.0, store="both", layout=layout)
+        ham.update(up, dn)
+        print(f"{layout}: {len(up)} dets, stats {ham.matrix.layout_stats()}", flush=True)
+        res[layout] = ham.matrix.mul(x)
+        res[layout + "32"] = ham.matrix.mul(x, fp32=True)
+        res[layout + "_nnz"] = ham.matrix.count_n_elems()
+        d = Davidson(1)
+        d.diagonalize(ham.matrix, init, 1e-10, max_calls=40)
+        res[layout + "_e"] = d.lowest_eigenvalues[0]
+        if layout == "segments":
+            st = ham.matrix.layout_stats()
+            groups = st["alpha_groups"] + st["beta_groups"]
+            assert st["tiles"] > groups  # some groups span several tiles
+        ham.clear()
+    assert res["segments_nnz"] == res["csr_nnz"]
+    np.testing.assert_allclose(res["segments"], res["csr"], rtol=1e-12, atol=1e-11)
+    np.testing.assert_allclose(res["segments32"], res["csr"], rtol=0, atol=2e-4 * np.abs(res["csr"]).max())
+    assert abs(res["segments_e"] - res["csr_e"]) < 1e-9
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("layout", ["csr", "segments"])
 def test_multiroot_davidson_vs_dense(system, layout):
     """Three roots (the multi-root path of cfg 5): GPU FP64 and mixed Davidson
