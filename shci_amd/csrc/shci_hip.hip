@@ -119,6 +119,10 @@ struct shci_ctx {
   DevBuf tile_tab;                      // tile det tables (internal det ids)
   DevBuf seg_loc;                       // membership -> index in its tile's table (m <= 64 only)
   DevBuf kept;                          // kept-pair counters of the segment fill
+  // Released device blocks kept for reuse (builds allocate and drop the same
+  // temporaries every time; hipMalloc/hipFree of GB blocks per build would
+  // cost more than the kernels).  Freed on memory pressure, clear and destroy.
+  std::vector<DevBuf> cache;
 
   hipEvent_t ev[9] = {};
   float phase_ms[8] = {};
@@ -146,28 +150,54 @@ int fail(shci_ctx* c, int code, const std::string& msg) {
     if (_rc != SHCI_OK) return _rc; \
   } while (0)
 
+// Gives b back to the context's block cache (its memory stays counted in use).
+void release(shci_ctx* ctx, DevBuf& b) {
+  if (b.p) ctx->cache.push_back(b);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+// Frees the cached blocks.
+void cache_trim(shci_ctx* ctx) {
+  for (DevBuf& c : ctx->cache) {
+    (void)hipFree(c.p);
+    ctx->mem_in_use -= c.bytes;
+  }
+  ctx->cache.clear();
+}
+
+// b holds at least `bytes`: kept if large enough, else replaced by the
+// smallest cached block of bytes .. 2 * bytes + 64 MiB, else a new block (the
+// cache is freed and the allocation retried once if memory runs out).
 int ensure(shci_ctx* ctx, DevBuf& b, size_t bytes) {
   if (bytes <= b.bytes && b.p) return SHCI_OK;
-  if (b.p) {
-    HIPC(hipFree(b.p));
-    ctx->mem_in_use -= b.bytes;
-    b.p = nullptr;
-    b.bytes = 0;
-  }
+  release(ctx, b);
   if (bytes == 0) return SHCI_OK;
-  HIPC(hipMalloc(&b.p, bytes));
+  size_t best = ctx->cache.size();
+  for (size_t i = 0; i < ctx->cache.size(); i++) {
+    const size_t cb = ctx->cache[i].bytes;
+    if (cb >= bytes && cb <= 2 * bytes + (64ull << 20) && (best == ctx->cache.size() || cb < ctx->cache[best].bytes))
+      best = i;
+  }
+  if (best < ctx->cache.size()) {
+    b = ctx->cache[best];
+    ctx->cache.erase(ctx->cache.begin() + best);
+    return SHCI_OK;
+  }
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e == hipErrorOutOfMemory && !ctx->cache.empty()) {
+    (void)hipGetLastError();
+    cache_trim(ctx);
+    e = hipMalloc(&b.p, bytes);
+  }
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return fail(ctx, e == hipErrorOutOfMemory ? SHCI_ERR_OOM : SHCI_ERR_HIP,
+                std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+  }
   b.bytes = bytes;
   ctx->mem_in_use += bytes;
   return SHCI_OK;
-}
-
-void release(shci_ctx* ctx, DevBuf& b) {
-  if (b.p) {
-    (void)hipFree(b.p);
-    ctx->mem_in_use -= b.bytes;
-  }
-  b.p = nullptr;
-  b.bytes = 0;
 }
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -1509,16 +1539,23 @@ __global__ void k_tile_check(const uint32_t* __restrict__ cnt, uint64_t T, unsig
 
 // One workgroup per tile: the distinct dets of its memberships (LDS hash
 // table with linear probing; at most kTileMax keys in kTileHash slots), their
-// table tab[u0 + i] in slot order and each membership's table index.
+// table tab[u0 + i] in ascending order (bitonic sort in LDS) and each
+// membership's table index.  A tiles list beta-order positions (the SpMV runs
+// them on the beta-order copies of x and y, where a beta group's dets are
+// adjacent), the others internal det ids; nu[t] = table size | (A tile << 31).
+// Ascending tables make the SpMV's gathers and atomics of a tile touch
+// neighbouring addresses.
 __global__ __launch_bounds__(256) void k_tile_tables(const uint32_t* __restrict__ rb, const uint32_t* __restrict__ re,
                                                      const uint32_t* __restrict__ u0, const SegDesc* __restrict__ desc,
                                                      uint64_t S, uint64_t n_mem, const uint32_t* __restrict__ mem,
-                                                     const uint8_t* __restrict__ type,
-                                                     const uint32_t* __restrict__ dnord, uint32_t* __restrict__ tab,
+                                                     const uint8_t* __restrict__ type, uint32_t* __restrict__ tab,
                                                      uint32_t* __restrict__ nu, uint16_t* __restrict__ loc) {
   __shared__ uint32_t hk[kTileHash];
   __shared__ uint16_t cid[kTileHash];
   __shared__ uint16_t sl[kTileMax];
+  __shared__ uint32_t key[kTileMax];
+  __shared__ uint16_t pay[kTileMax];
+  __shared__ uint16_t rank[kTileMax];
   __shared__ uint32_t wtot[4];
   const uint32_t t = blockIdx.x, tid = threadIdx.x;
   for (uint32_t h = tid; h < kTileHash; h += 256) hk[h] = 0xFFFFFFFFu;
@@ -1538,8 +1575,7 @@ __global__ __launch_bounds__(256) void k_tile_tables(const uint32_t* __restrict_
   for (int k = 0; k < 6; k++) {
     const uint32_t len = (uint32_t)(me[k] - mb[k]);
     for (uint32_t i = tid; i < len; i += 256) {
-      uint32_t d = mem[mb[k] + i];
-      if (isA) d = dnord[d];
+      const uint32_t d = mem[mb[k] + i];
       uint32_t h = (d * 2654435761u) >> 20;
       while (true) {
         const uint32_t old = atomicCAS(&hk[h], 0xFFFFFFFFu, d);
@@ -1551,7 +1587,7 @@ __global__ __launch_bounds__(256) void k_tile_tables(const uint32_t* __restrict_
     k0 += len;
   }
   __syncthreads();
-  // slot order -> table index: each thread counts 16 consecutive slots,
+  // occupied slots -> dense index: each thread counts 16 consecutive slots,
   // then a block-wide exclusive scan of the counts
   constexpr int PER = kTileHash / 256;
   uint32_t c = 0;
@@ -1570,16 +1606,46 @@ __global__ __launch_bounds__(256) void k_tile_tables(const uint32_t* __restrict_
     const uint32_t h = tid * PER + q;
     if (hk[h] != 0xFFFFFFFFu) {
       cid[h] = (uint16_t)base;
-      tab[(uint64_t)u0[t] + base] = hk[h];
+      key[base] = hk[h];
+      pay[base] = (uint16_t)base;
       base++;
     }
   }
-  if (tid == 0) nu[t] = total;
+  uint32_t P = 2;  // power of two >= total, padded with keys that sort last
+  while (P < total) P <<= 1;
+  for (uint32_t i = total + tid; i < P; i += 256) {
+    key[i] = 0xFFFFFFFFu;
+    pay[i] = 0;
+  }
+  __syncthreads();
+  for (uint32_t k = 2; k <= P; k <<= 1) {
+    for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+      for (uint32_t i = tid; i < P; i += 256) {
+        const uint32_t ixj = i ^ jj;
+        if (ixj > i) {
+          const uint32_t ka = key[i], kb = key[ixj];
+          if ((ka > kb) == ((i & k) == 0)) {
+            key[i] = kb;
+            key[ixj] = ka;
+            const uint16_t pa = pay[i];
+            pay[i] = pay[ixj];
+            pay[ixj] = pa;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t i = tid; i < total; i += 256) {
+    rank[pay[i]] = (uint16_t)i;
+    tab[(uint64_t)u0[t] + i] = key[i];
+  }
+  if (tid == 0) nu[t] = total | (isA ? 0x80000000u : 0u);
   __syncthreads();
   k0 = 0;
   for (int k = 0; k < 6; k++) {
     const uint32_t len = (uint32_t)(me[k] - mb[k]);
-    for (uint32_t i = tid; i < len; i += 256) loc[mb[k] + i] = cid[sl[k0 + i]];
+    for (uint32_t i = tid; i < len; i += 256) loc[mb[k] + i] = rank[cid[sl[k0 + i]]];
     k0 += len;
   }
 }
@@ -1958,10 +2024,10 @@ __device__ __forceinline__ double lane_read(double v, int a) {
 // loop runs for `steps` steps, the most of any segment of the wave (wave
 // uniform); lanes past their own steps re-read their last slot and add zero.
 // All lanes of the wave must call this.
-template <int S, int NB, typename VT, typename XT>
+template <int S, int NB, int CH, typename VT, typename XT>
 __device__ __forceinline__ void seg_group_sums(const SegDesc (&D)[NB], const XT (&xj)[NB], const VT* __restrict__ val,
                                                uint32_t steps, XT (&out)[NB]) {
-  constexpr int CH = S / 2 < 4 ? S / 2 : 4;  // steps per batch of value loads
+  // CH: steps per batch of value loads
   using gptr = const __attribute__((address_space(1))) VT*;  // global loads, not flat
   const uint32_t j = threadIdx.x & (S - 1);
   const uint32_t gbase = lane_id() & ~(uint32_t)(S - 1);
@@ -2018,8 +2084,9 @@ __device__ __forceinline__ void seg_group_sums(const SegDesc (&D)[NB], const XT 
     const uint32_t pj = j + mh >= mb ? j + mh - mb : j + mh;
     const XT xh = lane_read(xj[b], (int)((gbase + (valid ? pj : j)) << 2));
     acc[b] = fma(half ? (XT)w : (XT)0, xh, acc[b]);
-    // (j - T) mod m; T < 2m for the segments of a lane-group class (T is at
-    // most (S/2 - 1) rounded up to a multiple of CH, m > S/2 or m = S = 2)
+    // (j - T) mod m; T < 2m for the segments of a lane-group class: T is
+    // (S/2 - 1) rounded up to a multiple of CH <= S at most, and m > S/2
+    // (or m = S = 2, T = 0)
     uint32_t src = j + 2 * mb - T;
     src = src >= mb ? src - mb : src;
     src = src >= mb ? src - mb : src;
@@ -2027,22 +2094,36 @@ __device__ __forceinline__ void seg_group_sums(const SegDesc (&D)[NB], const XT 
   }
 }
 
-// The segments [b, e) of one class run of a tile (m <= S), lane groups of S
-// lanes with NB segments each per round; member x from the tile's LDS copy xs,
-// member sums added to its LDS y (ys).
-template <int S, int NB, typename VT, typename XT>
-__device__ __forceinline__ void tile_class_run(uint32_t b, uint32_t e, const SegDesc* __restrict__ desc,
-                                               const uint16_t* __restrict__ loc, const VT* __restrict__ val,
-                                               const XT* xs, XT* ys) {
+// A tile in LDS: its x and y, and for each of its segments (numbered class
+// run by class run) the slot offset from the run's first slot and the member
+// offset from the run's first member and m (packed (moff << 8) | m), and each
+// membership's index into x and y.  A tile has fewer than kTileMax
+// memberships, hence fewer than kTileMax / 2 segments.
+template <typename XT>
+struct TileLds {
+  XT xs[kTileMax];
+  XT ys[kTileMax];
+  uint32_t sv[kTileMax / 2];
+  uint32_t sm[kTileMax / 2];
+  uint16_t sl[kTileMax];
+};
+
+// One class run of a tile: its n segments (LDS entries k0 ..) with m <= S,
+// lane groups of S lanes with NB segments each per round; the run's first
+// slot voff0 and first membership m0 (LDS).
+template <int S, int NB, int CH, typename VT, typename XT>
+__device__ __forceinline__ void tile_class_run(uint32_t n, uint32_t k0, uint32_t m0, uint64_t voff0,
+                                               const VT* __restrict__ val, TileLds<XT>& L) {
+  static_assert(CH <= S, "the delivery of the travelling sums assumes fewer than 2m rotations");
   constexpr uint32_t GW = kWave / S;  // lane groups per wave
   constexpr uint32_t PW = GW * NB;    // segments per wave and round
   const uint32_t j = threadIdx.x & (S - 1), gl = (threadIdx.x & (kWave - 1)) / S, wv = threadIdx.x / kWave;
-  for (uint32_t r0 = b; r0 < e; r0 += 4 * PW) {
+  for (uint32_t r0 = 0; r0 < n; r0 += 4 * PW) {
     // the wave's segments are consecutive, the first the largest (a class
     // run lists its segments largest first); waves past the run idle
     const uint32_t sw = r0 + wv * PW;
-    if (sw >= e) break;
-    const uint32_t steps = (desc[sw].m - 1) / 2;
+    if (sw >= n) break;
+    const uint32_t steps = ((L.sm[k0 + sw] & 0xFFu) - 1) / 2;
     SegDesc D[NB];
     XT xj[NB], out[NB];
     uint32_t l[NB];
@@ -2050,62 +2131,130 @@ __device__ __forceinline__ void tile_class_run(uint32_t b, uint32_t e, const Seg
     for (int k = 0; k < NB; k++) {
       const uint32_t sk = sw + k * GW + gl;
       D[k] = SegDesc{0u, 0u, 0ull};
-      if (sk < e) D[k] = desc[sk];
+      if (sk < n) {
+        const uint32_t pk = L.sm[k0 + sk];
+        D[k] = SegDesc{m0 + (pk >> 8), pk & 0xFFu, voff0 + L.sv[k0 + sk]};
+      }
       const bool valid = j < D[k].m;
-      l[k] = valid ? loc[(uint64_t)D[k].moff + j] : 0u;
-      xj[k] = valid ? xs[l[k]] : (XT)0;
+      l[k] = valid ? L.sl[D[k].moff + j] : 0u;
+      xj[k] = valid ? L.xs[l[k]] : (XT)0;
     }
-    seg_group_sums<S, NB, VT, XT>(D, xj, val, steps, out);
+    seg_group_sums<S, NB, CH, VT, XT>(D, xj, val, steps, out);
 #pragma unroll
     for (int k = 0; k < NB; k++)
-      if (j < D[k].m) atomicAdd(&ys[l[k]], out[k]);
+      if (j < D[k].m) atomicAdd(&L.ys[l[k]], out[k]);
   }
 }
 
 // Off-diagonal part of the segments with m <= 64, one workgroup per tile: the
-// tile's x in LDS (one gather per distinct det), its class runs with lane
-// groups of 64, 32, 16, 8, 4, 2 lanes, member sums into an LDS y, then one
-// global atomic per distinct det.  VT stored value type, XT vector type.
+// tile's x (one gather per distinct det), segment descriptors and membership
+// indices staged in LDS, its class runs with lane groups of 64, 32, 16, 8, 4,
+// 2 lanes, member sums into an LDS y, then one global atomic per distinct
+// det.  VT stored value type, XT vector type.
 template <typename VT, typename XT>
 __global__ __launch_bounds__(256) void k_seg_spmv_tiles(const uint32_t* __restrict__ rb, const uint32_t* __restrict__ re,
                                                         const uint32_t* __restrict__ u0, const uint32_t* __restrict__ nu,
                                                         const uint32_t* __restrict__ tab, const SegDesc* __restrict__ desc,
                                                         const uint16_t* __restrict__ loc, const VT* __restrict__ val,
-                                                        const XT* __restrict__ x, XT* __restrict__ y) {
-  __shared__ XT xs[kTileMax];
-  __shared__ XT ys[kTileMax];
+                                                        const XT* __restrict__ xi, XT* __restrict__ yi,
+                                                        const XT* __restrict__ xb, XT* __restrict__ yb) {
+  __shared__ TileLds<XT> L;
   const uint64_t t = blockIdx.x;
-  const uint32_t U = nu[t];
+  const uint32_t nut = nu[t], U = nut & 0x7FFFFFFFu;
+  const XT* __restrict__ x = (nut >> 31) ? xb : xi;  // A tiles: beta-order copies
+  XT* __restrict__ y = (nut >> 31) ? yb : yi;
   const uint32_t* __restrict__ tb = tab + u0[t];
+  uint32_t nseg[6], k0[6], m0[6];
+  uint64_t voff0[6], moff0[6];
+  uint32_t ks = 0, ms = 0;
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    const uint32_t b = rb[t * 6 + c], e = re[t * 6 + c];
+    nseg[c] = e - b;
+    k0[c] = ks;
+    m0[c] = ms;
+    voff0[c] = moff0[c] = 0;
+    if (b < e) {
+      const SegDesc F = desc[b], G = desc[e - 1];
+      voff0[c] = F.voff;
+      moff0[c] = F.moff;
+      ks += e - b;
+      ms += G.moff + G.m - F.moff;
+    }
+  }
   for (uint32_t u = threadIdx.x; u < U; u += 256) {
-    xs[u] = x[tb[u]];
-    ys[u] = (XT)0;
+    L.xs[u] = x[tb[u]];
+    L.ys[u] = (XT)0;
+  }
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    const uint32_t b = rb[t * 6 + c];
+    for (uint32_t k = threadIdx.x; k < nseg[c]; k += 256) {
+      const SegDesc D = desc[b + k];
+      L.sv[k0[c] + k] = (uint32_t)(D.voff - voff0[c]);
+      L.sm[k0[c] + k] = ((uint32_t)(D.moff - moff0[c]) << 8) | D.m;
+    }
+    const uint32_t len = (c + 1 < 6 ? m0[c + 1] : ms) - m0[c];
+    for (uint32_t i = threadIdx.x; i < len; i += 256) L.sl[m0[c] + i] = loc[moff0[c] + i];
   }
   __syncthreads();
-  const uint32_t* rbt = rb + t * 6;
-  const uint32_t* ret = re + t * 6;
-  tile_class_run<64, 2, VT, XT>(rbt[0], ret[0], desc, loc, val, xs, ys);
-  tile_class_run<32, 2, VT, XT>(rbt[1], ret[1], desc, loc, val, xs, ys);
-  tile_class_run<16, 2, VT, XT>(rbt[2], ret[2], desc, loc, val, xs, ys);
-  tile_class_run<8, 4, VT, XT>(rbt[3], ret[3], desc, loc, val, xs, ys);
-  tile_class_run<4, 4, VT, XT>(rbt[4], ret[4], desc, loc, val, xs, ys);
-  tile_class_run<2, 4, VT, XT>(rbt[5], ret[5], desc, loc, val, xs, ys);
+  // (lanes per segment, segments per lane group and round, steps per load batch)
+  tile_class_run<64, 2, 4, VT, XT>(nseg[0], k0[0], m0[0], voff0[0], val, L);
+  tile_class_run<32, 2, 4, VT, XT>(nseg[1], k0[1], m0[1], voff0[1], val, L);
+  tile_class_run<16, 2, 4, VT, XT>(nseg[2], k0[2], m0[2], voff0[2], val, L);
+  tile_class_run<8, 4, 4, VT, XT>(nseg[3], k0[3], m0[3], voff0[3], val, L);
+  tile_class_run<4, 4, 2, VT, XT>(nseg[4], k0[4], m0[4], voff0[4], val, L);
+  tile_class_run<2, 4, 1, VT, XT>(nseg[5], k0[5], m0[5], voff0[5], val, L);
   __syncthreads();
-  for (uint32_t u = threadIdx.x; u < U; u += 256) atomicAdd(&y[tb[u]], ys[u]);
+  for (uint32_t u = threadIdx.x; u < U; u += 256) atomicAdd(&y[tb[u]], L.ys[u]);
 }
 
 
 // Off-diagonal part, one wavefront per (segment, group of 64 T members) for
-// m > 64.  The member x sit in LDS; each lane owns the members
-// j0 + lane + 64 c (c < T) of its group and keeps their sums in registers
-// while the steps d run outermost, so the
-// forward slot (j, j + d) and the backward slot (j - d, j) of a step are read
-// from the same row of the segment within a short window (cache hits for the
-// second read).  Every value load is unconditional at an in-bounds (clamped)
-// slot and the unused half-step term is masked, so 2T loads per step are in
-// flight per lane.
+// m > 64 (one vector).  The member x sit twice in LDS (sx[k] = sx[k + m] =
+// x_k), so the partners j + d and j - d mod m are read without wrapping; each
+// lane owns the members j0 + lane + 64 c (c < T, and only the chunks that hold
+// members) and keeps their sums in registers while the steps d run outermost,
+// so the forward slot (j, j + d) and the backward slot (j - d, j) of a step
+// are read from the same row of the segment within a short window (cache hits
+// for the second read).  Value loads are unconditional at in-bounds (clamped)
+// slots and the unused half-step term is masked.
 constexpr int kSegChunks = 4;
-template <int T, int NV, typename VT, typename XT>
+template <int NC, typename VT, typename XT>
+__device__ __forceinline__ void seg_large_rows(const SegDesc& D, uint32_t j0, const VT* __restrict__ val,
+                                               const XT* sx, const uint32_t* __restrict__ mem, XT* __restrict__ y) {
+  const uint32_t m = D.m, mh = m >> 1, t = threadIdx.x;
+  const bool even = (m & 1) == 0;
+  XT acc[NC];
+  uint32_t jj[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const uint32_t j = j0 + kWave * c + t;
+    jj[c] = j < m ? j : m - 1;
+    acc[c] = (XT)0;
+  }
+  const VT* __restrict__ vd = val + D.voff;
+  for (uint32_t d = 1; d <= mh; d++, vd += m) {
+    // last step of an even m: mh slots, the pair (i, i + mh) at slot i
+    const bool half = even && d == mh;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const uint32_t j = jj[c];
+      const uint32_t fi = half && j >= mh ? j - mh : j;
+      const uint32_t qi = j >= d ? j - d : j + m - d;
+      const XT vf = (XT)vd[fi];
+      const XT vb = half ? (XT)0 : (XT)vd[half ? 0 : qi];
+      acc[c] += vf * sx[j + d] + vb * sx[j + m - d];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const uint32_t j = j0 + kWave * c + t;
+    if (j < m) atomicAdd(&y[mem[D.moff + j]], acc[c]);
+  }
+}
+
+template <int T, typename VT, typename XT>
 __global__ __launch_bounds__(64) void k_seg_spmv_large(const SegDesc* __restrict__ desc, uint32_t s0, uint32_t s1,
                                                        const uint32_t* __restrict__ mem, const VT* __restrict__ val,
                                                        const XT* __restrict__ x, XT* __restrict__ y) {
@@ -2114,53 +2263,22 @@ __global__ __launch_bounds__(64) void k_seg_spmv_large(const SegDesc* __restrict
   if (s >= s1) return;
   const SegDesc D = desc[s];
   const uint32_t m = D.m, mh = m >> 1, t = threadIdx.x;
-  if (blockIdx.y * (kWave * T) >= m) return;
+  const uint32_t j0 = blockIdx.y * (kWave * T);
+  if (j0 >= m) return;
   XT* sx = reinterpret_cast<XT*>(seg_lds);
   for (uint32_t j = t; j < m; j += kWave) {
-    XT v[NV];
-    load_x<NV, XT>(x, mem[D.moff + j], v);
-    for (int k = 0; k < NV; k++) sx[j * NV + k] = v[k];
+    const XT v = x[mem[D.moff + j]];
+    sx[j] = v;
+    sx[j + m] = v;
   }
   __syncthreads();
-  const VT* __restrict__ vs = val + D.voff;
-  const bool even = (m & 1) == 0;
-  {
-    const uint32_t j0 = blockIdx.y * (kWave * T);
-    XT acc[T][NV];
-    uint32_t jj[T], p[T], q[T];
-#pragma unroll
-    for (int c = 0; c < T; c++) {
-      const uint32_t j = j0 + kWave * c + t;
-      jj[c] = p[c] = q[c] = j < m ? j : 0;
-      for (int k = 0; k < NV; k++) acc[c][k] = (XT)0;
-    }
-    const VT* __restrict__ vd = vs;
-    for (uint32_t d = 1; d <= mh; d++, vd += m) {
-      // last step of an even m: mh slots, the pair (i, i + mh) at slot i
-      const bool half = even && d == mh;
-      XT vf[T], vb[T];
-#pragma unroll
-      for (int c = 0; c < T; c++) {
-        p[c] = p[c] + 1 == m ? 0 : p[c] + 1;
-        q[c] = q[c] == 0 ? m - 1 : q[c] - 1;
-        const uint32_t fi = half && jj[c] >= mh ? jj[c] - mh : jj[c];
-        vf[c] = (XT)vd[fi];
-        vb[c] = (XT)vd[half ? 0 : q[c]];
-      }
-#pragma unroll
-      for (int c = 0; c < T; c++) {
-        const XT wb = half ? (XT)0 : vb[c];
-        for (int k = 0; k < NV; k++) acc[c][k] += vf[c] * sx[p[c] * NV + k] + wb * sx[q[c] * NV + k];
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < T; c++) {
-      const uint32_t j = j0 + kWave * c + t;
-      if (j < m) {
-        const uint32_t det = mem[D.moff + j];
-        for (int k = 0; k < NV; k++) atomicAdd(&y[(uint64_t)det * NV + k], acc[c][k]);
-      }
-    }
+  const uint32_t nc = min((uint32_t)T, (m - j0 + kWave - 1) / kWave);  // chunks holding members
+  static_assert(T == 4, "chunk dispatch below");
+  switch (nc) {
+    case 1: seg_large_rows<1>(D, j0, val, sx, mem, y); break;
+    case 2: seg_large_rows<2>(D, j0, val, sx, mem, y); break;
+    case 3: seg_large_rows<3>(D, j0, val, sx, mem, y); break;
+    default: seg_large_rows<4>(D, j0, val, sx, mem, y); break;
   }
 }
 
@@ -2610,14 +2728,17 @@ int build_seg_layout(shci_ctx* ctx) {
   if (M >= (1ull << 31)) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment layout: n_dets * keys per det >= 2^31");
   if (K.tag_shift + 2 > 64) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment keys exceed 64 bits");
   // The sort needs ~36 B per key; value stores of a previous build are
-  // reallocated after it anyway, so drop them first when memory is short.
+  // reallocated after it anyway, so drop them first when memory is short
+  // (free memory plus the context's cached blocks).
   {
     size_t free_b = 0, total_b = 0;
     HIPC(hipMemGetInfo(&free_b, &total_b));
+    for (const DevBuf& c : ctx->cache) free_b += c.bytes;
     if ((double)free_b < 40.0 * (double)M) {
       DevBuf* big[] = {&ctx->sv64, &ctx->sv32, &ctx->seg_mem, &ctx->seg_desc, &ctx->seg_core, &ctx->seg_type,
                        &ctx->seg_loc, &ctx->tile_tab, &ctx->dav, &ctx->tmp};
       for (DevBuf* b : big) release(ctx, *b);
+      cache_trim(ctx);
     }
   }
   // Keys of every (det, core) membership, one radix sort.
@@ -2782,8 +2903,8 @@ int build_seg_layout(shci_ctx* ctx) {
   }
   k_tile_tables<<<T, 256, 0, st>>>(P<uint32_t>(ctx->tile_rb), P<uint32_t>(ctx->tile_re), P<uint32_t>(ctx->tile_u0),
                                    desc, S, n_mem, P<uint32_t>(ctx->seg_mem), P<uint8_t>(ctx->seg_type),
-                                   P<uint32_t>(ctx->dnord_idx), P<uint32_t>(ctx->tile_tab),
-                                   P<uint32_t>(ctx->tile_nu), P<uint16_t>(ctx->seg_loc));
+                                   P<uint32_t>(ctx->tile_tab), P<uint32_t>(ctx->tile_nu),
+                                   P<uint16_t>(ctx->seg_loc));
   HIPC(hipGetLastError());
   ctx->n_tiles = T;
   return SHCI_OK;
@@ -2894,12 +3015,9 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
   const VT* val = sizeof(VT) == 4 ? reinterpret_cast<const VT*>(ctx->sv32.p) : reinterpret_cast<const VT*>(ctx->sv64.p);
   if (!val && ctx->n_slots) return fail(ctx, SHCI_ERR_STATE, "requested precision not stored");
   k_diag_apply<NV, XT><<<div_up(n, 256), 256, 0, st>>>(P<double>(ctx->diag), shift, x, y, n);
-  // A segments address beta-order copies of x and y: all of them in the block
-  // product, only the one-wavefront classes (m > 64) in the single product
-  // (the tiles hold internal det ids).
-  const int a_classes = NV == 4 ? kSegClasses : 3;
+  // A segments (and A tiles) address beta-order copies of x and y.
   bool have_a = false;
-  for (int c = 0; c < a_classes; c++) have_a |= ctx->seg_cs[3 * c + kSegA + 1] > ctx->seg_cs[3 * c + kSegA];
+  for (int c = 0; c < kSegClasses; c++) have_a |= ctx->seg_cs[3 * c + kSegA + 1] > ctx->seg_cs[3 * c + kSegA];
   XT* xb = nullptr;
   XT* yb = nullptr;
   const uint32_t* dnord = P<uint32_t>(ctx->dnord_idx);
@@ -2936,11 +3054,11 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
         }
       }
     } else if (c < 3) {
-      const size_t lds = (size_t)ctx->seg_maxm[r] * NV * sizeof(XT);
+      const size_t lds = (size_t)ctx->seg_maxm[r] * 2 * sizeof(XT);
       if (lds > 65536) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment too large for the one-wavefront kernel");
       // one wavefront per (segment, 256 members)
       const dim3 grid(s1 - s0, div_up(ctx->seg_maxm[r], kWave * kSegChunks));
-      k_seg_spmv_large<kSegChunks, NV, VT, XT><<<grid, kWave, lds, st>>>(desc, s0, s1, mem, val, xr, yr);
+      k_seg_spmv_large<kSegChunks, VT, XT><<<grid, kWave, lds, st>>>(desc, s0, s1, mem, val, xr, yr);
     }
   }
   if constexpr (NV == 1) {
@@ -2948,7 +3066,7 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
     if (ctx->n_tiles)
       k_seg_spmv_tiles<VT, XT><<<ctx->n_tiles, 256, 0, st>>>(
           P<uint32_t>(ctx->tile_rb), P<uint32_t>(ctx->tile_re), P<uint32_t>(ctx->tile_u0), P<uint32_t>(ctx->tile_nu),
-          P<uint32_t>(ctx->tile_tab), desc, P<uint16_t>(ctx->seg_loc), val, x, y);
+          P<uint32_t>(ctx->tile_tab), desc, P<uint16_t>(ctx->seg_loc), val, x, y, xb, yb);
   }
   if (have_a) k_perm_add<NV, XT><<<div_up(n, 256), 256, 0, st>>>(dnord, yb, y, n);
   HIPC(hipGetLastError());
@@ -3447,6 +3565,7 @@ int shci_ctx_destroy(shci_ctx* ctx) {
                     &ctx->va, &ctx->vb, &ctx->scal, &ctx->part, &ctx->xv, &ctx->yv, &ctx->x32, &ctx->y32,
                     &ctx->dav};
   for (DevBuf* b : bufs) release(ctx, *b);
+  cache_trim(ctx);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -3936,6 +4055,7 @@ int shci_clear(shci_ctx* ctx) {
                     &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb, &ctx->va, &ctx->vb, &ctx->xv,
                     &ctx->yv, &ctx->x32, &ctx->y32, &ctx->dav};
   for (DevBuf* b : bufs) release(ctx, *b);
+  cache_trim(ctx);
   ctx->n = 0;
   ctx->nnz = 0;
   ctx->n_seg = ctx->n_mem = ctx->n_slots = ctx->n_pairs = ctx->n_tiles = 0;
