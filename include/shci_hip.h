@@ -45,11 +45,14 @@ enum {
   SHCI_KEEP_ZEROS = 1u << 2,   /* keep |H_ij| < 1e-12 connections (exact RDM adjoint) */
   SHCI_DEVICE_PTRS = 1u << 3,  /* vector/det arguments are device pointers */
   SHCI_FP32 = 1u << 4,         /* SpMV in FP32 storage / FP32 row accumulation */
-  SHCI_LAYOUT_SEGMENTS = 1u << 5 /* build: segment layout instead of the CSR: dets
+  SHCI_LAYOUT_SEGMENTS = 1u << 5, /* build: segment layout instead of the CSR: dets
                                     grouped by shared (alpha-1, beta-1), (alpha-2) and
                                     (beta-2) cores, each connected pair stored once as a
                                     value slot of its owning segment, no column indices;
                                     faster SpMV, less memory; no row export */
+  SHCI_F64_VECTORS = 1u << 6   /* shci_time_spmv: FP64 vectors with the store SHCI_FP32
+                                  selects (FP32 values and FP64 accumulation: the
+                                  mixed Davidson's shift and energy re-evaluation) */
 };
 
 /* Davidson modes. */
@@ -141,7 +144,8 @@ int shci_export_upper(shci_ctx* ctx, int64_t* row_ptr, int64_t* cols, double* va
  * singles, diagonal, count, scan+alloc, fill, compact, total. */
 int shci_get_build_timings(shci_ctx* ctx, float* phases);
 /* Average kernel time of `reps` SpMVs (device-resident vectors) and the
- * algorithmic bytes one SpMV moves.  flags: SHCI_FP32 selects the FP32 store. */
+ * algorithmic bytes one SpMV moves.  flags: SHCI_FP32 selects the FP32 store,
+ * SHCI_F64_VECTORS FP64 vectors with it. */
 int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, double* bytes);
 
 /* Average kernel time of `reps` FP32 block products with k = 4 device-resident

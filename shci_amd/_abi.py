@@ -17,6 +17,7 @@ SHCI_KEEP_ZEROS = 1 << 2
 SHCI_DEVICE_PTRS = 1 << 3
 SHCI_FP32 = 1 << 4
 SHCI_LAYOUT_SEGMENTS = 1 << 5
+SHCI_F64_VECTORS = 1 << 6
 SHCI_DAVIDSON_FP64 = 0
 SHCI_DAVIDSON_MIXED = 1
 

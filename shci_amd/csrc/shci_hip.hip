@@ -2001,6 +2001,14 @@ __device__ __forceinline__ void load_x(const XT* __restrict__ x, uint32_t det, X
   }
 }
 
+// w if k, else 0, as a product: the compiler keeps the load of w unconditional
+// (a select on k feeding a widening conversion is turned into a branch around
+// the load that waits for it).  Stored values are finite.
+template <typename XT, typename VT>
+__device__ __forceinline__ XT keep_if(VT w, bool k) {
+  return (XT)w * (k ? (XT)1 : (XT)0);
+}
+
 // Value of the lane whose byte address (4 * lane) is a (ds_bpermute).  All
 // lanes of the wave must be active.
 __device__ __forceinline__ float lane_read(float v, int a) {
@@ -2083,7 +2091,7 @@ __device__ __forceinline__ void seg_group_sums(const SegDesc (&D)[NB], const XT 
     const VT w = val[D[b].voff + (half ? (uint64_t)(mh - 1) * mb + hj : 0ull)];
     const uint32_t pj = j + mh >= mb ? j + mh - mb : j + mh;
     const XT xh = lane_read(xj[b], (int)((gbase + (valid ? pj : j)) << 2));
-    acc[b] = fma(half ? (XT)w : (XT)0, xh, acc[b]);
+    acc[b] = fma(keep_if<XT>(w, half), xh, acc[b]);
     // (j - T) mod m; T < 2m for the segments of a lane-group class: T is
     // (S/2 - 1) rounded up to a multiple of CH <= S at most, and m > S/2
     // (or m = S = 2, T = 0)
@@ -2243,7 +2251,7 @@ __device__ __forceinline__ void seg_large_rows(const SegDesc& D, uint32_t j0, co
       const uint32_t fi = half && j >= mh ? j - mh : j;
       const uint32_t qi = j >= d ? j - d : j + m - d;
       const XT vf = (XT)vd[fi];
-      const XT vb = half ? (XT)0 : (XT)vd[half ? 0 : qi];
+      const XT vb = keep_if<XT>(vd[half ? 0 : qi], !half);
       acc[c] += vf * sx[j + d] + vb * sx[j + m - d];
     }
   }
@@ -3873,10 +3881,16 @@ int shci_davidson(shci_ctx* ctx, uint32_t n_states, const double* init, double t
     D.tmp = vb + (2 * K + 2 * ns) * nst;
     D.shift = 0.0;
     if (mixed) {
-      // Shift = FP64 Rayleigh quotient of the initial ground-state vector
-      // (FP32 values, FP64 accumulation and diagonal): keeps FP32 values of
-      // H - shift small so FP32 row sums lose no absolute accuracy.
-      CHECK_RC(launch_spmv<float, double, double>(ctx, P<float>(ctx->v32), 0.0, init_i, y64));
+      // Shift = Rayleigh quotient of the initial ground-state vector: keeps
+      // FP32 values of H - shift small so FP32 row sums lose no absolute
+      // accuracy.  It is subtracted on the FP64 diagonal and added back to
+      // the eigenvalues, so any nearby value serves: an FP32 product (vectors
+      // staged in w64's storage) with FP64 dots.
+      float* x32 = reinterpret_cast<float*>(w64);
+      float* y32 = x32 + nst;
+      k_copy_scale<double, float><<<g, 256, 0, st>>>(init_i, x32, 1.0, n);
+      CHECK_RC(launch_spmv<float, float, float>(ctx, P<float>(ctx->v32), 0.0, x32, y32));
+      k_copy_scale<float, double><<<g, 256, 0, st>>>(y32, y64, 1.0, n);
       const double* b1[1] = {y64};
       const double* b2[1] = {init_i};
       double num, den;
@@ -4078,13 +4092,14 @@ int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, doubl
   HIPC(hipSetDevice(ctx->device));
   const uint64_t n = ctx->n;
   const bool fp32 = flags & SHCI_FP32;
+  const bool f64v = fp32 && (flags & SHCI_F64_VECTORS);
   if (fp32 && !has_fp32(ctx)) return fail(ctx, SHCI_ERR_STATE, "FP32 store not built");
   if (!fp32 && !has_fp64(ctx)) return fail(ctx, SHCI_ERR_STATE, "FP64 store not built");
   CHECK_RC(ensure(ctx, ctx->xv, n * 8));
   CHECK_RC(ensure(ctx, ctx->yv, n * 8));
   const hipStream_t st = ctx->stream;
   // x = 1/sqrt(n) everywhere (values do not change the traffic).
-  if (fp32) {
+  if (fp32 && !f64v) {
     std::vector<float> h(n, (float)(1.0 / std::sqrt((double)n)));
     HIPC(hipMemcpyAsync(ctx->xv.p, h.data(), n * 4, hipMemcpyHostToDevice, st));
     HIPC(hipStreamSynchronize(st));
@@ -4094,6 +4109,8 @@ int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, doubl
     HIPC(hipStreamSynchronize(st));
   }
   auto once = [&]() -> int {
+    if (f64v)
+      return launch_spmv<float, double, double>(ctx, P<float>(ctx->v32), 0.0, P<double>(ctx->xv), P<double>(ctx->yv));
     if (fp32)
       return launch_spmv<float, float, float>(ctx, P<float>(ctx->v32), 0.0, reinterpret_cast<float*>(ctx->xv.p),
                                               reinterpret_cast<float*>(ctx->yv.p));
@@ -4109,16 +4126,17 @@ int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, doubl
   if (ms_avg) *ms_avg = ms / reps;
   if (bytes) {
     const double F = (double)ctx->nnz;  // off-diagonal full-symmetric entries
-    const double vs = fp32 ? 4.0 : 8.0;
+    const double vs = fp32 ? 4.0 : 8.0;  // value bytes
+    const double xs = fp32 && !f64v ? 4.0 : 8.0;  // vector element bytes
     if (ctx->segl) {
       // every value slot once, member ids, segment descriptors, FP64
       // diagonal, x once, y once (the per-member x gathers and y atomics
       // reuse x and y from the caches and are not counted).
       *bytes = (double)ctx->n_slots * vs + (double)ctx->n_mem * 4.0 + (double)ctx->n_seg * sizeof(SegDesc) +
-               (double)n * 8.0 + 2.0 * (double)n * vs;
+               (double)n * 8.0 + 2.0 * (double)n * xs;
     } else {
       // values + columns, row start/end (int64 each), FP64 diagonal, x once, y once.
-      *bytes = F * (vs + 4.0) + (double)n * 16.0 + (double)n * 8.0 + 2.0 * (double)n * vs;
+      *bytes = F * (vs + 4.0) + (double)n * 16.0 + (double)n * 8.0 + 2.0 * (double)n * xs;
     }
   }
   return SHCI_OK;

@@ -158,11 +158,15 @@ class SparseMatrix:
         self._built = False
 
     # -- timing helpers (bench / profiling) --
-    def time_spmv(self, fp32: bool = False, reps: int = 10):
+    def time_spmv(self, fp32: bool = False, reps: int = 10, f64_vectors: bool = False):
+        """Average time (ms) and algorithmic bytes of a device-resident SpMV.
+        ``fp32`` selects the FP32 store; ``f64_vectors`` pairs it with FP64
+        vectors (the mixed Davidson's shift and energy re-evaluation)."""
         ms = C.c_float()
         by = C.c_double()
-        self.ctx.check(self.ctx.lib.shci_time_spmv(self.ctx.handle, _abi.SHCI_FP32 if fp32 else 0, reps,
-                                                   C.byref(ms), C.byref(by)), "shci_time_spmv")
+        flags = (_abi.SHCI_FP32 if fp32 else 0) | (_abi.SHCI_F64_VECTORS if f64_vectors else 0)
+        self.ctx.check(self.ctx.lib.shci_time_spmv(self.ctx.handle, flags, reps, C.byref(ms), C.byref(by)),
+                       "shci_time_spmv")
         return ms.value, by.value
 
     def time_spmm(self, k: int = 4, reps: int = 10):
