@@ -155,12 +155,12 @@ int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, doubl
  * it, sparse_matrix.h:28-32). */
 int shci_time_spmm(shci_ctx* ctx, uint32_t k, int reps, float* ms_avg, double* bytes);
 
-/* Layout statistics (no reference counterpart; diagnostics): stats[13] =
+/* Layout statistics (no reference counterpart; diagnostics): stats[14] =
  * n_dets, stored off-diagonal entries (both directions), allocated entries
  * (both directions), segment layout?, segments, segment memberships,
  * segments of more than 64 members, value slots, mixed-double runs of the CSR
  * search, alpha groups, beta groups, CSR SpMV lanes per row, tiles of the
- * segments with at most 64 members. */
+ * segments with at most 64 members, distinct dets summed over the tiles. */
 int shci_layout_stats(shci_ctx* ctx, uint64_t* stats);
 
 /* Segment-layout ranges (no reference counterpart; diagnostics): for each of

@@ -697,11 +697,13 @@ __global__ __launch_bounds__(256) void k_rows(RowParams Pm) {
   }
 }
 
-// Sum of per-row lengths (the stored off-diagonal count).
-__global__ void k_sum_u32(const uint32_t* __restrict__ a, uint64_t n, unsigned long long* __restrict__ out) {
+// Sum of per-row lengths (the stored off-diagonal count); `mask` selects the
+// bits summed.
+__global__ void k_sum_u32(const uint32_t* __restrict__ a, uint64_t n, unsigned long long* __restrict__ out,
+                          uint32_t mask = 0xFFFFFFFFu) {
   __shared__ unsigned long long red[256];
   unsigned long long v = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) v += a[i];
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) v += a[i] & mask;
   red[threadIdx.x] = v;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
@@ -1360,6 +1362,34 @@ __global__ void k_seg_keys(SegKeyParams K, uint64_t total, uint64_t* __restrict_
   }
   keys[t] = ((uint64_t)type << K.tag_shift) | payload;
   vals[t] = (uint32_t)t;
+}
+
+// Keys without the region tag, 32 bits wide, when every region's key fits:
+// the three regions (C, A, B key indices) are then sorted separately.
+__global__ void k_seg_keys32(SegKeyParams K, uint64_t total, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  uint32_t type, k;
+  uint64_t i;
+  seg_decode(K, t, type, i, k);
+  uint64_t payload;
+  if (type == kSegC) {
+    const uint32_t k1 = k / K.nd, k2 = k - k1 * K.nd;
+    payload = ((uint64_t)K.m1a[(uint64_t)K.up_gid[i] * K.nu + k1] << K.bC) | K.m1b[(uint64_t)K.dn_gid[i] * K.nd + k2];
+  } else if (type == kSegA) {
+    payload = ((uint64_t)K.dn_gid[i] << K.bA) | K.mA[(uint64_t)K.up_gid[i] * K.PA + k];
+  } else {
+    payload = ((uint64_t)K.up_gid[i] << K.bB) | K.mB[(uint64_t)K.dn_gid[i] * K.PB + k];
+  }
+  keys[t] = (uint32_t)payload;
+  vals[t] = (uint32_t)t;
+}
+
+// Run heads of region-wise sorted 32-bit keys (a region start always opens a run).
+__global__ void k_head_flags32(const uint32_t* __restrict__ keys, uint32_t* __restrict__ flag, uint64_t n,
+                               uint64_t eC, uint64_t eA) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = (i == 0 || i == eC || i == eA || keys[i] != keys[i - 1]) ? 1u : 0u;
 }
 
 // A member is kept when the run owns at least one pair of it with another
@@ -2207,9 +2237,9 @@ __global__ __launch_bounds__(256) void k_seg_spmv_tiles(const uint32_t* __restri
   }
   __syncthreads();
   // (lanes per segment, segments per lane group and round, steps per load batch)
-  tile_class_run<64, 2, 4, VT, XT>(nseg[0], k0[0], m0[0], voff0[0], val, L);
-  tile_class_run<32, 2, 4, VT, XT>(nseg[1], k0[1], m0[1], voff0[1], val, L);
-  tile_class_run<16, 2, 4, VT, XT>(nseg[2], k0[2], m0[2], voff0[2], val, L);
+  tile_class_run<64, 2, 8, VT, XT>(nseg[0], k0[0], m0[0], voff0[0], val, L);
+  tile_class_run<32, 2, 8, VT, XT>(nseg[1], k0[1], m0[1], voff0[1], val, L);
+  tile_class_run<16, 2, 8, VT, XT>(nseg[2], k0[2], m0[2], voff0[2], val, L);
   tile_class_run<8, 4, 4, VT, XT>(nseg[3], k0[3], m0[3], voff0[3], val, L);
   tile_class_run<4, 4, 2, VT, XT>(nseg[4], k0[4], m0[4], voff0[4], val, L);
   tile_class_run<2, 4, 1, VT, XT>(nseg[5], k0[5], m0[5], voff0[5], val, L);
@@ -2231,8 +2261,9 @@ constexpr int kSegChunks = 4;
 template <int NC, typename VT, typename XT>
 __device__ __forceinline__ void seg_large_rows(const SegDesc& D, uint32_t j0, const VT* __restrict__ val,
                                                const XT* sx, const uint32_t* __restrict__ mem, XT* __restrict__ y) {
+  constexpr uint32_t U = sizeof(XT) == 8 ? 1 : 4;  // steps per batch of loads (FP64: register budget)
   const uint32_t m = D.m, mh = m >> 1, t = threadIdx.x;
-  const bool even = (m & 1) == 0;
+  const uint32_t fs = (m & 1) ? mh : mh - 1;  // steps with m slots
   XT acc[NC];
   uint32_t jj[NC];
 #pragma unroll
@@ -2241,18 +2272,45 @@ __device__ __forceinline__ void seg_large_rows(const SegDesc& D, uint32_t j0, co
     jj[c] = j < m ? j : m - 1;
     acc[c] = (XT)0;
   }
-  const VT* __restrict__ vd = val + D.voff;
-  for (uint32_t d = 1; d <= mh; d++, vd += m) {
-    // last step of an even m: mh slots, the pair (i, i + mh) at slot i
-    const bool half = even && d == mh;
+  const VT* __restrict__ vs = val + D.voff;
+  uint32_t d = 1;
+  for (; d + U - 1 <= fs; d += U) {
+    XT vf[U][NC], vb[U][NC];
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const VT* __restrict__ vd = vs + (uint64_t)(d + u - 1) * m;
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const uint32_t j = jj[c], du = d + u;
+        vf[u][c] = (XT)vd[j];
+        vb[u][c] = (XT)vd[j >= du ? j - du : j + m - du];
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const uint32_t j = jj[c], du = d + u;
+        acc[c] += vf[u][c] * sx[j + du] + vb[u][c] * sx[j + m - du];
+      }
+  }
+  for (; d <= fs; d++) {
+    const VT* __restrict__ vd = vs + (uint64_t)(d - 1) * m;
 #pragma unroll
     for (int c = 0; c < NC; c++) {
       const uint32_t j = jj[c];
-      const uint32_t fi = half && j >= mh ? j - mh : j;
-      const uint32_t qi = j >= d ? j - d : j + m - d;
-      const XT vf = (XT)vd[fi];
-      const XT vb = keep_if<XT>(vd[half ? 0 : qi], !half);
-      acc[c] += vf * sx[j + d] + vb * sx[j + m - d];
+      const XT f = (XT)vd[j], bk = (XT)vd[j >= d ? j - d : j + m - d];
+      acc[c] += f * sx[j + d] + bk * sx[j + m - d];
+    }
+  }
+  if ((m & 1) == 0) {
+    // last step of an even m: mh slots, the pair (i, i + mh) at slot i, read
+    // by both of its members (forward term only)
+    const VT* __restrict__ vd = vs + (uint64_t)(mh - 1) * m;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const uint32_t j = jj[c];
+      acc[c] += (XT)vd[j >= mh ? j - mh : j] * sx[j + mh];
     }
   }
 #pragma unroll
@@ -2749,21 +2807,41 @@ int build_seg_layout(shci_ctx* ctx) {
       cache_trim(ctx);
     }
   }
-  // Keys of every (det, core) membership, one radix sort.
+  // Keys of every (det, core) membership, radix-sorted: one 64-bit sort with
+  // the region tag on top, or, when each region's key fits 32 bits, three
+  // 32-bit sorts of the regions (the same order, a third less traffic).
+  const uint64_t eC = K.n * K.PC, eA = eC + K.n * K.PA;
+  const int wC = bits_for(ctx->r_alpha) + K.bC;
+  const int wA = bits_for(ctx->n_dgroups) + K.bA, wB = bits_for(ctx->n_ugroups) + K.bB;
+  const bool k32 = std::max({wC, wA, wB}) <= 32;
   TmpBuf keys(ctx), keys_s(ctx), vals(ctx), vals_s(ctx);
-  CHECK_RC(ensure(ctx, keys.b, M * 8));
+  CHECK_RC(ensure(ctx, keys.b, M * 8));  // later holds the head flags and their scan
   CHECK_RC(ensure(ctx, vals.b, M * 4));
-  CHECK_RC(ensure(ctx, keys_s.b, M * 8));
+  CHECK_RC(ensure(ctx, keys_s.b, M * (k32 ? 4 : 8)));
   CHECK_RC(ensure(ctx, vals_s.b, M * 4));
-  k_seg_keys<<<div_up(M, 256), 256, 0, st>>>(K, M, keys.p<uint64_t>(), vals.p<uint32_t>());
-  CHECK_RC(radix_sort_pairs(ctx, keys.p<uint64_t>(), keys_s.p<uint64_t>(), vals.p<uint32_t>(), vals_s.p<uint32_t>(),
-                            M, K.tag_shift + 2));
-  release(ctx, vals.b);
-  release(ctx, ctx->tmp);
-  // Runs of equal keys: head flags and their inclusive scan share the old key buffer.
   uint32_t* flag = reinterpret_cast<uint32_t*>(keys.b.p);
   uint32_t* incl = flag + M;
-  k_head_flags<<<div_up(M, 256), 256, 0, st>>>(keys_s.p<uint64_t>(), flag, M);
+  if (k32) {
+    uint32_t* k32in = keys.p<uint32_t>();
+    k_seg_keys32<<<div_up(M, 256), 256, 0, st>>>(K, M, k32in, vals.p<uint32_t>());
+    const uint64_t rb[4] = {0, eC, eA, M};
+    const int w[3] = {wC, wA, wB};
+    for (int r = 0; r < 3; r++)
+      if (rb[r + 1] > rb[r])
+        CHECK_RC(radix_sort_pairs(ctx, k32in + rb[r], keys_s.p<uint32_t>() + rb[r], vals.p<uint32_t>() + rb[r],
+                                  vals_s.p<uint32_t>() + rb[r], rb[r + 1] - rb[r], w[r]));
+    release(ctx, vals.b);
+    release(ctx, ctx->tmp);
+    k_head_flags32<<<div_up(M, 256), 256, 0, st>>>(keys_s.p<uint32_t>(), flag, M, eC, eA);
+  } else {
+    k_seg_keys<<<div_up(M, 256), 256, 0, st>>>(K, M, keys.p<uint64_t>(), vals.p<uint32_t>());
+    CHECK_RC(radix_sort_pairs(ctx, keys.p<uint64_t>(), keys_s.p<uint64_t>(), vals.p<uint32_t>(),
+                              vals_s.p<uint32_t>(), M, K.tag_shift + 2));
+    release(ctx, vals.b);
+    release(ctx, ctx->tmp);
+    // Runs of equal keys: head flags and their inclusive scan share the old key buffer.
+    k_head_flags<<<div_up(M, 256), 256, 0, st>>>(keys_s.p<uint64_t>(), flag, M);
+  }
   HIPC(hipGetLastError());
   release(ctx, keys_s.b);
   CHECK_RC(incl_scan(ctx, flag, incl, M));
@@ -4195,6 +4273,17 @@ int shci_layout_stats(shci_ctx* ctx, uint64_t* stats) {
   stats[10] = ctx->n_dgroups;
   stats[11] = (uint64_t)ctx->lanes;
   stats[12] = ctx->segl ? ctx->n_tiles : 0;
+  stats[13] = 0;  // distinct dets summed over the tiles (the tile kernel's gathers and atomics)
+  if (ctx->segl && ctx->n_tiles) {
+    CHECK_RC(ensure(ctx, ctx->scal, 64 * 8));
+    unsigned long long* out = reinterpret_cast<unsigned long long*>(ctx->scal.p) + 4;
+    HIPC(hipMemsetAsync(out, 0, 8, ctx->stream));
+    k_sum_u32<<<std::min<uint64_t>(1024, div_up(ctx->n_tiles, 256)), 256, 0, ctx->stream>>>(
+        P<uint32_t>(ctx->tile_nu), ctx->n_tiles, out, 0x7FFFFFFFu);
+    unsigned long long v = 0;
+    CHECK_RC(copy_back(ctx, &v, out));
+    stats[13] = v;
+  }
   return SHCI_OK;
 }
 

@@ -178,10 +178,10 @@ class SparseMatrix:
         return ms.value, by.value
 
     def layout_stats(self):
-        st = (C.c_uint64 * 13)()
+        st = (C.c_uint64 * 14)()
         self.ctx.check(self.ctx.lib.shci_layout_stats(self.ctx.handle, st), "shci_layout_stats")
         names = ["n_dets", "nnz", "nnz_alloc", "segments_layout", "segments", "memberships", "large_segments",
-                 "value_slots", "mixed_runs", "alpha_groups", "beta_groups", "lanes", "tiles"]
+                 "value_slots", "mixed_runs", "alpha_groups", "beta_groups", "lanes", "tiles", "tile_dets"]
         return dict(zip(names, list(st)))
 
     def segment_ranges(self):
