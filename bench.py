@@ -245,14 +245,14 @@ def main():
         if args.layout == "segments":
             # value slots written, member ids read, one det pair (16 B) per membership
             fill_bytes = ls["value_slots"] * 4 + ls["memberships"] * (4 + 16)
-            kname = "k_seg_fill_small<S>/k_seg_fill_large (Hamiltonian build, FP32 segments)"
+            kname = "k_seg_fill_c/k_seg_fill_small/k_seg_fill_large(_c) (Hamiltonian build, FP32 segments)"
         else:
             fill_bytes = F * (4 + 4) + (n_dets + 1) * 8 + n_dets * 16
             kname = "k_rows<fill> (Hamiltonian build, FP32 CSR)"
         roof = dict(kernel=kname, bound="hbm", achieved=fill_bytes / (fill_avg / 1e3) / 1e9, peak=HBM_PEAK_GBS,
                     unit="GB/s")
     else:
-        kname = ("segment SpMV: k_diag_apply + k_seg_spmv_small<S> + k_seg_spmv_large (FP32)"
+        kname = ("segment SpMV: k_diag_apply + k_seg_spmv_tiles + k_seg_spmv_large (FP32)"
                  if args.layout == "segments" else "k_spmv<fp32>")
         roof = dict(kernel=kname, bound="hbm", achieved=spmv32_bytes / (spmv32_ms / 1e3) / 1e9,
                     peak=HBM_PEAK_GBS, unit="GB/s")

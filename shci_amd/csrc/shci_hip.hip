@@ -1641,34 +1641,48 @@ __global__ __launch_bounds__(256) void k_tile_tables(const uint32_t* __restrict_
       base++;
     }
   }
-  uint32_t P = 2;  // power of two >= total, padded with keys that sort last
-  while (P < total) P <<= 1;
-  for (uint32_t i = total + tid; i < P; i += 256) {
-    key[i] = 0xFFFFFFFFu;
-    pay[i] = 0;
-  }
   __syncthreads();
-  for (uint32_t k = 2; k <= P; k <<= 1) {
-    for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
-      for (uint32_t i = tid; i < P; i += 256) {
-        const uint32_t ixj = i ^ jj;
-        if (ixj > i) {
-          const uint32_t ka = key[i], kb = key[ixj];
-          if ((ka > kb) == ((i & k) == 0)) {
-            key[i] = kb;
-            key[ixj] = ka;
-            const uint16_t pa = pay[i];
-            pay[i] = pay[ixj];
-            pay[ixj] = pa;
+  if (total <= 512) {
+    // small tables: rank = number of smaller keys (the keys are distinct;
+    // every lane reads the same key per iteration, an LDS broadcast)
+    for (uint32_t i = tid; i < total; i += 256) {
+      const uint32_t ki = key[i];
+      uint32_t r = 0;
+      for (uint32_t q = 0; q < total; q++) r += key[q] < ki ? 1u : 0u;
+      rank[i] = (uint16_t)r;
+      tab[(uint64_t)u0[t] + r] = ki;
+    }
+  } else {
+    // bitonic sort of (key, dense index) pairs, padded to a power of two
+    uint32_t P = 2;
+    while (P < total) P <<= 1;
+    for (uint32_t i = total + tid; i < P; i += 256) {
+      key[i] = 0xFFFFFFFFu;
+      pay[i] = 0;
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+      for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+        for (uint32_t i = tid; i < P; i += 256) {
+          const uint32_t ixj = i ^ jj;
+          if (ixj > i) {
+            const uint32_t ka = key[i], kb = key[ixj];
+            if ((ka > kb) == ((i & k) == 0)) {
+              key[i] = kb;
+              key[ixj] = ka;
+              const uint16_t pa = pay[i];
+              pay[i] = pay[ixj];
+              pay[ixj] = pa;
+            }
           }
         }
+        __syncthreads();
       }
-      __syncthreads();
     }
-  }
-  for (uint32_t i = tid; i < total; i += 256) {
-    rank[pay[i]] = (uint16_t)i;
-    tab[(uint64_t)u0[t] + i] = key[i];
+    for (uint32_t i = tid; i < total; i += 256) {
+      rank[pay[i]] = (uint16_t)i;
+      tab[(uint64_t)u0[t] + i] = key[i];
+    }
   }
   if (tid == 0) nu[t] = total | (isA ? 0x80000000u : 0u);
   __syncthreads();
@@ -2236,10 +2250,13 @@ __global__ __launch_bounds__(256) void k_seg_spmv_tiles(const uint32_t* __restri
     for (uint32_t i = threadIdx.x; i < len; i += 256) L.sl[m0[c] + i] = loc[moff0[c] + i];
   }
   __syncthreads();
-  // (lanes per segment, segments per lane group and round, steps per load batch)
-  tile_class_run<64, 2, 8, VT, XT>(nseg[0], k0[0], m0[0], voff0[0], val, L);
-  tile_class_run<32, 2, 8, VT, XT>(nseg[1], k0[1], m0[1], voff0[1], val, L);
-  tile_class_run<16, 2, 8, VT, XT>(nseg[2], k0[2], m0[2], voff0[2], val, L);
+  // (lanes per segment, segments per lane group and round, steps per load
+  // batch); the widening product (FP32 values, FP64 vectors) runs the wide
+  // classes one segment per lane group with longer batches
+  constexpr bool MX = sizeof(VT) != sizeof(XT);
+  tile_class_run<64, MX ? 1 : 2, MX ? 16 : 8, VT, XT>(nseg[0], k0[0], m0[0], voff0[0], val, L);
+  tile_class_run<32, MX ? 1 : 2, MX ? 16 : 8, VT, XT>(nseg[1], k0[1], m0[1], voff0[1], val, L);
+  tile_class_run<16, MX ? 1 : 2, 8, VT, XT>(nseg[2], k0[2], m0[2], voff0[2], val, L);
   tile_class_run<8, 4, 4, VT, XT>(nseg[3], k0[3], m0[3], voff0[3], val, L);
   tile_class_run<4, 4, 2, VT, XT>(nseg[4], k0[4], m0[4], voff0[4], val, L);
   tile_class_run<2, 4, 1, VT, XT>(nseg[5], k0[5], m0[5], voff0[5], val, L);
