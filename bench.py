@@ -252,7 +252,7 @@ def main():
         roof = dict(kernel=kname, bound="hbm", achieved=fill_bytes / (fill_avg / 1e3) / 1e9, peak=HBM_PEAK_GBS,
                     unit="GB/s")
     else:
-        kname = ("segment SpMV: k_diag_apply + k_seg_spmv_tiles + k_seg_spmv_large (FP32)"
+        kname = ("segment SpMV: k_diag_apply + k_seg_spmv_tiles + k_seg_spmv_wave + k_seg_spmv_large (FP32)"
                  if args.layout == "segments" else "k_spmv<fp32>")
         roof = dict(kernel=kname, bound="hbm", achieved=spmv32_bytes / (spmv32_ms / 1e3) / 1e9,
                     peak=HBM_PEAK_GBS, unit="GB/s")

@@ -2337,6 +2337,153 @@ __device__ __forceinline__ void seg_large_rows(const SegDesc& D, uint32_t j0, co
   }
 }
 
+// Value of lane l + 1 (lane 63: lane 0), a wave-wide DPP rotate.
+__device__ __forceinline__ float wave_next(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double wave_next(double v) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x134, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x134, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ float lane63(float v) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ double lane63(double v) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), 63);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// One-member rotation of a segment held in NC chunks (member j = 64 c + lane):
+// member j takes member j + 1's value, member m - 1 member 0's.  A wave-wide
+// rotate, then lane 63 of chunk c takes chunk c + 1's first member and the last
+// member takes the first.
+template <int NC, typename XT>
+__device__ __forceinline__ void seg_rotate(XT (&v)[NC], uint32_t lane, uint32_t cl, uint32_t ll) {
+  XT r[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) r[c] = wave_next(v[c]);
+  const XT first = lane63(r[0]);  // member 0
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    XT t = r[c];
+    if (c + 1 < NC) t = lane == 63 ? r[c + 1] : t;
+    if ((uint32_t)c == cl) t = lane == ll ? first : t;
+    v[c] = t;
+  }
+}
+
+// Segments of 64 < m <= 64 NC members, one wavefront each (m <= 256): the
+// travelling-sum scheme of seg_group_sums with one member per lane and chunk.
+// At step d member j loads the value of the pair (j, j + d mod m) (a row of the
+// segment read once, coalesced), adds v x_{j+d} to its sum and v x_j to its
+// travelling sum; x_{j+d} and the travelling sums rotate one member per step.
+// The half step of an even m reads x_{j+m/2} from LDS; the travelling sums are
+// delivered through LDS.
+template <int NC, typename VT, typename XT>
+__device__ __forceinline__ void seg_wave_rows(const SegDesc& D, const VT* __restrict__ val,
+                                              const uint32_t* __restrict__ mem, const XT* __restrict__ x,
+                                              XT* __restrict__ y, XT* sx, XT* sr) {
+  constexpr uint32_t U = sizeof(XT) == 8 ? 2 : 4;  // steps per batch of loads
+  using gptr = const __attribute__((address_space(1))) VT*;
+  const uint32_t m = D.m, lane = threadIdx.x;
+  const uint32_t fs = (m - 1) / 2;  // steps with m slots
+  const uint32_t cl = (m - 1) >> 6, ll = (m - 1) & 63;
+  XT xj[NC], xp[NC], R[NC], acc[NC];
+  uint32_t det[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const uint32_t j = 64 * c + lane;
+    det[c] = mem[D.moff + (j < m ? j : 0)];
+    const XT xv = x[det[c]];
+    xj[c] = j < m ? xv : (XT)0;
+    if (j < m) sx[j] = xv;
+    xp[c] = xj[c];
+    R[c] = acc[c] = (XT)0;
+  }
+  gptr vs = (gptr)(val + D.voff);
+  uint32_t d = 1;
+  for (; d + U - 1 <= fs; d += U) {
+    XT v[U][NC];
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const uint32_t j = 64 * c + lane;
+        v[u][c] = (XT)__builtin_nontemporal_load(vs + (uint64_t)(d + u - 1) * m + (j < m ? j : 0));
+      }
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      seg_rotate<NC, XT>(xp, lane, cl, ll);
+      seg_rotate<NC, XT>(R, lane, cl, ll);
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        acc[c] = fma(v[u][c], xp[c], acc[c]);
+        R[c] = fma(v[u][c], xj[c], R[c]);
+      }
+    }
+  }
+  for (; d <= fs; d++) {
+    XT v[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const uint32_t j = 64 * c + lane;
+      v[c] = (XT)__builtin_nontemporal_load(vs + (uint64_t)(d - 1) * m + (j < m ? j : 0));
+    }
+    seg_rotate<NC, XT>(xp, lane, cl, ll);
+    seg_rotate<NC, XT>(R, lane, cl, ll);
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      acc[c] = fma(v[c], xp[c], acc[c]);
+      R[c] = fma(v[c], xj[c], R[c]);
+    }
+  }
+  // the travelling sums: member j's now belongs to member j + fs mod m
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const uint32_t j = 64 * c + lane;
+    const uint32_t to = j + fs >= m ? j + fs - m : j + fs;
+    if (j < m) sr[to] = R[c];
+  }
+  __syncthreads();
+  const uint32_t mh = m >> 1;
+  const bool even = (m & 1) == 0;
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const uint32_t j = 64 * c + lane;
+    if (j >= m) continue;
+    XT sum = acc[c] + sr[j];
+    if (even) {
+      // last step: mh slots, the pair (i, i + mh) at slot i, read by both members
+      const VT w = val[D.voff + (uint64_t)(mh - 1) * m + (j >= mh ? j - mh : j)];
+      sum += (XT)w * sx[j + mh >= m ? j + mh - m : j + mh];
+    }
+    atomicAdd(&y[det[c]], sum);
+  }
+}
+
+// 64 < m <= 256: one wavefront per segment (seg_wave_rows), the chunk count
+// dispatched per segment.
+template <typename VT, typename XT>
+__global__ __launch_bounds__(64) void k_seg_spmv_wave(const SegDesc* __restrict__ desc, uint32_t s0, uint32_t s1,
+                                                      const uint32_t* __restrict__ mem, const VT* __restrict__ val,
+                                                      const XT* __restrict__ x, XT* __restrict__ y) {
+  __shared__ XT sx[256];
+  __shared__ XT sr[256];
+  const uint32_t s = s0 + blockIdx.x;
+  if (s >= s1) return;
+  const SegDesc D = desc[s];
+  const uint32_t nc = (D.m + 63) / 64;
+  switch (nc) {
+    case 2: seg_wave_rows<2, VT, XT>(D, val, mem, x, y, sx, sr); break;
+    case 3: seg_wave_rows<3, VT, XT>(D, val, mem, x, y, sx, sr); break;
+    default: seg_wave_rows<4, VT, XT>(D, val, mem, x, y, sx, sr); break;
+  }
+}
+
 template <int T, typename VT, typename XT>
 __global__ __launch_bounds__(64) void k_seg_spmv_large(const SegDesc* __restrict__ desc, uint32_t s0, uint32_t s1,
                                                        const uint32_t* __restrict__ mem, const VT* __restrict__ val,
@@ -3157,6 +3304,11 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
         }
       }
     } else if (c < 3) {
+      if (c == 2) {
+        // 64 < m <= 256: one wavefront per segment, rotating travelling sums
+        k_seg_spmv_wave<VT, XT><<<s1 - s0, kWave, 0, st>>>(desc, s0, s1, mem, val, xr, yr);
+        continue;
+      }
       const size_t lds = (size_t)ctx->seg_maxm[r] * 2 * sizeof(XT);
       if (lds > 65536) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment too large for the one-wavefront kernel");
       // one wavefront per (segment, 256 members)
