@@ -47,6 +47,21 @@ int shci_host_run_variation(void* sys, const double* eps_vars, unsigned n_eps, c
                             unsigned n_sched, double target_error, unsigned n_states, int mode, int device,
                             int verbose, double* energies);
 
+/* RDM file outputs (src/chem/rdm.cc), from shci_rdm's arrays: rdm1 n x n
+ * row-major (spin-summed), rdm2 packed a = p n + s, b = q n + r, a >= b.
+ * orb_order[i] = original (FCIDUMP) index of internal orbital i, as from
+ * shci_host_system_integrals; NULL = identity.  Return 0, or -1 with
+ * shci_host_io_last_error().
+ *   shci_host_dump_1rdm  RDM::dump_1rdm (rdm.cc:202-221): "p,r,1rdm", p <= r,
+ *                        |x| >= 1e-9, "%d,%d,%#.15g"
+ *   shci_host_dump_2rdm  RDM::dump_2rdm (rdm.cc:271-331): csv != 0 ->
+ *                        "p,q,r,s,2rdm" (p <= q, |x| >= 1e-9, %#.15g); csv == 0
+ *                        -> spatialRDM.txt (n, then "p q s r value" in the
+ *                        original order for |x| > 1e-6, %10.8g) */
+int shci_host_dump_1rdm(const char* path, unsigned n_orbs, const unsigned* orb_order, const double* rdm1);
+int shci_host_dump_2rdm(const char* path, unsigned n_orbs, const unsigned* orb_order, const double* rdm2, int csv);
+const char* shci_host_io_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

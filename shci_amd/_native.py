@@ -57,5 +57,8 @@ def host_lib():
     lib.shci_host_set_dets.argtypes = [_vp, _u64p, _u64p, C.c_size_t]
     lib.shci_host_select.restype = C.c_longlong
     lib.shci_host_select.argtypes = [_vp, _dp, C.c_uint, C.c_double]
+    lib.shci_host_dump_1rdm.argtypes = [C.c_char_p, C.c_uint, C.POINTER(C.c_uint), _dp]
+    lib.shci_host_dump_2rdm.argtypes = [C.c_char_p, C.c_uint, C.POINTER(C.c_uint), _dp, C.c_int]
+    lib.shci_host_io_last_error.restype = C.c_char_p
     _host = lib
     return lib

@@ -1412,12 +1412,24 @@ __global__ void k_seg_useful(SegKeyParams K, const uint32_t* __restrict__ vals_s
                                                            : drop_electrons(K.dn[i0], k0, K.rb);
     const uint32_t nrm = ty == kSegA ? K.ra : K.rb;
     const uint64_t ui = K.up[i], di = K.dn[i];
-    for (uint32_t q = a; q < b && !useful; q++) {
-      if (q == p) continue;
-      uint32_t tq, kq;
-      uint64_t l;
-      seg_decode(K, vals_s[q], tq, l, kq);
-      useful = seg_owns(ty, core, nrm, ui, di, K.up[l], K.dn[l]) ? 1u : 0u;
+    // four candidates per trip, so that their det loads are in flight together
+    constexpr uint32_t Q = 4;
+    for (uint32_t q0 = a; q0 < b && !useful; q0 += Q) {
+      uint64_t uq[Q], dq[Q];
+      bool ok[Q];
+#pragma unroll
+      for (uint32_t t = 0; t < Q; t++) {
+        const uint32_t q = q0 + t;
+        ok[t] = q < b && q != p;
+        uint32_t tq, kq;
+        uint64_t l;
+        seg_decode(K, vals_s[q < b ? q : a], tq, l, kq);
+        uq[t] = K.up[l];
+        dq[t] = K.dn[l];
+      }
+#pragma unroll
+      for (uint32_t t = 0; t < Q; t++)
+        if (ok[t] && seg_owns(ty, core, nrm, ui, di, uq[t], dq[t])) useful = 1u;
     }
   }
   use[p] = useful;
@@ -1588,9 +1600,9 @@ __global__ __launch_bounds__(256) void k_tile_tables(const uint32_t* __restrict_
   __shared__ uint16_t rank[kTileMax];
   __shared__ uint32_t wtot[4];
   const uint32_t t = blockIdx.x, tid = threadIdx.x;
-  for (uint32_t h = tid; h < kTileHash; h += 256) hk[h] = 0xFFFFFFFFu;
   uint64_t mb[6], me[6];
   bool isA = false;
+  uint32_t cnt = 0;
   for (int k = 0; k < 6; k++) {
     const uint32_t b = rb[(uint64_t)t * 6 + k], e = re[(uint64_t)t * 6 + k];
     mb[k] = me[k] = 0;
@@ -1598,30 +1610,36 @@ __global__ __launch_bounds__(256) void k_tile_tables(const uint32_t* __restrict_
       mb[k] = desc[b].moff;
       me[k] = e < S ? (uint64_t)desc[e].moff : n_mem;
       isA = type[b] == kSegA;
+      cnt += (uint32_t)(me[k] - mb[k]);
     }
   }
+  // hash slots: a power of two >= 2 x memberships, 256 .. kTileHash
+  uint32_t hbits = 8;
+  while ((1u << hbits) < 2 * cnt && (1u << hbits) < kTileHash) hbits++;
+  const uint32_t H = 1u << hbits;
+  for (uint32_t h = tid; h < H; h += 256) hk[h] = 0xFFFFFFFFu;
   __syncthreads();
   uint32_t k0 = 0;
   for (int k = 0; k < 6; k++) {
     const uint32_t len = (uint32_t)(me[k] - mb[k]);
     for (uint32_t i = tid; i < len; i += 256) {
       const uint32_t d = mem[mb[k] + i];
-      uint32_t h = (d * 2654435761u) >> 20;
+      uint32_t h = (d * 2654435761u) >> (32 - hbits);
       while (true) {
         const uint32_t old = atomicCAS(&hk[h], 0xFFFFFFFFu, d);
         if (old == 0xFFFFFFFFu || old == d) break;
-        h = (h + 1) & (kTileHash - 1);
+        h = (h + 1) & (H - 1);
       }
       sl[k0 + i] = (uint16_t)h;
     }
     k0 += len;
   }
   __syncthreads();
-  // occupied slots -> dense index: each thread counts 16 consecutive slots,
+  // occupied slots -> dense index: each thread counts PER consecutive slots,
   // then a block-wide exclusive scan of the counts
-  constexpr int PER = kTileHash / 256;
+  const uint32_t PER = H / 256;
   uint32_t c = 0;
-  for (int q = 0; q < PER; q++) c += hk[tid * PER + q] != 0xFFFFFFFFu ? 1u : 0u;
+  for (uint32_t q = 0; q < PER; q++) c += hk[tid * PER + q] != 0xFFFFFFFFu ? 1u : 0u;
   uint32_t inc = c;
   for (int o = 1; o < kWave; o <<= 1) {
     const uint32_t v = (uint32_t)__shfl_up((int)inc, o, kWave);
@@ -1632,7 +1650,7 @@ __global__ __launch_bounds__(256) void k_tile_tables(const uint32_t* __restrict_
   uint32_t base = inc - c;
   for (uint32_t wv = 0; wv < tid / kWave; wv++) base += wtot[wv];
   const uint32_t total = wtot[0] + wtot[1] + wtot[2] + wtot[3];
-  for (int q = 0; q < PER; q++) {
+  for (uint32_t q = 0; q < PER; q++) {
     const uint32_t h = tid * PER + q;
     if (hk[h] != 0xFFFFFFFFu) {
       cid[h] = (uint16_t)base;

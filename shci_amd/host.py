@@ -95,3 +95,36 @@ class HostSystem:
         if r < 0:
             raise RuntimeError("shci_host: " + _lib().shci_host_last_error().decode())
         return int(r)
+
+
+def _orb_order_arg(n_orbs, orb_order):
+    if orb_order is None:
+        return None, C.cast(None, C.POINTER(C.c_uint))
+    o = np.ascontiguousarray(orb_order, dtype=np.uint32)
+    if o.shape != (n_orbs,):
+        raise ValueError(f"orb_order must have {n_orbs} entries")
+    return o, o.ctypes.data_as(C.POINTER(C.c_uint))
+
+
+def dump_1rdm(path, rdm1, orb_order=None):
+    """RDM::dump_1rdm (rdm.cc:202-221): ``1rdm.csv`` from a spin-summed n x n
+    1-RDM (``Hamiltonian.rdm``), orbitals in the original order given by
+    ``orb_order`` (``HostSystem.integrals()[2]``; identity if None)."""
+    r1 = np.ascontiguousarray(rdm1, dtype=np.float64)
+    n = r1.shape[0]
+    keep, op = _orb_order_arg(n, orb_order)
+    if _lib().shci_host_dump_1rdm(str(path).encode(), n, op, r1.ctypes.data_as(_dp)) != 0:
+        raise RuntimeError("shci_host_dump_1rdm: " + _lib().shci_host_io_last_error().decode())
+
+
+def dump_2rdm(path, rdm2, n_orbs, orb_order=None, csv=True):
+    """RDM::dump_2rdm (rdm.cc:271-331): ``2rdm.csv`` (csv=True) or the
+    Dice-convention ``spatialRDM.txt`` (csv=False) from the packed 2-RDM of
+    ``Hamiltonian.rdm``."""
+    r2 = np.ascontiguousarray(rdm2, dtype=np.float64)
+    nn = n_orbs * n_orbs
+    if r2.shape != (nn * (nn + 1) // 2,):
+        raise ValueError("rdm2 must be the packed 2-RDM of n_orbs orbitals")
+    keep, op = _orb_order_arg(n_orbs, orb_order)
+    if _lib().shci_host_dump_2rdm(str(path).encode(), n_orbs, op, r2.ctypes.data_as(_dp), 1 if csv else 0) != 0:
+        raise RuntimeError("shci_host_dump_2rdm: " + _lib().shci_host_io_last_error().decode())
