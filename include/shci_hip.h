@@ -139,6 +139,24 @@ int shci_rdm(shci_ctx* ctx, uint32_t n_states, const double* coefs, double* rdm1
  * except row_ptr. */
 int shci_export_upper(shci_ctx* ctx, int64_t* row_ptr, int64_t* cols, double* vals);
 
+/* One heat-bath selection sweep of the variational loop (solver.h:435-495,
+ * the body of the loop over dets with ChemSystem::find_connected_dets,
+ * chem_system.cc:289-402; queues of chem_system.cc:115-287) on the device.
+ * up/dn: the n_dets variational dets (host, caller order; orbitals < n_orb,
+ * equal electron counts); coefs: n_states x n_dets (host); eps_tried_prev:
+ * n_dets values, in/out (1e100 for dets never tried; the sweep sets
+ * 1.000000001 * eps_min for every det it walks, solver.h:466).  For each det
+ * with eps_min = eps_var / max_state |c| below its eps_tried_prev: singles
+ * with S >= eps_min and |H_ij| >= eps_min, doubles with
+ * eps_min <= |H| < eps_tried_prev, not in the variational space.  *n_new
+ * receives the number of distinct new dets; shci_select_result copies them
+ * out, (up, dn) ascending (the reference appends them in the order of its
+ * hash set). */
+int shci_select(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n_dets, const double* coefs,
+                uint32_t n_states, double eps_var, double* eps_tried_prev, uint64_t* n_new);
+/* The last sweep's new dets: n must equal its *n_new. */
+int shci_select_result(shci_ctx* ctx, uint64_t* new_up, uint64_t* new_dn, uint64_t n);
+
 /* Timing of the most recent build phases and a timed SpMV loop, measured with
  * HIP events on the context's stream (milliseconds).  phases[8]: sort/group,
  * singles, diagonal, count, scan+alloc, fill, compact, total. */

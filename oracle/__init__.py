@@ -30,6 +30,12 @@ Davidson::diagonalize                  DavidsonTest.HilbertSystem known answer
                                        examples/C per-iteration "Davidson #k" trajectory
 RDM 1/2 (src/chem/rdm.cc)              trace / partial-trace identities; energy identity
                                        rdm.cc:731-745 vs the eigenvalue; finite differences
+selection sweep                        tests/test_oracle_selection.py: examples/C per-
+(src/solver/solver.h:435-495,          iteration det counts of the reference's log with
+src/chem/chem_system.cc:115-402)       this selection driving the loop; hand-built
+                                       cases (a det whose only connections are known);
+                                       the criteria's limits (eps -> 0 gives every det
+                                       within two excitations; eps_tried_prev skips)
 =====================================  ===================================================
 
 The reference itself cannot be compiled here (its submodules fgpl, hps, eigen,
@@ -87,6 +93,9 @@ def _load():
     lib.orc_sm_davidson.argtypes = [C.c_void_p, C.c_uint, _dp, C.c_double, _dp, _dp, C.POINTER(C.c_int), _dp,
                                     C.c_int, C.POINTER(C.c_int)]
     lib.orc_rdm.argtypes = [C.c_void_p, C.c_uint, _dp, _dp, _dp]
+    lib.orc_select.argtypes = [C.c_void_p, _u64p, _u64p, C.c_size_t, _dp, C.c_uint, C.c_double, _dp,
+                               C.POINTER(C.c_size_t)]
+    lib.orc_select_result.argtypes = [C.c_void_p, _u64p, _u64p]
     _lib = lib
     return lib
 
@@ -237,6 +246,21 @@ class System:
         _check(_load().orc_rdm(self._h, coefs.shape[0], _dptr(coefs), _dptr(r1),
                                _dptr(r2) if want_2rdm else C.cast(None, _dp)))
         return r1, r2
+
+    def select(self, up, dn, coefs, eps_var, eps_tried_prev):
+        """One selection sweep over the dets (up, dn): the new dets, (up, dn)
+        ascending; ``eps_tried_prev`` (float64, one per det) updated in place."""
+        up = np.ascontiguousarray(up, dtype=np.uint64)
+        dn = np.ascontiguousarray(dn, dtype=np.uint64)
+        coefs = np.atleast_2d(np.ascontiguousarray(coefs, dtype=np.float64))
+        assert eps_tried_prev.dtype == np.float64 and eps_tried_prev.flags.c_contiguous
+        m = C.c_size_t()
+        _check(_load().orc_select(self._h, up.ctypes.data_as(_u64p), dn.ctypes.data_as(_u64p), len(up),
+                                  _dptr(coefs), coefs.shape[0], eps_var, _dptr(eps_tried_prev), C.byref(m)))
+        nu = np.zeros(m.value, dtype=np.uint64)
+        nd = np.zeros(m.value, dtype=np.uint64)
+        _check(_load().orc_select_result(self._h, nu.ctypes.data_as(_u64p), nd.ctypes.data_as(_u64p)))
+        return nu, nd
 
 
 def sparse_davidson(dim, entries, init, tol):

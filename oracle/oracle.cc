@@ -1145,12 +1145,110 @@ using namespace orc;
 
 static thread_local std::string g_err;
 
+
+// ---------------------------------------------------------------------------
+// One heat-bath selection sweep: src/solver/solver.h:435-495 with
+// ChemSystem::find_connected_dets (src/chem/chem_system.cc:289-402; second
+// rejection and active space off, all orbitals in one irrep), restated
+// without the sorted queues: every single and double excitation of a walked
+// det is tested against the queue criteria directly.
+//   single p -> r: S(p, r) > 0 and S >= eps_min (singles queue,
+//     chem_system.cc:237-259), |H_ij| >= eps_min (solver.h:456-459);
+//   double: |H| > 0 and eps_min <= |H| < eps_tried_prev, |H| the hci-queue
+//     value (chem_system.cc:115-196,261-287): same spin |(pr|qs) - (ps|qr)|,
+//     opposite spin |(pr|qs)| (p, r up; q, s down);
+//   never a det of the variational space.
+// ---------------------------------------------------------------------------
+static double selection_S(const Integrals& I, unsigned p, unsigned r, unsigned n_elecs) {
+  const unsigned n = I.n_orbs;
+  std::vector<double> el;
+  for (unsigned o = 0; o < n; o++) {
+    const double di = I.get_2b(p, r, o, o);
+    el.push_back(di);
+    if (o != p && o != r) el.push_back(di - I.get_2b(p, o, o, r));
+  }
+  std::sort(el.begin(), el.end());
+  double smin = I.get_1b(p, r), smax = smin;
+  for (unsigned i = 0; i < std::min(n_elecs - 1, 2 * n - 2); i++) {
+    smin += el[i];
+    smax += el[2 * n - 3 - i];
+  }
+  return std::max(std::abs(smin), std::abs(smax));
+}
+
+static std::vector<Det> select_sweep(const Integrals& I, const Chem& chem, const std::vector<Det>& dets,
+                                     const double* coefs, unsigned n_states, double eps_var, double* eps_tried) {
+  const unsigned n = I.n_orbs;
+  const size_t nd = dets.size();
+  const unsigned ne = (unsigned)(__builtin_popcountll(dets[0].up) + __builtin_popcountll(dets[0].dn));
+  std::vector<double> S((size_t)n * n, 0.0);
+  for (unsigned p = 0; p < n; p++)
+    for (unsigned r = 0; r < n; r++)
+      if (p != r) S[p * n + r] = selection_S(I, p, r, ne);
+  std::unordered_set<Det, DetHash> var(dets.begin(), dets.end());
+  const uint64_t all = n < 64 ? (1ull << n) - 1 : ~0ull;
+  std::vector<std::vector<Det>> found(omp_get_max_threads());
+#pragma omp parallel for schedule(dynamic, 16)
+  for (size_t i = 0; i < nd; i++) {
+    auto& out = found[omp_get_thread_num()];
+    double mc = coefs[i];
+    for (unsigned s = 1; s < n_states; s++)
+      if (std::abs(coefs[s * nd + i]) > std::abs(mc)) mc = coefs[s * nd + i];
+    const double emin = eps_var / std::abs(mc), emax = eps_tried[i];
+    if (emin >= emax) continue;
+    const Det& d = dets[i];
+    auto keep = [&](const Det& c, bool single) {
+      if (var.count(c)) return;
+      if (single && std::abs(chem.elem(d, c, 1)) < emin) return;
+      out.push_back(c);
+    };
+    for (int spin = 0; spin < 2; spin++) {
+      const uint64_t str = spin ? d.dn : d.up;
+      const auto oc = occ(str), vi = occ(all & ~str);
+      for (unsigned p : oc)
+        for (unsigned r : vi) {
+          const double s = S[p * n + r];
+          if (s == 0.0 || s < emin) continue;
+          const uint64_t t = (str & ~(1ull << p)) | (1ull << r);
+          keep(spin ? Det{d.up, t} : Det{t, d.dn}, true);
+        }
+      for (size_t a = 0; a < oc.size(); a++)
+        for (size_t b = a + 1; b < oc.size(); b++)
+          for (size_t x = 0; x < vi.size(); x++)
+            for (size_t y = x + 1; y < vi.size(); y++) {
+              const unsigned p = oc[a], q = oc[b], r = vi[x], s = vi[y];
+              const double H = std::abs(I.get_2b(p, r, q, s) - I.get_2b(p, s, q, r));
+              if (H == 0.0 || H < emin || H >= emax) continue;
+              const uint64_t t = (str & ~(1ull << p) & ~(1ull << q)) | (1ull << r) | (1ull << s);
+              keep(spin ? Det{d.up, t} : Det{t, d.dn}, false);
+            }
+    }
+    const auto ou = occ(d.up), vu = occ(all & ~d.up), od = occ(d.dn), vd = occ(all & ~d.dn);
+    for (unsigned p : ou)
+      for (unsigned r : vu)
+        for (unsigned q : od)
+          for (unsigned s : vd) {
+            const double H = std::abs(I.get_2b(p, r, q, s));
+            if (H == 0.0 || H < emin || H >= emax) continue;
+            keep(Det{(d.up & ~(1ull << p)) | (1ull << r), (d.dn & ~(1ull << q)) | (1ull << s)}, false);
+          }
+    eps_tried[i] = 1.000000001 * emin;
+  }
+  std::vector<Det> res;
+  for (auto& v : found) res.insert(res.end(), v.begin(), v.end());
+  auto lt = [](const Det& a, const Det& b) { return a.up < b.up || (a.up == b.up && a.dn < b.dn); };
+  std::sort(res.begin(), res.end(), lt);
+  res.erase(std::unique(res.begin(), res.end()), res.end());
+  return res;
+}
+
 struct OrcSystem {
   Integrals ints;
   Chem chem;
   unsigned n_up = 0, n_dn = 0;
   std::vector<Det> dets;
   Hamiltonian ham;
+  std::vector<Det> selected;  // the last selection sweep's new dets
   OrcSystem() { chem.ints = &ints; }
 };
 
@@ -1402,6 +1500,30 @@ int orc_sm_davidson(void* mh, unsigned n_states, const double* init, double tol,
 
 // 1-RDM (n x n) and 2-RDM (packed, n^2(n^2+1)/2) of coefs (n_states x n_dets)
 // over the system's det list and stored Hamiltonian connections.
+int orc_select(void* h, const uint64_t* up, const uint64_t* dn, size_t n_dets, const double* coefs,
+               unsigned n_states, double eps_var, double* eps_tried, size_t* n_new) {
+  try {
+    auto* s = static_cast<OrcSystem*>(h);
+    std::vector<Det> dets(n_dets);
+    for (size_t i = 0; i < n_dets; i++) dets[i] = Det{up[i], dn[i]};
+    s->selected = select_sweep(s->ints, s->chem, dets, coefs, n_states, eps_var, eps_tried);
+    *n_new = s->selected.size();
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+int orc_select_result(void* h, uint64_t* up, uint64_t* dn) {
+  auto* s = static_cast<OrcSystem*>(h);
+  for (size_t i = 0; i < s->selected.size(); i++) {
+    up[i] = s->selected[i].up;
+    dn[i] = s->selected[i].dn;
+  }
+  return 0;
+}
+
 int orc_rdm(void* h, unsigned n_states, const double* coefs, double* rdm1, double* rdm2) {
   auto* s = static_cast<OrcSystem*>(h);
   ORC_TRY({

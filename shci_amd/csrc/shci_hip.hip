@@ -119,6 +119,14 @@ struct shci_ctx {
   DevBuf tile_tab;                      // tile det tables (internal det ids)
   DevBuf seg_loc;                       // membership -> index in its tile's table (m <= 64 only)
   DevBuf kept;                          // kept-pair counters of the segment fill
+  // Heat-bath selection: the singles and double-excitation queues of the
+  // current integrals (rebuilt when the integrals or the electron count
+  // change) and the new determinants of the last sweep.
+  uint32_t sel_ne = 0;                  // electron count of the queues (0 = not built)
+  DevBuf sq_S, sq_r, sq_len;            // n singles lists of n entries: S descending, target r
+  DevBuf hq_H, hq_rs, hq_len;           // n*n double lists of n*n entries: |H| descending, (r, s)
+  DevBuf sel_up, sel_dn;                // last sweep's new dets, (up, dn) ascending
+  uint64_t sel_n = 0;
   // Released device blocks kept for reuse (builds allocate and drop the same
   // temporaries every time; hipMalloc/hipFree of GB blocks per build would
   // cost more than the kernels).  Freed on memory pressure, clear and destroy.
@@ -3804,6 +3812,584 @@ int init_vectors(shci_ctx* ctx, Davidson<T>& D, const double* init_dev_internal)
   return SHCI_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Heat-bath selection: one sweep of the variational loop
+// (src/solver/solver.h:435-495) with ChemSystem::find_connected_dets
+// (src/chem/chem_system.cc:289-402; second rejection and active space off,
+// all orbitals in one irrep).
+//
+// Queues (chem_system.cc:115-287): singles list p holds (S(p, r), r) with
+// S > 0, S descending; double list L of an occupied spin-orbital pair holds
+// (|H|, r, s) with |H| > 0, descending.  A det with eps_min = eps_var / max
+// state |c| below its eps_tried_prev walks, for every occupied orbital, the
+// singles list while S >= eps_min and, for every occupied pair, the double
+// list over eps_min <= |H| < eps_tried_prev.  A candidate is kept when it is
+// not in the variational space and, for a single, when |H_ij| >= eps_min.
+// The kept candidates are radix-sorted on the packed (up, dn) key and made
+// unique: the new dets come out (up, dn) ascending.
+//
+// Double lists: same-spin pair p < q < n is list q(q-1)/2 + p with r < s,
+// r, s not in {p, q}, |(pr|qs) - (ps|qr)|; opposite-spin pair (a up, b down),
+// a <= b, is list n(n-1)/2 + b(b+1)/2 + a with r != a, s != b, |(ar|bs)|
+// (the up electron goes a -> r, the down one b -> s).  An up/down pair with
+// the up orbital above the down one uses the swapped list (chem_system.cc:
+// 357-375).
+// ---------------------------------------------------------------------------
+struct DetKey {
+  uint64_t hi, lo;
+};
+
+struct DetKeyDecomposer {
+  __host__ __device__ ::rocprim::tuple<uint64_t&, uint64_t&> operator()(DetKey& k) const {
+    return ::rocprim::tuple<uint64_t&, uint64_t&>(k.hi, k.lo);
+  }
+};
+
+// (up, dn) as the 2n-bit number up * 2^n + dn, so that the key order is the
+// (up, dn) order.
+__device__ __forceinline__ DetKey pack_det(uint64_t u, uint64_t d, uint32_t n) {
+  return n < 64 ? DetKey{u >> (64 - n), (u << n) | d} : DetKey{u, d};
+}
+
+__device__ __forceinline__ void unpack_det(const DetKey& k, uint32_t n, uint64_t& u, uint64_t& d) {
+  if (n < 64) {
+    d = k.lo & ((1ull << n) - 1ull);
+    u = (k.hi << (64 - n)) | (k.lo >> n);
+  } else {
+    u = k.hi;
+    d = k.lo;
+  }
+}
+
+__device__ __forceinline__ uint64_t det_hash(uint64_t u, uint64_t d) {
+  uint64_t x = u * 0x9E3779B97F4A7C15ull ^ (d + 0x632BE59BD9B4E019ull + (u >> 17));
+  x ^= x >> 30;
+  x *= 0xBF58476D1CE4E5B9ull;
+  x ^= x >> 27;
+  x *= 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return x;
+}
+
+// S(p, r) of the singles queue (chem_system.cc:237-259): h_pr plus the
+// n_e - 1 smallest (largest) of the 2n - 2 one-orbital terms (pr|oo) and
+// (pr|oo) - (po|or), summed smallest (largest) first; S = max(|min|, |max|).
+__global__ void k_sel_singles_S(Ints I, uint32_t n_elecs, double* __restrict__ S) {
+  const uint32_t n = (uint32_t)I.n;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * n) return;
+  const uint32_t p = t / n, r = t % n;
+  if (p == r) {
+    S[t] = 0.0;
+    return;
+  }
+  double el[126];
+  uint32_t m = 0;
+  for (uint32_t o = 0; o < n; o++) {
+    const double di = eri4(I, p, r, o, o);
+    el[m++] = di;
+    if (o != p && o != r) el[m++] = di - eri4(I, p, o, o, r);
+  }
+  for (uint32_t i = 1; i < m; i++) {
+    const double v = el[i];
+    int j = (int)i - 1;
+    while (j >= 0 && el[j] > v) {
+      el[j + 1] = el[j];
+      j--;
+    }
+    el[j + 1] = v;
+  }
+  double smin = I.h1[p * n + r], smax = smin;
+  const uint32_t k = min(n_elecs - 1u, 2 * n - 2);
+  for (uint32_t i = 0; i < k; i++) {
+    smin += el[i];
+    smax += el[2 * n - 3 - i];
+  }
+  S[t] = fmax(fabs(smin), fabs(smax));
+}
+
+// Singles list p: the r with S(p, r) > 0, S descending (one wavefront per p).
+__global__ __launch_bounds__(64) void k_sel_singles_sort(const double* __restrict__ S, uint32_t n,
+                                                          double* __restrict__ sqS, uint8_t* __restrict__ sqR,
+                                                          uint32_t* __restrict__ sqLen) {
+  __shared__ double sv[64];
+  const uint32_t p = blockIdx.x, r = threadIdx.x;
+  const double v = r < n ? S[p * n + r] : 0.0;
+  sv[r] = v;
+  __syncthreads();
+  const bool live = r < n && v != 0.0;
+  if (live) {
+    uint32_t rank = 0;
+    for (uint32_t q = 0; q < n; q++) {
+      const double w = sv[q];
+      rank += (w != 0.0 && (w > v || (w == v && q < r))) ? 1u : 0u;
+    }
+    sqS[p * n + rank] = v;
+    sqR[p * n + rank] = (uint8_t)r;
+  }
+  const uint64_t b = __ballot(live);
+  if (r == 0) sqLen[p] = (uint32_t)__popcll(b);
+}
+
+// Double list L (one workgroup per list): every (r, s) of the list's pair,
+// |H| as the bits of a non-negative double, bitonic-sorted descending in LDS.
+__global__ __launch_bounds__(256) void k_sel_hci_lists(Ints I, double* __restrict__ hqH,
+                                                        uint16_t* __restrict__ hqRS, uint32_t* __restrict__ hqLen) {
+  __shared__ uint64_t key[4096];
+  __shared__ uint16_t pay[4096];
+  const uint32_t n = (uint32_t)I.n, nn = n * n, ns = n * (n - 1) / 2, L = blockIdx.x, tid = threadIdx.x;
+  const bool same = L < ns;
+  uint32_t a, b;
+  if (same) {
+    b = 1;
+    while ((b + 1) * b / 2 <= L) b++;
+    a = L - b * (b - 1) / 2;
+  } else {
+    const uint32_t l = L - ns;
+    b = 0;
+    while ((b + 1) * (b + 2) / 2 <= l) b++;
+    a = l - b * (b + 1) / 2;
+  }
+  uint32_t P = 64;
+  while (P < nn) P <<= 1;
+  for (uint32_t e = tid; e < P; e += 256) {
+    double H = 0.0;
+    uint16_t rs = 0;
+    if (e < nn) {
+      const uint32_t r = e / n, s = e % n;
+      rs = (uint16_t)(r | (s << 8));
+      if (same) {
+        if (r < s && a != r && b != s && a != s && b != r) H = fabs(eri4(I, a, r, b, s) - eri4(I, a, s, b, r));
+      } else if (a != r && b != s) {
+        H = fabs(eri4(I, a, r, b, s));
+      }
+    }
+    key[e] = (uint64_t)__double_as_longlong(H);
+    pay[e] = rs;
+  }
+  __syncthreads();
+  for (uint32_t k = 2; k <= P; k <<= 1) {
+    for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+      for (uint32_t i = tid; i < P; i += 256) {
+        const uint32_t ixj = i ^ jj;
+        if (ixj > i) {
+          const uint64_t ka = key[i], kb = key[ixj];
+          if ((ka < kb) == ((i & k) == 0)) {
+            key[i] = kb;
+            key[ixj] = ka;
+            const uint16_t pa = pay[i];
+            pay[i] = pay[ixj];
+            pay[ixj] = pa;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const uint64_t base = (uint64_t)L * nn;
+  for (uint32_t e = tid; e < nn; e += 256) {
+    hqH[base + e] = __longlong_as_double((long long)key[e]);
+    hqRS[base + e] = pay[e];
+    if (key[e] != 0 && (e + 1 == nn || key[e + 1] == 0)) hqLen[L] = e + 1;
+  }
+  if (tid == 0 && key[0] == 0) hqLen[L] = 0;
+}
+
+__global__ void k_sel_hash_insert(const uint64_t* __restrict__ up, const uint64_t* __restrict__ dn, uint64_t n,
+                                  uint32_t* __restrict__ tab, uint64_t mask) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t h = det_hash(up[i], dn[i]) & mask;
+  while (atomicCAS(&tab[h], 0u, (uint32_t)(i + 1)) != 0u) h = (h + 1) & mask;
+}
+
+struct SelParams {
+  const uint64_t* __restrict__ up;
+  const uint64_t* __restrict__ dn;
+  uint64_t n;
+  const double* __restrict__ coefs;  // [state * n + det]
+  uint32_t n_states;
+  double eps_var;
+  double* __restrict__ eps_tried;
+  const uint32_t* __restrict__ htab;  // open-addressing set of the dets: det index + 1, 0 = empty
+  uint64_t hmask;
+  const double* __restrict__ sqS;
+  const uint8_t* __restrict__ sqR;
+  const uint32_t* __restrict__ sqLen;
+  const double* __restrict__ hqH;
+  const uint16_t* __restrict__ hqRS;
+  const uint32_t* __restrict__ hqLen;
+  Ints I;
+  uint64_t* __restrict__ cnt;        // count pass: candidates per det
+  const uint64_t* __restrict__ off;  // fill pass: exclusive offsets
+  uint64_t base;                     // fill pass: offset of the chunk's first candidate
+  DetKey* __restrict__ out;
+  uint32_t* __restrict__ bad;        // fill pass: set when a det's count differs from the count pass
+  uint64_t i0, i1;                   // det range
+};
+
+__device__ __forceinline__ bool sel_in_var(const SelParams& S, uint64_t u, uint64_t d) {
+  uint64_t h = det_hash(u, d) & S.hmask;
+  while (true) {
+    const uint32_t v = S.htab[h];
+    if (v == 0) return false;
+    if (S.up[v - 1] == u && S.dn[v - 1] == d) return true;
+    h = (h + 1) & S.hmask;
+  }
+}
+
+__device__ __forceinline__ uint32_t rdl(uint32_t v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+// One wavefront (a workgroup of its own) per det.  Lanes find the non-empty
+// walks (one per occupied orbital and per occupied pair); the wavefront then
+// runs each of them 64 list entries at a time.  FILL = false counts the kept
+// candidates of each det; FILL = true writes them at the det's offset and
+// records eps_tried_prev = 1.000000001 eps_min (solver.h:466).
+template <bool FILL>
+__global__ __launch_bounds__(64) void k_sel_walk(SelParams S) {
+  __shared__ uint8_t occ[128];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t n = (uint32_t)S.I.n, nn = n * n, ns = n * (n - 1) / 2;
+  for (uint64_t i = S.i0 + blockIdx.x; i < S.i1; i += gridDim.x) {
+    const uint64_t u = S.up[i], d = S.dn[i];
+    double mc = S.coefs[i];
+    for (uint32_t s = 1; s < S.n_states; s++) {
+      const double c = S.coefs[(uint64_t)s * S.n + i];
+      if (fabs(c) > fabs(mc)) mc = c;
+    }
+    const double emin = S.eps_var / fabs(mc);
+    const double emax = S.eps_tried[i];
+    uint64_t cur = FILL ? S.off[i] - S.base : 0;
+    const uint64_t lim = FILL ? S.off[i + 1] - S.base : 0;  // the det's range from the count pass
+    if (emin >= emax) {
+      if (!FILL && lane == 0) S.cnt[i] = 0;
+      continue;
+    }
+    const uint32_t nu = (uint32_t)__popcll(u), ne = nu + (uint32_t)__popcll(d);
+    __syncthreads();  // the previous det's occ reads are done
+    const uint64_t below = (1ull << lane) - 1ull;
+    if ((u >> lane) & 1ull) occ[__popcll(u & below)] = (uint8_t)lane;
+    if ((d >> lane) & 1ull) occ[nu + __popcll(d & below)] = (uint8_t)(lane + n);
+    __syncthreads();
+    const uint32_t nw = ne + ne * (ne - 1) / 2;
+    for (uint32_t w0 = 0; w0 < nw; w0 += kWave) {
+      const uint32_t w = w0 + lane;
+      uint32_t kind = 0, p = 0, q = 0, ks = 0, len = 0, L = 0;
+      if (w < ne) {
+        p = occ[w];
+        L = p < n ? p : p - n;
+        len = S.sqLen[L];
+        if (len > 0 && S.sqS[(uint64_t)L * n] >= emin) kind = 1;
+      } else if (w < nw) {
+        const uint32_t k = w - ne;
+        uint32_t b = (uint32_t)((1.0f + sqrtf(1.0f + 8.0f * (float)k)) * 0.5f);
+        while (b > 1 && b * (b - 1) / 2 > k) b--;
+        while ((b + 1) * b / 2 <= k) b++;
+        p = occ[k - b * (b - 1) / 2];
+        q = occ[b];
+        if (q < n) {
+          L = q * (q - 1) / 2 + p;
+        } else if (p >= n) {
+          L = (q - n) * (q - n - 1) / 2 + (p - n);
+        } else if (p <= q - n) {
+          L = ns + (q - n) * (q - n + 1) / 2 + p;
+        } else {
+          L = ns + p * (p + 1) / 2 + (q - n);
+        }
+        len = S.hqLen[L];
+        const double* H = S.hqH + (uint64_t)L * nn;
+        if (len > 0 && H[0] >= emin) {
+          // entries at or above eps_tried_prev were walked before: skip them
+          uint32_t hi = 0;
+          if (H[0] >= emax) {
+            uint32_t lo = 0;
+            hi = len;
+            while (hi - lo > 1) {
+              const uint32_t mid = (lo + hi) / 2;
+              if (H[mid] >= emax) lo = mid; else hi = mid;
+            }
+          }
+          ks = hi;
+          if (ks < len && H[ks] >= emin) kind = 2;
+        }
+      }
+      uint64_t todo = __ballot(kind != 0);
+      while (todo) {
+        const int bl = ctz64(todo);
+        todo &= todo - 1;
+        const uint32_t bk = rdl(kind, bl), bp = rdl(p, bl), bq = rdl(q, bl), bks = rdl(ks, bl), blen = rdl(len, bl);
+        const uint64_t lb = (uint64_t)rdl(L, bl) * (bk == 1 ? n : nn);
+        for (uint32_t k0 = bks; k0 < blen; k0 += kWave) {
+          const uint32_t k = k0 + lane;
+          uint64_t cu = u, cd = d;
+          const double hv = k < blen ? (bk == 1 ? S.sqS[lb + k] : S.hqH[lb + k]) : -1.0;
+          bool ok = hv >= emin;
+          const bool more = __ballot(ok) == ~0ull;  // every lane in range: the list may go on
+          if (ok && bk == 1) {
+            const uint32_t r = S.sqR[lb + k];
+            if (bp < n) {
+              ok = !((u >> r) & 1ull);
+              cu = (u & ~(1ull << bp)) | (1ull << r);
+            } else {
+              ok = !((d >> r) & 1ull);
+              cd = (d & ~(1ull << (bp - n))) | (1ull << r);
+            }
+          } else if (ok) {
+            const uint32_t rs = S.hqRS[lb + k];
+            uint32_t r = rs & 0xFFu, s = rs >> 8;
+            if (bq < n) {
+            } else if (bp >= n) {
+              r += n;
+              s += n;
+            } else if (bp <= bq - n) {
+              s += n;
+            } else {
+              const uint32_t t = r;
+              r = s;
+              s = t + n;
+            }
+            const bool occ_r = r < n ? ((u >> r) & 1ull) : ((d >> (r - n)) & 1ull);
+            const bool occ_s = s < n ? ((u >> s) & 1ull) : ((d >> (s - n)) & 1ull);
+            ok = !occ_r && !occ_s;
+            if (ok) {
+              if (bp < n) cu &= ~(1ull << bp); else cd &= ~(1ull << (bp - n));
+              if (bq < n) cu &= ~(1ull << bq); else cd &= ~(1ull << (bq - n));
+              if (r < n) cu |= 1ull << r; else cd |= 1ull << (r - n);
+              if (s < n) cu |= 1ull << s; else cd |= 1ull << (s - n);
+            }
+          }
+          if (ok) ok = !sel_in_var(S, cu, cd);
+          if (ok && bk == 1) {
+            const double h = bp < n ? single_elem(S.I, u, cu, d) : single_elem(S.I, d, cd, u);
+            ok = fabs(h) >= emin;
+          }
+          const uint64_t em = __ballot(ok);
+          if (FILL && ok && cur + mbcnt(em) < lim) S.out[cur + mbcnt(em)] = pack_det(cu, cd, n);
+          cur += (uint64_t)__popcll(em);
+          if (!more) break;
+        }
+      }
+    }
+    if (lane == 0) {
+      if (FILL) {
+        S.eps_tried[i] = 1.000000001 * emin;
+        if (cur != lim) atomicOr(S.bad, 1u);
+      } else {
+        S.cnt[i] = cur;
+      }
+    }
+  }
+}
+
+__global__ void k_sel_first_flags(const DetKey* __restrict__ k, uint64_t m, uint64_t* __restrict__ flag) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  flag[i] = (i == 0 || k[i].hi != k[i - 1].hi || k[i].lo != k[i - 1].lo) ? 1ull : 0ull;
+}
+
+__global__ void k_sel_compact(const DetKey* __restrict__ k, const uint64_t* __restrict__ flag,
+                              const uint64_t* __restrict__ pos, uint64_t m, DetKey* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m && flag[i]) out[pos[i]] = k[i];
+}
+
+__global__ void k_sel_unpack(const DetKey* __restrict__ k, uint64_t m, uint32_t n, uint64_t* __restrict__ up,
+                             uint64_t* __restrict__ dn) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) unpack_det(k[i], n, up[i], dn[i]);
+}
+
+// Builds the queues for the context's integrals and n_elecs electrons.
+int sel_queues(shci_ctx* ctx, uint32_t n_elecs) {
+  if (ctx->sel_ne == n_elecs) return SHCI_OK;
+  const uint32_t n = ctx->n_orb, nn = n * n;
+  const Ints I = make_ints(ctx);
+  const hipStream_t st = ctx->stream;
+  DevBuf Sd;
+  CHECK_RC(ensure(ctx, Sd, (size_t)nn * 8));
+  CHECK_RC(ensure(ctx, ctx->sq_S, (size_t)nn * 8));
+  CHECK_RC(ensure(ctx, ctx->sq_r, (size_t)nn));
+  CHECK_RC(ensure(ctx, ctx->sq_len, (size_t)n * 4));
+  CHECK_RC(ensure(ctx, ctx->hq_H, (size_t)nn * nn * 8));
+  CHECK_RC(ensure(ctx, ctx->hq_rs, (size_t)nn * nn * 2));
+  CHECK_RC(ensure(ctx, ctx->hq_len, (size_t)nn * 4));
+  k_sel_singles_S<<<div_up(nn, 64), 64, 0, st>>>(I, n_elecs, P<double>(Sd));
+  k_sel_singles_sort<<<n, 64, 0, st>>>(P<double>(Sd), n, P<double>(ctx->sq_S), P<uint8_t>(ctx->sq_r),
+                                       P<uint32_t>(ctx->sq_len));
+  k_sel_hci_lists<<<nn, 256, 0, st>>>(I, P<double>(ctx->hq_H), P<uint16_t>(ctx->hq_rs), P<uint32_t>(ctx->hq_len));
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(st));
+  release(ctx, Sd);
+  ctx->sel_ne = n_elecs;
+  return SHCI_OK;
+}
+
+int sort_det_keys(shci_ctx* ctx, const DetKey* kin, DetKey* kout, uint64_t m, int end_bit) {
+  size_t tb = 0;
+  HIPC(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, kin, kout, m, DetKeyDecomposer{}, 0, end_bit, ctx->stream));
+  CHECK_RC(ensure(ctx, ctx->tmp, tb));
+  HIPC(hipcub::DeviceRadixSort::SortKeys(ctx->tmp.p, tb, kin, kout, m, DetKeyDecomposer{}, 0, end_bit,
+                                         ctx->stream));
+  return SHCI_OK;
+}
+
+// Sorts keys[0..m) (into srt) and writes the distinct ones to out; returns
+// their number in *mu.
+int unique_det_keys(shci_ctx* ctx, const DetKey* keys, DetKey* srt, uint64_t m, DetKey* out, uint64_t* mu) {
+  *mu = 0;
+  if (m == 0) return SHCI_OK;
+  CHECK_RC(sort_det_keys(ctx, keys, srt, m, 2 * (int)ctx->n_orb));
+  DevBuf fl, pos;
+  int rc = SHCI_OK;
+  if ((rc = ensure(ctx, fl, (m + 1) * 8)) || (rc = ensure(ctx, pos, (m + 1) * 8))) {
+    release(ctx, fl);
+    release(ctx, pos);
+    return rc;
+  }
+  const hipStream_t st = ctx->stream;
+  k_sel_first_flags<<<div_up(m, 256), 256, 0, st>>>(srt, m, P<uint64_t>(fl));
+  if (hipMemsetAsync(P<uint64_t>(fl) + m, 0, 8, st) != hipSuccess) rc = fail(ctx, SHCI_ERR_HIP, "memset");
+  if (rc == SHCI_OK) rc = excl_scan(ctx, P<uint64_t>(fl), P<uint64_t>(pos), m + 1);
+  if (rc == SHCI_OK) {
+    k_sel_compact<<<div_up(m, 256), 256, 0, st>>>(srt, P<uint64_t>(fl), P<uint64_t>(pos), m, out);
+    if (hipMemcpyAsync(mu, P<uint64_t>(pos) + m, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      rc = fail(ctx, SHCI_ERR_HIP, "unique count");
+  }
+  release(ctx, fl);
+  release(ctx, pos);
+  return rc;
+}
+
+__global__ void k_sel_check(const uint64_t* __restrict__ up, const uint64_t* __restrict__ dn, uint64_t n,
+                            uint64_t omask, uint32_t ne, uint32_t* __restrict__ bad) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t u = up[i], d = dn[i];
+  if ((u & ~omask) || (d & ~omask) || (uint32_t)(__popcll(u) + __popcll(d)) != ne) atomicOr(bad, 1u);
+}
+
+int select_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n, const double* coefs,
+                 uint32_t n_states, double eps_var, double* eps_tried, uint64_t* n_new, DevBuf* bufs) {
+  DevBuf &dU = bufs[0], &dD = bufs[1], &dC = bufs[2], &dE = bufs[3], &tab = bufs[4], &cnt = bufs[5],
+         &off = bufs[6], &cand = bufs[7], &srt = bufs[8], &acc = bufs[9];
+  const hipStream_t st = ctx->stream;
+  const uint32_t n_orb = ctx->n_orb;
+  const uint64_t omask = n_orb < 64 ? (1ull << n_orb) - 1ull : ~0ull;
+  const uint32_t ne = (uint32_t)(__builtin_popcountll(up[0]) + __builtin_popcountll(dn[0]));
+  CHECK_RC(ensure(ctx, dU, n * 8));
+  CHECK_RC(ensure(ctx, dD, n * 8));
+  CHECK_RC(ensure(ctx, dC, (size_t)n_states * n * 8));
+  CHECK_RC(ensure(ctx, dE, n * 8));
+  CHECK_RC(ensure(ctx, ctx->scal, 64));
+  HIPC(hipMemcpyAsync(dU.p, up, n * 8, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(dD.p, dn, n * 8, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(dC.p, coefs, (size_t)n_states * n * 8, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(dE.p, eps_tried, n * 8, hipMemcpyHostToDevice, st));
+  HIPC(hipMemsetAsync(ctx->scal.p, 0, 8, st));
+  k_sel_check<<<div_up(n, 256), 256, 0, st>>>(P<uint64_t>(dU), P<uint64_t>(dD), n, omask, ne, P<uint32_t>(ctx->scal));
+  uint32_t bad = 0;
+  HIPC(hipMemcpyAsync(&bad, ctx->scal.p, 4, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  if (bad) return fail(ctx, SHCI_ERR_ARG, "selection: dets need orbitals < n_orb and equal electron counts");
+  CHECK_RC(sel_queues(ctx, ne));
+  // the variational space as an open-addressing set (load <= 1/2)
+  uint64_t H = 1024;
+  while (H < 2 * n) H <<= 1;
+  CHECK_RC(ensure(ctx, tab, H * 4));
+  HIPC(hipMemsetAsync(tab.p, 0, H * 4, st));
+  k_sel_hash_insert<<<div_up(n, 256), 256, 0, st>>>(P<uint64_t>(dU), P<uint64_t>(dD), n, P<uint32_t>(tab), H - 1);
+  SelParams S;
+  S.up = P<uint64_t>(dU);
+  S.dn = P<uint64_t>(dD);
+  S.n = n;
+  S.coefs = P<double>(dC);
+  S.n_states = n_states;
+  S.eps_var = eps_var;
+  S.eps_tried = P<double>(dE);
+  S.htab = P<uint32_t>(tab);
+  S.hmask = H - 1;
+  S.sqS = P<double>(ctx->sq_S);
+  S.sqR = P<uint8_t>(ctx->sq_r);
+  S.sqLen = P<uint32_t>(ctx->sq_len);
+  S.hqH = P<double>(ctx->hq_H);
+  S.hqRS = P<uint16_t>(ctx->hq_rs);
+  S.hqLen = P<uint32_t>(ctx->hq_len);
+  S.I = make_ints(ctx);
+  S.out = nullptr;
+  S.bad = P<uint32_t>(ctx->scal) + 1;
+  S.base = 0;
+  S.i0 = 0;
+  S.i1 = n;
+  // count pass, then offsets
+  CHECK_RC(ensure(ctx, cnt, (n + 1) * 8));
+  CHECK_RC(ensure(ctx, off, (n + 1) * 8));
+  S.cnt = P<uint64_t>(cnt);
+  S.off = P<uint64_t>(off);
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(n, 1u << 20);
+  k_sel_walk<false><<<grid, 64, 0, st>>>(S);
+  HIPC(hipMemsetAsync(P<uint64_t>(cnt) + n, 0, 8, st));
+  CHECK_RC(excl_scan(ctx, P<uint64_t>(cnt), P<uint64_t>(off), n + 1));
+  uint64_t total = 0;
+  HIPC(hipMemcpyAsync(&total, P<uint64_t>(off) + n, 8, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  // candidates per chunk: ~64 B each (keys, sorted keys, flags, offsets, sort
+  // storage) within 3/4 of the free memory
+  size_t free_b = 0, total_b = 0;
+  HIPC(hipMemGetInfo(&free_b, &total_b));
+  for (const DevBuf& c : ctx->cache) free_b += c.bytes;
+  const uint64_t mem_cap = std::max<uint64_t>(1ull << 20, (uint64_t)(0.75 * (double)free_b / 64.0));
+  uint64_t cap = mem_cap;
+  if (const char* e = std::getenv("SHCI_SELECT_CAP")) {  // smaller chunks (tests of the chunked path)
+    const unsigned long long v = std::strtoull(e, nullptr, 10);
+    if (v > 0) cap = std::min<uint64_t>(cap, v);
+  }
+  std::vector<uint64_t> hoff;
+  if (total > cap) {
+    hoff.resize(n + 1);
+    HIPC(hipMemcpy(hoff.data(), off.p, (n + 1) * 8, hipMemcpyDeviceToHost));
+  }
+  uint64_t n_acc = 0;
+  for (uint64_t a = 0; a < n;) {
+    uint64_t b = n, oa = 0, ob = total;
+    if (total > cap) {
+      oa = hoff[a];
+      b = a + 1;
+      while (b < n && hoff[b + 1] - oa <= cap / 2) b++;
+      ob = hoff[b];
+      if (ob - oa > mem_cap) return fail(ctx, SHCI_ERR_OOM, "selection: one det's candidates exceed the memory budget");
+    }
+    const uint64_t m = ob - oa;
+    CHECK_RC(ensure(ctx, cand, (n_acc + m + 1) * sizeof(DetKey)));
+    if (n_acc) HIPC(hipMemcpyAsync(cand.p, acc.p, n_acc * sizeof(DetKey), hipMemcpyDeviceToDevice, st));
+    S.out = P<DetKey>(cand) + n_acc;
+    S.base = oa;
+    S.i0 = a;
+    S.i1 = b;
+    k_sel_walk<true><<<(uint32_t)std::min<uint64_t>(b - a, 1u << 20), 64, 0, st>>>(S);
+    HIPC(hipGetLastError());
+    if (n_acc + m > 0) {
+      CHECK_RC(ensure(ctx, srt, (n_acc + m) * sizeof(DetKey)));
+      CHECK_RC(ensure(ctx, acc, (n_acc + m) * sizeof(DetKey)));
+      CHECK_RC(unique_det_keys(ctx, P<DetKey>(cand), P<DetKey>(srt), n_acc + m, P<DetKey>(acc), &n_acc));
+    }
+    a = b;
+  }
+  HIPC(hipMemcpyAsync(&bad, P<uint32_t>(ctx->scal) + 1, 4, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  if (bad) return fail(ctx, SHCI_ERR_STATE, "selection: fill pass differs from the count pass");
+  CHECK_RC(ensure(ctx, ctx->sel_up, std::max<uint64_t>(n_acc, 1) * 8));
+  CHECK_RC(ensure(ctx, ctx->sel_dn, std::max<uint64_t>(n_acc, 1) * 8));
+  if (n_acc)
+    k_sel_unpack<<<div_up(n_acc, 256), 256, 0, st>>>(P<DetKey>(acc), n_acc, n_orb, P<uint64_t>(ctx->sel_up),
+                                                     P<uint64_t>(ctx->sel_dn));
+  HIPC(hipMemcpyAsync(eps_tried, dE.p, n * 8, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  ctx->sel_n = n_acc;
+  *n_new = n_acc;
+  return SHCI_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -3836,7 +4422,8 @@ int shci_ctx_destroy(shci_ctx* ctx) {
                     &ctx->cnt,
                     &ctx->row_ptr, &ctx->row_end, &ctx->col, &ctx->v64, &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb,
                     &ctx->va, &ctx->vb, &ctx->scal, &ctx->part, &ctx->xv, &ctx->yv, &ctx->x32, &ctx->y32,
-                    &ctx->dav};
+                    &ctx->dav, &ctx->sq_S, &ctx->sq_r, &ctx->sq_len, &ctx->hq_H, &ctx->hq_rs, &ctx->hq_len,
+                    &ctx->sel_up, &ctx->sel_dn};
   for (DevBuf* b : bufs) release(ctx, *b);
   cache_trim(ctx);
   for (auto& e : ctx->ev)
@@ -3877,6 +4464,7 @@ int shci_set_integrals(shci_ctx* ctx, uint32_t n_orb, const double* h1, const do
   ctx->n_orb = n_orb;
   ctx->core = core;
   ctx->have_ints = true;
+  ctx->sel_ne = 0;  // selection queues follow the integrals
   return SHCI_OK;
 }
 
@@ -4318,6 +4906,33 @@ int shci_export_upper(shci_ctx* ctx, int64_t* row_ptr, int64_t* cols, double* va
   }
   cleanup();
   return rc;
+}
+
+int shci_select(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n_dets, const double* coefs,
+                uint32_t n_states, double eps_var, double* eps_tried_prev, uint64_t* n_new) {
+  if (!ctx) return SHCI_ERR_ARG;
+  if (!up || !dn || !coefs || !eps_tried_prev || !n_new || n_dets == 0 || n_states == 0)
+    return fail(ctx, SHCI_ERR_ARG, "selection: empty or null arguments");
+  if (!ctx->have_ints) return fail(ctx, SHCI_ERR_STATE, "selection: no integrals");
+  if (n_dets >= 0xFFFFFFFFull) return fail(ctx, SHCI_ERR_UNSUPPORTED, "selection: n_dets >= 2^32 - 1");
+  HIPC(hipSetDevice(ctx->device));
+  *n_new = 0;
+  ctx->sel_n = 0;
+  DevBuf bufs[10];
+  const int rc = select_sweep(ctx, up, dn, n_dets, coefs, n_states, eps_var, eps_tried_prev, n_new, bufs);
+  for (DevBuf& b : bufs) release(ctx, b);
+  return rc;
+}
+
+int shci_select_result(shci_ctx* ctx, uint64_t* new_up, uint64_t* new_dn, uint64_t n) {
+  if (!ctx) return SHCI_ERR_ARG;
+  if (n != ctx->sel_n) return fail(ctx, SHCI_ERR_ARG, "selection result: n differs from the last sweep's count");
+  if (n == 0) return SHCI_OK;
+  if (!new_up || !new_dn) return fail(ctx, SHCI_ERR_ARG, "selection result: null output");
+  HIPC(hipSetDevice(ctx->device));
+  HIPC(hipMemcpy(new_up, ctx->sel_up.p, n * 8, hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(new_dn, ctx->sel_dn.p, n * 8, hipMemcpyDeviceToHost));
+  return SHCI_OK;
 }
 
 int shci_clear(shci_ctx* ctx) {

@@ -230,6 +230,13 @@ class Hamiltonian:
         self.ctx.check(self.ctx.lib.shci_set_integrals(self.ctx.handle, self.n_orb, self._h1.ctypes.data_as(_dp),
                                                        self._eri.ctypes.data_as(_dp), self.core),
                        "shci_set_integrals")
+        self.ctx.integrals_owner = self
+
+    def ensure_integrals(self):
+        """Makes these integrals the context's current ones (contexts are
+        shared per device; the selection reads the context's integrals)."""
+        if getattr(self.ctx, "integrals_owner", None) is not self:
+            self.set_integrals(self.n_orb, self._h1, self._eri, self.core)
 
     def _flags(self):
         f = {"fp64": _abi.SHCI_STORE_FP64, "fp32": _abi.SHCI_STORE_FP32,
