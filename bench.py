@@ -57,6 +57,45 @@ def measured_traffic(layout, n_dets, kernel):
     return None
 
 
+def selection_bench(w, ham, coefs):
+    """One heat-bath selection sweep over the whole det list with its
+    converged coefficients at the schedule's last eps, every det walked
+    (eps_tried_prev = 1e100): on the device (shci_select; the first call also
+    builds the queues) and with the host's queue walk on the host cores, on
+    the same inputs; the two det lists must be identical."""
+    from shci_amd.host import HostSystem
+    from shci_amd.selection import Selector
+
+    spec = w.spec
+    eps = float(w.log[-1]["eps"])
+    n = len(w.up)
+    coefs = np.atleast_2d(coefs)
+    sel = Selector(ham)
+    e_first = np.full(n, 1e100)
+    sel.select(w.up, w.dn, coefs, eps, e_first)
+    first_ms = sel.last_ms
+    e_dev = np.full(n, 1e100)
+    nu, nd = sel.select(w.up, w.dn, coefs, eps, e_dev)
+    dev_ms, phases = sel.last_ms, sel.last_phases_ms
+    host = HostSystem.from_arrays(w.n, spec.n_up, spec.n_dn, w.h1, w.eri, w.core, n_states=spec.n_states)
+    host.set_dets(w.up, w.dn)
+    t0 = time.time()
+    m = host.select(coefs, eps)
+    host_ms = (time.time() - t0) * 1e3
+    hu, hd = host.dets()
+    same = m == len(nu) and np.array_equal(hu[n:], nu) and np.array_equal(hd[n:], nd)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"[bench] selection at eps {eps:.1e} over {n:,} dets: {len(nu):,} new; device {dev_ms:.1f} ms "
+        f"(first call {first_ms:.1f} ms with the queues; phases {phases}), host queue walk {host_ms:.1f} ms "
+        f"on {threads} threads; identical={same}")
+    if not same:
+        raise SystemExit("selection: device and host det lists differ")
+    return dict(eps=eps, n_dets=n, new_dets=int(len(nu)), device_ms=dev_ms, device_first_call_ms=first_ms,
+                device_phases_ms=phases, host_ms=host_ms, host_threads=threads, identical=bool(same),
+                note="device_ms is wall time of shci_select + shci_select_result, host arrays in and out "
+                     "(PCIe included); phases are HIP-event times on the library's stream")
+
+
 def cpu_baseline(w, sample_dets, reps_spmv=2):
     """The oracle (C++/OpenMP restatement of the reference) on the first
     ``sample_dets`` dets of the same list: build + Davidson to the same
@@ -147,6 +186,8 @@ def main():
     ap.add_argument("--target-dets", type=int, default=0)
     ap.add_argument("--cpu-sample-dets", type=int, default=500_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-selection", action="store_true",
+                    help="skip the selection sweep (device vs the host's queue walk)")
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--layout", choices=["segments", "csr"], default="segments")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -262,6 +303,7 @@ def main():
         f"Davidson {t_mixed:.3f}s, E {e_mixed:.10f} err {abs(e_mixed - e_ref) * 1e6:.3f} uHa; "
         f"SpMV32 {spmv32_ms:.3f} ms = {spmv32_bytes / spmv32_ms / 1e6:.0f} GB/s")
     ham.clear()
+    selection = None if args.no_selection else selection_bench(w, ham, d.get_lowest_eigenvectors())
 
     # ---------------- CPU baseline ----------------
     cpu = None
@@ -321,6 +363,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if selection is not None:
+        out["selection"] = selection
     if ns > 1:
         out["n_states"] = ns
         out["energies_ref_fp64"] = e_ref_all.tolist()

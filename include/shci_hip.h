@@ -5,6 +5,9 @@
  *   SparseMatrix::mul               src/solver/sparse_matrix.cc:88-116
  *   Davidson::diagonalize           src/solver/davidson.cc:37-225
  *   RDM::get_1rdm / get_2rdm        src/chem/rdm.cc:12-131, 374-395
+ * and the step before it, the heat-bath selection sweep
+ *   Solver<S>::run_variation loop   src/solver/solver.h:435-495
+ *   ChemSystem::find_connected_dets src/chem/chem_system.cc:289-402
  * Each entry point below names the reference interface it replaces.  All
  * arguments are plain pointers and sizes; host pointers unless a flag says
  * otherwise.  Every call returns SHCI_OK (0) or a negative status; the message
@@ -156,6 +159,11 @@ int shci_select(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t 
                 uint32_t n_states, double eps_var, double* eps_tried_prev, uint64_t* n_new);
 /* The last sweep's new dets: n must equal its *n_new. */
 int shci_select_result(shci_ctx* ctx, uint64_t* new_up, uint64_t* new_dn, uint64_t n);
+/* Phases of the last sweep (no reference counterpart; diagnostics), HIP
+ * events on the context's stream, ms[4]: upload, checks, queues and the
+ * variational-space hash set; the walk (one append pass); any repeated or
+ * chunked passes, sort and unique; unpack and download. */
+int shci_select_timings(shci_ctx* ctx, float* ms);
 
 /* Timing of the most recent build phases and a timed SpMV loop, measured with
  * HIP events on the context's stream (milliseconds).  phases[8]: sort/group,

@@ -123,10 +123,11 @@ struct shci_ctx {
   // current integrals (rebuilt when the integrals or the electron count
   // change) and the new determinants of the last sweep.
   uint32_t sel_ne = 0;                  // electron count of the queues (0 = not built)
-  DevBuf sq_S, sq_r, sq_len;            // n singles lists of n entries: S descending, target r
-  DevBuf hq_H, hq_rs, hq_len;           // n*n double lists of n*n entries: |H| descending, (r, s)
+  DevBuf sq_S, sq_r, sq_len, sq_max;    // n singles lists of n entries: S descending, target r
+  DevBuf hq_H, hq_rs, hq_len, hq_max;   // n*n double lists of n*n entries: |H| descending, (r, s)
   DevBuf sel_up, sel_dn;                // last sweep's new dets, (up, dn) ascending
   uint64_t sel_n = 0;
+  float sel_ms[4] = {};                 // last sweep: set-up, count walk, fill walk + unique, output
   // Released device blocks kept for reuse (builds allocate and drop the same
   // temporaries every time; hipMalloc/hipFree of GB blocks per build would
   // cost more than the kernels).  Freed on memory pressure, clear and destroy.
@@ -3911,7 +3912,7 @@ __global__ void k_sel_singles_S(Ints I, uint32_t n_elecs, double* __restrict__ S
 // Singles list p: the r with S(p, r) > 0, S descending (one wavefront per p).
 __global__ __launch_bounds__(64) void k_sel_singles_sort(const double* __restrict__ S, uint32_t n,
                                                           double* __restrict__ sqS, uint8_t* __restrict__ sqR,
-                                                          uint32_t* __restrict__ sqLen) {
+                                                          uint32_t* __restrict__ sqLen, double* __restrict__ sqMax) {
   __shared__ double sv[64];
   const uint32_t p = blockIdx.x, r = threadIdx.x;
   const double v = r < n ? S[p * n + r] : 0.0;
@@ -3926,15 +3927,20 @@ __global__ __launch_bounds__(64) void k_sel_singles_sort(const double* __restric
     }
     sqS[p * n + rank] = v;
     sqR[p * n + rank] = (uint8_t)r;
+    if (rank == 0) sqMax[p] = v;
   }
   const uint64_t b = __ballot(live);
-  if (r == 0) sqLen[p] = (uint32_t)__popcll(b);
+  if (r == 0) {
+    sqLen[p] = (uint32_t)__popcll(b);
+    if (b == 0) sqMax[p] = 0.0;
+  }
 }
 
 // Double list L (one workgroup per list): every (r, s) of the list's pair,
 // |H| as the bits of a non-negative double, bitonic-sorted descending in LDS.
 __global__ __launch_bounds__(256) void k_sel_hci_lists(Ints I, double* __restrict__ hqH,
-                                                        uint16_t* __restrict__ hqRS, uint32_t* __restrict__ hqLen) {
+                                                        uint16_t* __restrict__ hqRS, uint32_t* __restrict__ hqLen,
+                                                        double* __restrict__ hqMax) {
   __shared__ uint64_t key[4096];
   __shared__ uint16_t pay[4096];
   const uint32_t n = (uint32_t)I.n, nn = n * n, ns = n * (n - 1) / 2, L = blockIdx.x, tid = threadIdx.x;
@@ -3992,15 +3998,28 @@ __global__ __launch_bounds__(256) void k_sel_hci_lists(Ints I, double* __restric
     hqRS[base + e] = pay[e];
     if (key[e] != 0 && (e + 1 == nn || key[e + 1] == 0)) hqLen[L] = e + 1;
   }
-  if (tid == 0 && key[0] == 0) hqLen[L] = 0;
+  if (tid == 0) {
+    hqMax[L] = __longlong_as_double((long long)key[0]);
+    if (key[0] == 0) hqLen[L] = 0;
+  }
 }
 
+// Open-addressing set of the variational dets: slot (up, dn), up = ~0 empty
+// (no det has every one of 64 orbitals occupied in one spin: rejected by the
+// checks).  A slot is claimed by a CAS on its up word; the dets are distinct,
+// so a claimed slot never needs its dn compared during the build.
+constexpr uint64_t kSelEmpty = ~0ull;
+
 __global__ void k_sel_hash_insert(const uint64_t* __restrict__ up, const uint64_t* __restrict__ dn, uint64_t n,
-                                  uint32_t* __restrict__ tab, uint64_t mask) {
+                                  uint64_t* __restrict__ tab, uint64_t mask) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint64_t h = det_hash(up[i], dn[i]) & mask;
-  while (atomicCAS(&tab[h], 0u, (uint32_t)(i + 1)) != 0u) h = (h + 1) & mask;
+  const uint64_t u = up[i];
+  uint64_t h = det_hash(u, dn[i]) & mask;
+  while (atomicCAS((unsigned long long*)&tab[2 * h], (unsigned long long)kSelEmpty, (unsigned long long)u) !=
+         (unsigned long long)kSelEmpty)
+    h = (h + 1) & mask;
+  tab[2 * h + 1] = dn[i];
 }
 
 struct SelParams {
@@ -4011,29 +4030,33 @@ struct SelParams {
   uint32_t n_states;
   double eps_var;
   double* __restrict__ eps_tried;
-  const uint32_t* __restrict__ htab;  // open-addressing set of the dets: det index + 1, 0 = empty
+  const uint64_t* __restrict__ htab;  // the dets' open-addressing set: (up, dn) slots
   uint64_t hmask;
   const double* __restrict__ sqS;
   const uint8_t* __restrict__ sqR;
   const uint32_t* __restrict__ sqLen;
+  const double* __restrict__ sqMax;   // first (largest) S of each singles list, 0 if empty
   const double* __restrict__ hqH;
   const uint16_t* __restrict__ hqRS;
   const uint32_t* __restrict__ hqLen;
+  const double* __restrict__ hqMax;   // first (largest) |H| of each double list, 0 if empty
   Ints I;
   uint64_t* __restrict__ cnt;        // count pass: candidates per det
   const uint64_t* __restrict__ off;  // fill pass: exclusive offsets
   uint64_t base;                     // fill pass: offset of the chunk's first candidate
   DetKey* __restrict__ out;
   uint32_t* __restrict__ bad;        // fill pass: set when a det's count differs from the count pass
+  unsigned long long* __restrict__ total;  // append pass: candidates found (also past cap)
+  uint64_t cap;                      // append pass: capacity of out
   uint64_t i0, i1;                   // det range
 };
 
 __device__ __forceinline__ bool sel_in_var(const SelParams& S, uint64_t u, uint64_t d) {
   uint64_t h = det_hash(u, d) & S.hmask;
   while (true) {
-    const uint32_t v = S.htab[h];
-    if (v == 0) return false;
-    if (S.up[v - 1] == u && S.dn[v - 1] == d) return true;
+    const uint64_t tu = S.htab[2 * h], td = S.htab[2 * h + 1];
+    if (tu == kSelEmpty) return false;
+    if (tu == u && td == d) return true;
     h = (h + 1) & S.hmask;
   }
 }
@@ -4042,12 +4065,36 @@ __device__ __forceinline__ uint32_t rdl(uint32_t v, int l) { return __builtin_am
 
 // One wavefront (a workgroup of its own) per det.  Lanes find the non-empty
 // walks (one per occupied orbital and per occupied pair); the wavefront then
-// runs each of them 64 list entries at a time.  FILL = false counts the kept
-// candidates of each det; FILL = true writes them at the det's offset and
-// records eps_tried_prev = 1.000000001 eps_min (solver.h:466).
-template <bool FILL>
+// runs each of them 64 list entries at a time.  Modes:
+//   kSelCount:  candidates kept per det (no writes);
+//   kSelFill:   writes them at the det's offset from the count pass;
+//   kSelAppend: one pass, candidates staged in LDS and appended to out with
+//               one global atomic per flush (writes past cap dropped; the
+//               total counts them all).
+// Fill and append record eps_tried_prev = 1.000000001 eps_min (solver.h:466).
+constexpr int kSelCount = 0, kSelFill = 1, kSelAppend = 2;
+constexpr uint32_t kSelStage = 256;  // LDS stage of the append pass (flushed above 192)
+
+// Appends the wavefront's staged candidates: one atomic for the block of
+// slots, then coalesced copies (slots at or past cap are dropped).
+__device__ __forceinline__ void sel_flush(const SelParams& S, const DetKey* stage, uint32_t& scnt) {
+  if (!scnt) return;
+  unsigned long long b0 = 0;
+  if (threadIdx.x == 0) b0 = atomicAdd(S.total, (unsigned long long)scnt);
+  const uint64_t b = (uint64_t)__shfl((long long)b0, 0, kWave);
+  __syncthreads();  // staged entries visible to every lane
+  for (uint32_t k = threadIdx.x; k < scnt; k += kWave)
+    if (b + k < S.cap) S.out[b + k] = stage[k];
+  __syncthreads();  // stage reads done before it is refilled
+  scnt = 0;
+}
+
+template <int MODE>
 __global__ __launch_bounds__(64) void k_sel_walk(SelParams S) {
+  constexpr bool FILL = MODE != kSelCount;
   __shared__ uint8_t occ[128];
+  __shared__ DetKey stage[MODE == kSelAppend ? kSelStage : 1];
+  uint32_t scnt = 0;  // staged entries (append)
   const uint32_t lane = threadIdx.x;
   const uint32_t n = (uint32_t)S.I.n, nn = n * n, ns = n * (n - 1) / 2;
   for (uint64_t i = S.i0 + blockIdx.x; i < S.i1; i += gridDim.x) {
@@ -4059,8 +4106,8 @@ __global__ __launch_bounds__(64) void k_sel_walk(SelParams S) {
     }
     const double emin = S.eps_var / fabs(mc);
     const double emax = S.eps_tried[i];
-    uint64_t cur = FILL ? S.off[i] - S.base : 0;
-    const uint64_t lim = FILL ? S.off[i + 1] - S.base : 0;  // the det's range from the count pass
+    uint64_t cur = MODE == kSelFill ? S.off[i] - S.base : 0;
+    const uint64_t lim = MODE == kSelFill ? S.off[i + 1] - S.base : 0;  // the det's range from the count pass
     if (emin >= emax) {
       if (!FILL && lane == 0) S.cnt[i] = 0;
       continue;
@@ -4078,8 +4125,10 @@ __global__ __launch_bounds__(64) void k_sel_walk(SelParams S) {
       if (w < ne) {
         p = occ[w];
         L = p < n ? p : p - n;
-        len = S.sqLen[L];
-        if (len > 0 && S.sqS[(uint64_t)L * n] >= emin) kind = 1;
+        if (S.sqMax[L] >= emin) {
+          kind = 1;
+          len = S.sqLen[L];
+        }
       } else if (w < nw) {
         const uint32_t k = w - ne;
         uint32_t b = (uint32_t)((1.0f + sqrtf(1.0f + 8.0f * (float)k)) * 0.5f);
@@ -4096,9 +4145,9 @@ __global__ __launch_bounds__(64) void k_sel_walk(SelParams S) {
         } else {
           L = ns + p * (p + 1) / 2 + (q - n);
         }
-        len = S.hqLen[L];
         const double* H = S.hqH + (uint64_t)L * nn;
-        if (len > 0 && H[0] >= emin) {
+        if (S.hqMax[L] >= emin) {
+          len = S.hqLen[L];
           // entries at or above eps_tried_prev were walked before: skip them
           uint32_t hi = 0;
           if (H[0] >= emax) {
@@ -4164,21 +4213,24 @@ __global__ __launch_bounds__(64) void k_sel_walk(SelParams S) {
             ok = fabs(h) >= emin;
           }
           const uint64_t em = __ballot(ok);
-          if (FILL && ok && cur + mbcnt(em) < lim) S.out[cur + mbcnt(em)] = pack_det(cu, cd, n);
+          if (MODE == kSelFill && ok && cur + mbcnt(em) < lim) S.out[cur + mbcnt(em)] = pack_det(cu, cd, n);
+          if (MODE == kSelAppend && em) {
+            if (ok) stage[scnt + mbcnt(em)] = pack_det(cu, cd, n);
+            scnt += (uint32_t)__popcll(em);
+            if (scnt > kSelStage - kWave) sel_flush(S, stage, scnt);
+          }
           cur += (uint64_t)__popcll(em);
           if (!more) break;
         }
       }
     }
     if (lane == 0) {
-      if (FILL) {
-        S.eps_tried[i] = 1.000000001 * emin;
-        if (cur != lim) atomicOr(S.bad, 1u);
-      } else {
-        S.cnt[i] = cur;
-      }
+      if (FILL) S.eps_tried[i] = 1.000000001 * emin;
+      if (MODE == kSelFill && cur != lim) atomicOr(S.bad, 1u);
+      if (MODE == kSelCount) S.cnt[i] = cur;
     }
   }
+  if (MODE == kSelAppend) sel_flush(S, stage, scnt);
 }
 
 __global__ void k_sel_first_flags(const DetKey* __restrict__ k, uint64_t m, uint64_t* __restrict__ flag) {
@@ -4213,10 +4265,13 @@ int sel_queues(shci_ctx* ctx, uint32_t n_elecs) {
   CHECK_RC(ensure(ctx, ctx->hq_H, (size_t)nn * nn * 8));
   CHECK_RC(ensure(ctx, ctx->hq_rs, (size_t)nn * nn * 2));
   CHECK_RC(ensure(ctx, ctx->hq_len, (size_t)nn * 4));
+  CHECK_RC(ensure(ctx, ctx->sq_max, (size_t)n * 8));
+  CHECK_RC(ensure(ctx, ctx->hq_max, (size_t)nn * 8));
   k_sel_singles_S<<<div_up(nn, 64), 64, 0, st>>>(I, n_elecs, P<double>(Sd));
   k_sel_singles_sort<<<n, 64, 0, st>>>(P<double>(Sd), n, P<double>(ctx->sq_S), P<uint8_t>(ctx->sq_r),
-                                       P<uint32_t>(ctx->sq_len));
-  k_sel_hci_lists<<<nn, 256, 0, st>>>(I, P<double>(ctx->hq_H), P<uint16_t>(ctx->hq_rs), P<uint32_t>(ctx->hq_len));
+                                       P<uint32_t>(ctx->sq_len), P<double>(ctx->sq_max));
+  k_sel_hci_lists<<<nn, 256, 0, st>>>(I, P<double>(ctx->hq_H), P<uint16_t>(ctx->hq_rs), P<uint32_t>(ctx->hq_len),
+                                      P<double>(ctx->hq_max));
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(st));
   release(ctx, Sd);
@@ -4266,7 +4321,8 @@ __global__ void k_sel_check(const uint64_t* __restrict__ up, const uint64_t* __r
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint64_t u = up[i], d = dn[i];
-  if ((u & ~omask) || (d & ~omask) || (uint32_t)(__popcll(u) + __popcll(d)) != ne) atomicOr(bad, 1u);
+  if ((u & ~omask) || (d & ~omask) || u == kSelEmpty || (uint32_t)(__popcll(u) + __popcll(d)) != ne)
+    atomicOr(bad, 1u);
 }
 
 int select_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n, const double* coefs,
@@ -4282,6 +4338,7 @@ int select_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t
   CHECK_RC(ensure(ctx, dC, (size_t)n_states * n * 8));
   CHECK_RC(ensure(ctx, dE, n * 8));
   CHECK_RC(ensure(ctx, ctx->scal, 64));
+  HIPC(hipEventRecord(ctx->ev[0], st));
   HIPC(hipMemcpyAsync(dU.p, up, n * 8, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(dD.p, dn, n * 8, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(dC.p, coefs, (size_t)n_states * n * 8, hipMemcpyHostToDevice, st));
@@ -4293,12 +4350,12 @@ int select_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t
   HIPC(hipStreamSynchronize(st));
   if (bad) return fail(ctx, SHCI_ERR_ARG, "selection: dets need orbitals < n_orb and equal electron counts");
   CHECK_RC(sel_queues(ctx, ne));
-  // the variational space as an open-addressing set (load <= 1/2)
+  // the variational space as an open-addressing set of (up, dn) slots (load <= 1/2)
   uint64_t H = 1024;
   while (H < 2 * n) H <<= 1;
-  CHECK_RC(ensure(ctx, tab, H * 4));
-  HIPC(hipMemsetAsync(tab.p, 0, H * 4, st));
-  k_sel_hash_insert<<<div_up(n, 256), 256, 0, st>>>(P<uint64_t>(dU), P<uint64_t>(dD), n, P<uint32_t>(tab), H - 1);
+  CHECK_RC(ensure(ctx, tab, H * 16));
+  HIPC(hipMemsetAsync(tab.p, 0xFF, H * 16, st));
+  k_sel_hash_insert<<<div_up(n, 256), 256, 0, st>>>(P<uint64_t>(dU), P<uint64_t>(dD), n, P<uint64_t>(tab), H - 1);
   SelParams S;
   S.up = P<uint64_t>(dU);
   S.dn = P<uint64_t>(dD);
@@ -4307,84 +4364,125 @@ int select_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t
   S.n_states = n_states;
   S.eps_var = eps_var;
   S.eps_tried = P<double>(dE);
-  S.htab = P<uint32_t>(tab);
+  S.htab = P<uint64_t>(tab);
   S.hmask = H - 1;
   S.sqS = P<double>(ctx->sq_S);
   S.sqR = P<uint8_t>(ctx->sq_r);
   S.sqLen = P<uint32_t>(ctx->sq_len);
+  S.sqMax = P<double>(ctx->sq_max);
   S.hqH = P<double>(ctx->hq_H);
   S.hqRS = P<uint16_t>(ctx->hq_rs);
   S.hqLen = P<uint32_t>(ctx->hq_len);
+  S.hqMax = P<double>(ctx->hq_max);
   S.I = make_ints(ctx);
   S.out = nullptr;
   S.bad = P<uint32_t>(ctx->scal) + 1;
+  S.total = reinterpret_cast<unsigned long long*>(P<uint64_t>(ctx->scal) + 1);
+  S.cnt = nullptr;
+  S.off = nullptr;
   S.base = 0;
   S.i0 = 0;
   S.i1 = n;
-  // count pass, then offsets
-  CHECK_RC(ensure(ctx, cnt, (n + 1) * 8));
-  CHECK_RC(ensure(ctx, off, (n + 1) * 8));
-  S.cnt = P<uint64_t>(cnt);
-  S.off = P<uint64_t>(off);
-  const uint32_t grid = (uint32_t)std::min<uint64_t>(n, 1u << 20);
-  k_sel_walk<false><<<grid, 64, 0, st>>>(S);
-  HIPC(hipMemsetAsync(P<uint64_t>(cnt) + n, 0, 8, st));
-  CHECK_RC(excl_scan(ctx, P<uint64_t>(cnt), P<uint64_t>(off), n + 1));
-  uint64_t total = 0;
-  HIPC(hipMemcpyAsync(&total, P<uint64_t>(off) + n, 8, hipMemcpyDeviceToHost, st));
-  HIPC(hipStreamSynchronize(st));
-  // candidates per chunk: ~64 B each (keys, sorted keys, flags, offsets, sort
-  // storage) within 3/4 of the free memory
+  // Candidate budget: ~64 B each (keys, sorted keys, flags, offsets, sort
+  // storage) within 3/4 of the free memory (plus the context's cached blocks).
   size_t free_b = 0, total_b = 0;
   HIPC(hipMemGetInfo(&free_b, &total_b));
   for (const DevBuf& c : ctx->cache) free_b += c.bytes;
   const uint64_t mem_cap = std::max<uint64_t>(1ull << 20, (uint64_t)(0.75 * (double)free_b / 64.0));
   uint64_t cap = mem_cap;
-  if (const char* e = std::getenv("SHCI_SELECT_CAP")) {  // smaller chunks (tests of the chunked path)
+  if (const char* e = std::getenv("SHCI_SELECT_CAP")) {  // smaller buffers (tests of the fallback paths)
     const unsigned long long v = std::strtoull(e, nullptr, 10);
     if (v > 0) cap = std::min<uint64_t>(cap, v);
   }
-  std::vector<uint64_t> hoff;
-  if (total > cap) {
-    hoff.resize(n + 1);
-    HIPC(hipMemcpy(hoff.data(), off.p, (n + 1) * 8, hipMemcpyDeviceToHost));
-  }
-  uint64_t n_acc = 0;
-  for (uint64_t a = 0; a < n;) {
-    uint64_t b = n, oa = 0, ob = total;
-    if (total > cap) {
-      oa = hoff[a];
-      b = a + 1;
-      while (b < n && hoff[b + 1] - oa <= cap / 2) b++;
-      ob = hoff[b];
-      if (ob - oa > mem_cap) return fail(ctx, SHCI_ERR_OOM, "selection: one det's candidates exceed the memory budget");
-    }
-    const uint64_t m = ob - oa;
-    CHECK_RC(ensure(ctx, cand, (n_acc + m + 1) * sizeof(DetKey)));
-    if (n_acc) HIPC(hipMemcpyAsync(cand.p, acc.p, n_acc * sizeof(DetKey), hipMemcpyDeviceToDevice, st));
-    S.out = P<DetKey>(cand) + n_acc;
-    S.base = oa;
-    S.i0 = a;
-    S.i1 = b;
-    k_sel_walk<true><<<(uint32_t)std::min<uint64_t>(b - a, 1u << 20), 64, 0, st>>>(S);
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(n, 1u << 20);
+  uint64_t total = 0, n_acc = 0;
+  auto run_append = [&](uint64_t cap_a) -> int {
+    CHECK_RC(ensure(ctx, cand, (cap_a + 1) * sizeof(DetKey)));
+    S.out = P<DetKey>(cand);
+    S.cap = cap_a;
+    HIPC(hipMemsetAsync(P<uint64_t>(ctx->scal) + 1, 0, 8, st));
+    k_sel_walk<kSelAppend><<<grid, 64, 0, st>>>(S);
     HIPC(hipGetLastError());
-    if (n_acc + m > 0) {
-      CHECK_RC(ensure(ctx, srt, (n_acc + m) * sizeof(DetKey)));
-      CHECK_RC(ensure(ctx, acc, (n_acc + m) * sizeof(DetKey)));
-      CHECK_RC(unique_det_keys(ctx, P<DetKey>(cand), P<DetKey>(srt), n_acc + m, P<DetKey>(acc), &n_acc));
-    }
-    a = b;
+    HIPC(hipMemcpyAsync(&total, P<uint64_t>(ctx->scal) + 1, 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    return SHCI_OK;
+  };
+  // One pass into a buffer sized for the common case; if the candidates do
+  // not fit, the sweep starts again (eps_tried_prev re-uploaded) with an
+  // exact buffer, or, beyond the budget, in chunks of dets from a count pass.
+  uint64_t cap0 = std::min<uint64_t>(cap, std::max<uint64_t>(1ull << 22, 8 * n));
+  if (const char* e = std::getenv("SHCI_SELECT_FIRST")) {  // a smaller first buffer (tests of the repeat)
+    const unsigned long long v = std::strtoull(e, nullptr, 10);
+    if (v > 0) cap0 = std::min<uint64_t>(cap0, v);
   }
-  HIPC(hipMemcpyAsync(&bad, P<uint32_t>(ctx->scal) + 1, 4, hipMemcpyDeviceToHost, st));
-  HIPC(hipStreamSynchronize(st));
-  if (bad) return fail(ctx, SHCI_ERR_STATE, "selection: fill pass differs from the count pass");
+  HIPC(hipEventRecord(ctx->ev[1], st));
+  CHECK_RC(run_append(cap0));
+  HIPC(hipEventRecord(ctx->ev[2], st));
+  bool done = total <= cap0;
+  if (!done) {
+    HIPC(hipMemcpyAsync(dE.p, eps_tried, n * 8, hipMemcpyHostToDevice, st));
+    if (total <= cap) {
+      const uint64_t need = total;
+      CHECK_RC(run_append(need));
+      if (total != need) return fail(ctx, SHCI_ERR_STATE, "selection: the repeated pass found a different count");
+      done = true;
+    }
+  }
+  if (done) {
+    if (total > 0) {
+      CHECK_RC(ensure(ctx, srt, total * sizeof(DetKey)));
+      CHECK_RC(ensure(ctx, acc, total * sizeof(DetKey)));
+      CHECK_RC(unique_det_keys(ctx, P<DetKey>(cand), P<DetKey>(srt), total, P<DetKey>(acc), &n_acc));
+    }
+  } else {
+    // count pass, offsets, then fill passes over chunks of dets
+    CHECK_RC(ensure(ctx, cnt, (n + 1) * 8));
+    CHECK_RC(ensure(ctx, off, (n + 1) * 8));
+    S.cnt = P<uint64_t>(cnt);
+    S.off = P<uint64_t>(off);
+    k_sel_walk<kSelCount><<<grid, 64, 0, st>>>(S);
+    HIPC(hipMemsetAsync(P<uint64_t>(cnt) + n, 0, 8, st));
+    CHECK_RC(excl_scan(ctx, P<uint64_t>(cnt), P<uint64_t>(off), n + 1));
+    HIPC(hipMemcpyAsync(&total, P<uint64_t>(off) + n, 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    std::vector<uint64_t> hoff(n + 1);
+    HIPC(hipMemcpy(hoff.data(), off.p, (n + 1) * 8, hipMemcpyDeviceToHost));
+    HIPC(hipMemsetAsync(P<uint32_t>(ctx->scal) + 1, 0, 4, st));
+    for (uint64_t a = 0; a < n;) {
+      const uint64_t oa = hoff[a];
+      uint64_t b = a + 1;
+      while (b < n && hoff[b + 1] - oa <= cap / 2) b++;
+      const uint64_t m = hoff[b] - oa;
+      if (m > mem_cap) return fail(ctx, SHCI_ERR_OOM, "selection: one det's candidates exceed the memory budget");
+      CHECK_RC(ensure(ctx, cand, (n_acc + m + 1) * sizeof(DetKey)));
+      if (n_acc) HIPC(hipMemcpyAsync(cand.p, acc.p, n_acc * sizeof(DetKey), hipMemcpyDeviceToDevice, st));
+      S.out = P<DetKey>(cand) + n_acc;
+      S.base = oa;
+      S.i0 = a;
+      S.i1 = b;
+      k_sel_walk<kSelFill><<<(uint32_t)std::min<uint64_t>(b - a, 1u << 20), 64, 0, st>>>(S);
+      HIPC(hipGetLastError());
+      if (n_acc + m > 0) {
+        CHECK_RC(ensure(ctx, srt, (n_acc + m) * sizeof(DetKey)));
+        CHECK_RC(ensure(ctx, acc, (n_acc + m) * sizeof(DetKey)));
+        CHECK_RC(unique_det_keys(ctx, P<DetKey>(cand), P<DetKey>(srt), n_acc + m, P<DetKey>(acc), &n_acc));
+      }
+      a = b;
+    }
+    HIPC(hipMemcpyAsync(&bad, P<uint32_t>(ctx->scal) + 1, 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    if (bad) return fail(ctx, SHCI_ERR_STATE, "selection: fill pass differs from the count pass");
+  }
+  HIPC(hipEventRecord(ctx->ev[3], st));
   CHECK_RC(ensure(ctx, ctx->sel_up, std::max<uint64_t>(n_acc, 1) * 8));
   CHECK_RC(ensure(ctx, ctx->sel_dn, std::max<uint64_t>(n_acc, 1) * 8));
   if (n_acc)
     k_sel_unpack<<<div_up(n_acc, 256), 256, 0, st>>>(P<DetKey>(acc), n_acc, n_orb, P<uint64_t>(ctx->sel_up),
                                                      P<uint64_t>(ctx->sel_dn));
   HIPC(hipMemcpyAsync(eps_tried, dE.p, n * 8, hipMemcpyDeviceToHost, st));
+  HIPC(hipEventRecord(ctx->ev[4], st));
   HIPC(hipStreamSynchronize(st));
+  for (int k = 0; k < 4; k++) HIPC(hipEventElapsedTime(&ctx->sel_ms[k], ctx->ev[k], ctx->ev[k + 1]));
   ctx->sel_n = n_acc;
   *n_new = n_acc;
   return SHCI_OK;
@@ -4422,7 +4520,7 @@ int shci_ctx_destroy(shci_ctx* ctx) {
                     &ctx->cnt,
                     &ctx->row_ptr, &ctx->row_end, &ctx->col, &ctx->v64, &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb,
                     &ctx->va, &ctx->vb, &ctx->scal, &ctx->part, &ctx->xv, &ctx->yv, &ctx->x32, &ctx->y32,
-                    &ctx->dav, &ctx->sq_S, &ctx->sq_r, &ctx->sq_len, &ctx->hq_H, &ctx->hq_rs, &ctx->hq_len,
+                    &ctx->dav, &ctx->sq_S, &ctx->sq_r, &ctx->sq_len, &ctx->sq_max, &ctx->hq_H, &ctx->hq_rs, &ctx->hq_len, &ctx->hq_max,
                     &ctx->sel_up, &ctx->sel_dn};
   for (DevBuf* b : bufs) release(ctx, *b);
   cache_trim(ctx);
@@ -4922,6 +5020,12 @@ int shci_select(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t 
   const int rc = select_sweep(ctx, up, dn, n_dets, coefs, n_states, eps_var, eps_tried_prev, n_new, bufs);
   for (DevBuf& b : bufs) release(ctx, b);
   return rc;
+}
+
+int shci_select_timings(shci_ctx* ctx, float* ms) {
+  if (!ctx || !ms) return SHCI_ERR_ARG;
+  for (int k = 0; k < 4; k++) ms[k] = ctx->sel_ms[k];
+  return SHCI_OK;
 }
 
 int shci_select_result(shci_ctx* ctx, uint64_t* new_up, uint64_t* new_dn, uint64_t n) {

@@ -24,6 +24,7 @@ class Selector:
         self.ham = ham
         self.ctx = ham.ctx
         self.last_ms = 0.0
+        self.last_phases_ms = {}
 
     def select(self, up, dn, coefs, eps_var, eps_tried_prev):
         """One sweep: returns the new dets (up, dn), (up, dn) ascending.
@@ -51,6 +52,9 @@ class Selector:
         self.ctx.check(lib.shci_select_result(self.ctx.handle, nu.ctypes.data_as(_u64p), nd.ctypes.data_as(_u64p), m),
                        "shci_select_result")
         self.last_ms = (time.perf_counter() - t0) * 1e3
+        ph = (C.c_float * 4)()
+        self.ctx.check(lib.shci_select_timings(self.ctx.handle, ph), "shci_select_timings")
+        self.last_phases_ms = dict(setup=ph[0], walk=ph[1], unique=ph[2], output=ph[3])
         return nu, nd
 
 

@@ -1,7 +1,8 @@
 """Benchmark workloads: seeded synthetic integrals (shci_amd.integrals) and the
 variational determinant list grown by the reference's selection loop
-(src/solver/solver.h:338-539) -- CPU heat-bath selection, GPU Hamiltonian and
-Davidson for the coefficients."""
+(src/solver/solver.h:338-539) -- heat-bath selection (on the device by
+default, or the host's queue walk), GPU Hamiltonian and Davidson for the
+coefficients."""
 from __future__ import annotations
 
 import sys
@@ -12,6 +13,7 @@ import numpy as np
 from . import integrals as si
 from .hamiltonian import Davidson, Hamiltonian
 from .host import HostSystem
+from .selection import GpuSystem
 
 __all__ = ["Workload", "grow_det_list", "SCHEDULES"]
 
@@ -39,8 +41,10 @@ class Workload:
 
 
 def grow_det_list(cfg: int, target: int | None = None, device: int = 0, verbose: bool = True,
-                  schedule=None, max_final_sweeps: int = 3) -> Workload:
-    """Grow the variational space of config ``cfg`` until ``target`` dets."""
+                  schedule=None, max_final_sweeps: int = 3, selection: str = "device") -> Workload:
+    """Grow the variational space of config ``cfg`` until ``target`` dets.
+    ``selection``: ``"device"`` (shci_select) or ``"host"`` (the host's queue
+    walk); both give the same det lists."""
     w = Workload(cfg)
     spec = w.spec
     target = target or spec.target_dets
@@ -51,6 +55,8 @@ def grow_det_list(cfg: int, target: int | None = None, device: int = 0, verbose:
     # (its atomic row sums may move a borderline det between runs).
     layout = "csr" if cfg in (1, 2, 4) else "segments"
     ham = Hamiltonian(w.n, w.h1, w.eri, w.core, device=device, store="fp32", layout=layout)
+    if selection == "device":
+        host = GpuSystem(ham, host.hf_up, host.hf_dn)
     dav = Davidson(spec.n_states)
     coefs = np.zeros((spec.n_states, 1))
     coefs[0, 0] = 1.0

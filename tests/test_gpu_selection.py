@@ -101,9 +101,10 @@ def test_gpu_select_lockstep_with_host():
 
 
 @pytest.mark.gpu
-def test_gpu_select_chunked_equals_single_pass():
-    """Candidates processed in many chunks (merged and made unique again)
-    give the same result as one pass."""
+def test_gpu_select_fallbacks_equal_single_pass():
+    """The single pass, its repeat with an exact buffer (first buffer too
+    small) and the count + chunked fill passes (budget too small: chunks
+    merged and made unique again) give the same result."""
     from shci_amd.hamiltonian import Hamiltonian
     from shci_amd.selection import Selector
 
@@ -114,15 +115,17 @@ def test_gpu_select_chunked_equals_single_pass():
     sel = Selector(Hamiltonian(n, h1, eri, core))
     e1 = eps_tried.copy()
     a = sel.select(up, dn, coefs, 2e-4, e1)
-    os.environ["SHCI_SELECT_CAP"] = "2000"
-    try:
-        e2 = eps_tried.copy()
-        b = sel.select(up, dn, coefs, 2e-4, e2)
-    finally:
-        del os.environ["SHCI_SELECT_CAP"]
-    print(f"[select] chunked: {len(a[0])} new dets", flush=True)
+    print(f"[select] single pass: {len(a[0])} new dets", flush=True)
     assert len(a[0]) > 2000
-    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(e1, e2)
+    for var, val in (("SHCI_SELECT_FIRST", "1000"), ("SHCI_SELECT_CAP", "2000")):
+        os.environ[var] = val
+        try:
+            e2 = eps_tried.copy()
+            b = sel.select(up, dn, coefs, 2e-4, e2)
+        finally:
+            del os.environ[var]
+        print(f"[select] {var}={val}: {len(b[0])} new dets", flush=True)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(e1, e2)
 
 
 @pytest.mark.gpu
