@@ -41,8 +41,9 @@ long long shci_host_select(void* sys, const double* coefs, unsigned n_states, do
 
 /* Solver::run_all_variations (solver.h:338-397) with the Hamiltonian build and
  * Davidson on the GPU (include/shci_hip.h).  eps_vars / schedule as in the
- * reference config; mode 0 = FP64 Davidson, 1 = mixed.  energies[n_states]
- * receives the final variational energies. */
+ * reference config; mode bit 0: mixed Davidson (else FP64); bit 1: selection
+ * on the device (shci_select, else the host queue walk).  energies[n_states]
+ * receives the final variational energies; the det list stays in sys. */
 int shci_host_run_variation(void* sys, const double* eps_vars, unsigned n_eps, const double* schedule,
                             unsigned n_sched, double target_error, unsigned n_states, int mode, int device,
                             int verbose, double* energies);

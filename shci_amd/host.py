@@ -88,6 +88,23 @@ class HostSystem:
     def reset(self):
         _lib().shci_host_reset_dets(self._h)
 
+    def run_variation(self, eps_vars, schedule, target_error=5e-5, n_states=1, mixed=False, gpu_select=False,
+                      device=0, verbose=False):
+        """The reference's variational loop in C++ (Solver::run_all_variations,
+        solver.h:338-397) through the drop-in adapter: GPU build and Davidson,
+        selection by the host queue walk or on the device.  Returns the final
+        variational energies; the det list stays here (``dets()``)."""
+        ev = np.ascontiguousarray(eps_vars, dtype=np.float64)
+        sc = np.ascontiguousarray(schedule, dtype=np.float64)
+        out = np.zeros(n_states)
+        mode = (1 if mixed else 0) | (2 if gpu_select else 0)
+        rc = _lib().shci_host_run_variation(self._h, ev.ctypes.data_as(_dp), len(ev), sc.ctypes.data_as(_dp),
+                                            len(sc), target_error, n_states, mode, device, int(verbose),
+                                            out.ctypes.data_as(_dp))
+        if rc != 0:
+            raise RuntimeError("shci_host: " + _lib().shci_host_last_error().decode())
+        return out
+
     def select(self, coefs, eps_var) -> int:
         """One selection sweep; appends new dets, returns how many were added."""
         coefs = np.atleast_2d(np.ascontiguousarray(coefs, dtype=np.float64))

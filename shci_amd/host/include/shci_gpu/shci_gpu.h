@@ -1,7 +1,9 @@
 // Drop-in C++ adapter: the reference's Hamiltonian<S> / SparseMatrix /
 // Davidson call surface (src/solver/hamiltonian.h:14-79,
-// src/solver/sparse_matrix.h:12-60, src/solver/davidson.h:6-29) implemented
-// over the C-ABI of include/shci_hip.h.  Header-only; link libshci_hip.so.
+// src/solver/sparse_matrix.h:12-60, src/solver/davidson.h:6-29) and the
+// selection sweep of its variational loop (src/solver/solver.h:435-495)
+// implemented over the C-ABI of include/shci_hip.h.  Header-only; link
+// libshci_hip.so.
 //
 // A system type S plugs in through shci_gpu::SystemTraits<S>, which must
 // provide
@@ -15,6 +17,7 @@
 // Errors throw std::runtime_error like the reference.
 #pragma once
 
+#include <algorithm>
 #include <complex>
 #include <cstdint>
 #include <stdexcept>
@@ -179,6 +182,52 @@ class Hamiltonian {
   }
   size_t n_dets_ = 0, n_dets_prev_ = 0;
   uint32_t flags_;
+  bool ints_set_ = false;
+};
+
+// One heat-bath selection sweep of the variational loop on the device
+// (solver.h:435-495, ChemSystem::find_connected_dets chem_system.cc:289-402):
+// the new dets come back (up, dn) ascending in new_up / new_dn (the caller
+// appends them, as the reference appends its hash set's dets); eps_tried_prev
+// (one per det, 1e100 for dets never tried) is updated and grown with 1e100
+// for the new dets.  coefs: n_states vectors of the current det count.
+template <class S>
+class Selector {
+ public:
+  explicit Selector(shci_ctx* ctx = nullptr) : ctx_(ctx ? ctx : default_context()) {}
+
+  size_t select(const S& system, const std::vector<std::vector<double>>& coefs, double eps_var,
+                std::vector<double>& eps_tried_prev, std::vector<uint64_t>& new_up, std::vector<uint64_t>& new_dn) {
+    using T = SystemTraits<S>;
+    if (!ints_set_) {
+      unsigned n_orb;
+      std::vector<double> h1, eri;
+      double core;
+      T::integrals(system, n_orb, h1, eri, core);
+      check(ctx_, shci_set_integrals(ctx_, n_orb, h1.data(), eri.data(), core), "shci_set_integrals");
+      ints_set_ = true;
+    }
+    std::vector<uint64_t> up, dn;
+    T::dets(system, up, dn);
+    const size_t n = up.size();
+    eps_tried_prev.resize(n, 1.0e100);
+    std::vector<double> c(coefs.size() * n, 0.0);
+    for (size_t s = 0; s < coefs.size(); s++)
+      std::copy(coefs[s].begin(), coefs[s].begin() + std::min(n, coefs[s].size()), c.begin() + s * n);
+    uint64_t m = 0;
+    check(ctx_,
+          shci_select(ctx_, up.data(), dn.data(), n, c.data(), (uint32_t)coefs.size(), eps_var,
+                      eps_tried_prev.data(), &m),
+          "shci_select");
+    new_up.resize(m);
+    new_dn.resize(m);
+    check(ctx_, shci_select_result(ctx_, new_up.data(), new_dn.data(), m), "shci_select_result");
+    eps_tried_prev.resize(n + m, 1.0e100);
+    return m;
+  }
+
+ private:
+  shci_ctx* ctx_;
   bool ints_set_ = false;
 };
 

@@ -1,6 +1,7 @@
 // The reference's variational loop (src/solver/solver.h:338-539) with the
-// CPU heat-bath selection feeding the GPU Hamiltonian build and Davidson
-// through the drop-in adapter (include/shci_gpu/shci_gpu.h).
+// heat-bath selection (host queue walk, or the device through the adapter)
+// feeding the GPU Hamiltonian build and Davidson through the drop-in adapter
+// (include/shci_gpu/shci_gpu.h).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -53,8 +54,11 @@ extern "C" int shci_host_run_variation(void* h, const double* eps_vars, unsigned
   auto* s = static_cast<HostSystem*>(h);
   try {
     shci_ctx* ctx = shci_gpu::default_context(device);
-    const uint32_t store = mode == 1 ? (SHCI_STORE_FP32 | SHCI_STORE_FP64) : SHCI_STORE_FP64;
+    const bool mixed = (mode & 1) != 0, gpu_select = (mode & 2) != 0;
+    const uint32_t store = mixed ? (SHCI_STORE_FP32 | SHCI_STORE_FP64) : SHCI_STORE_FP64;
     shci_gpu::Hamiltonian<VarSystem> hamiltonian(ctx, store);
+    shci_gpu::Selector<VarSystem> selector(ctx);
+    std::vector<uint64_t> new_up, new_dn;
     VarSystem sys{&s->ints, &s->dets};
     s->dets.assign(1, Det{s->ints.hf_up, s->ints.hf_dn});
     s->eps_tried.assign(1, 1.0e100);
@@ -73,12 +77,18 @@ extern "C" int shci_host_run_variation(void* h, const double* eps_vars, unsigned
       for (const auto& step : plan) {
         const double eps = step.first;
         const bool until = step.second;
-        shci_gpu::Davidson davidson(n_states, mode == 1 ? SHCI_DAVIDSON_MIXED : SHCI_DAVIDSON_FP64);
+        shci_gpu::Davidson davidson(n_states, mixed ? SHCI_DAVIDSON_MIXED : SHCI_DAVIDSON_FP64);
         size_t n_dets = s->dets.size();
         bool dets_converged = false, converged = false;
         while (!converged) {
           if (!dets_converged) {
-            const size_t n_new = s->sel->select(s->dets, coefs, eps, s->eps_tried);
+            size_t n_new;
+            if (gpu_select) {
+              n_new = selector.select(sys, coefs, eps, s->eps_tried, new_up, new_dn);
+              for (size_t i = 0; i < n_new; i++) s->dets.push_back(Det{new_up[i], new_dn[i]});
+            } else {
+              n_new = s->sel->select(s->dets, coefs, eps, s->eps_tried);
+            }
             for (auto& c : coefs) c.resize(s->dets.size(), 1e-16);
             if (verbose) printf("Number of dets / new dets: %zu / %zu\n", s->dets.size(), n_new);
             hamiltonian.update(sys);

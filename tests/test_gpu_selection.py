@@ -183,3 +183,24 @@ def test_examples_C_gpu_selection(golden_dir):
         m = min(len(g["davidson"]), len(rec["traj"]))
         np.testing.assert_allclose(rec["traj"][:m], g["davidson"][:m], rtol=0, atol=2e-10 if k < 6 else 2e-9)
     assert abs(recs[-1]["evals"][0] - gold["energy_var"]["5.00e-05"]) < 1e-8
+
+
+@pytest.mark.gpu
+def test_examples_C_cpp_loop_selection_modes(golden_dir):
+    """The reference's loop in C++ through the drop-in adapter
+    (shci_host_run_variation), examples/C to eps 5e-5: the host queue walk and
+    the adapter's device Selector give the same det list and the reference's
+    final energy."""
+    from shci_amd.host import HostSystem
+
+    gold = json.loads((golden_dir / "examples_C.json").read_text())
+    lists, energies = [], []
+    for gpu_select in (False, True):
+        host = HostSystem.from_fcidump(golden_dir / "C_FCIDUMP", 4, 2)
+        e = host.run_variation([5e-5], [2e-3, 1e-3, 5e-4, 2e-4, 1e-4], gpu_select=gpu_select)
+        lists.append(host.dets())
+        energies.append(e[0])
+        print(f"[select] C++ loop, device selection {gpu_select}: {host.n_dets} dets, E {e[0]:.10f}", flush=True)
+    assert np.array_equal(lists[0][0], lists[1][0]) and np.array_equal(lists[0][1], lists[1][1])
+    for e in energies:
+        assert abs(e - gold["energy_var"]["5.00e-05"]) < 1e-8
