@@ -96,6 +96,8 @@ def _load():
     lib.orc_select.argtypes = [C.c_void_p, _u64p, _u64p, C.c_size_t, _dp, C.c_uint, C.c_double, _dp,
                                C.POINTER(C.c_size_t)]
     lib.orc_select_result.argtypes = [C.c_void_p, _u64p, _u64p]
+    lib.orc_pt_dtm.argtypes = [C.c_void_p, _u64p, _u64p, C.c_size_t, _dp, C.c_double, C.c_double, _dp,
+                               C.POINTER(C.c_size_t)]
     _lib = lib
     return lib
 
@@ -261,6 +263,19 @@ class System:
         nd = np.zeros(m.value, dtype=np.uint64)
         _check(_load().orc_select_result(self._h, nu.ctypes.data_as(_u64p), nd.ctypes.data_as(_u64p)))
         return nu, nd
+
+
+    def pt_dtm(self, up, dn, coefs, eps_pt_dtm, energy_var):
+        """Deterministic PT2 correction (one batch) of the wavefunction
+        (up, dn, coefs of one state): (E_pt2, number of PT dets)."""
+        up = np.ascontiguousarray(up, dtype=np.uint64)
+        dn = np.ascontiguousarray(dn, dtype=np.uint64)
+        c = np.ascontiguousarray(coefs, dtype=np.float64).ravel()
+        e = C.c_double()
+        m = C.c_size_t()
+        _check(_load().orc_pt_dtm(self._h, up.ctypes.data_as(_u64p), dn.ctypes.data_as(_u64p), len(up), _dptr(c),
+                                  eps_pt_dtm, energy_var, C.byref(e), C.byref(m)))
+        return e.value, m.value
 
 
 def sparse_davidson(dim, entries, init, tol):

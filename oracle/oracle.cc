@@ -1242,6 +1242,101 @@ static std::vector<Det> select_sweep(const Integrals& I, const Chem& chem, const
   return res;
 }
 
+
+// ---------------------------------------------------------------------------
+// Deterministic PT2 correction, one batch: Solver::get_energy_pt_dtm
+// (src/solver/solver.h:637-764).  For every variational det i, the dets a
+// that find_connected_dets yields with eps_min = eps_pt_dtm / |c_i| and no
+// upper cut (chem_system.cc:289-402: singles with S(p, r) >= eps_min, doubles
+// with queue |H| >= eps_min), restated without the queues; a outside the
+// variational space with |H_ai c_i| >= eps_pt_dtm adds H_ai c_i to
+// hc_sums[a] (solver.h:710-720, summed in increasing i here).  Then
+// E_pt2 = sum_a hc_sums[a]^2 / (E_var - H_aa) (solver.h:728-733), summed over
+// a in (up, dn) order.
+// ---------------------------------------------------------------------------
+static double pt_dtm(const Integrals& I, const Chem& chem, const std::vector<Det>& dets, const double* coefs,
+                     double eps_pt, double energy_var, size_t* n_pt_dets) {
+  const unsigned n = I.n_orbs;
+  const size_t nd = dets.size();
+  const unsigned ne = (unsigned)(__builtin_popcountll(dets[0].up) + __builtin_popcountll(dets[0].dn));
+  std::vector<double> S((size_t)n * n, 0.0);
+  for (unsigned p = 0; p < n; p++)
+    for (unsigned r = 0; r < n; r++)
+      if (p != r) S[p * n + r] = selection_S(I, p, r, ne);
+  std::unordered_set<Det, DetHash> var(dets.begin(), dets.end());
+  const uint64_t all = n < 64 ? (1ull << n) - 1 : ~0ull;
+  struct Term {
+    Det a;
+    size_t i;
+    double hc;
+  };
+  std::vector<std::vector<Term>> found(omp_get_max_threads());
+#pragma omp parallel for schedule(dynamic, 16)
+  for (size_t i = 0; i < nd; i++) {
+    auto& out = found[omp_get_thread_num()];
+    const double c = coefs[i];
+    const double emin = eps_pt / std::abs(c);
+    if (!(emin < 1e300)) continue;  // c == 0: nothing passes
+    const Det& d = dets[i];
+    auto add = [&](const Det& a, int n_excite) {
+      if (var.count(a)) return;
+      const double hc = chem.elem(d, a, n_excite) * c;
+      if (std::abs(hc) < eps_pt) return;
+      out.push_back(Term{a, i, hc});
+    };
+    for (int spin = 0; spin < 2; spin++) {
+      const uint64_t str = spin ? d.dn : d.up;
+      const auto oc = occ(str), vi = occ(all & ~str);
+      for (unsigned p : oc)
+        for (unsigned r : vi) {
+          const double s = S[p * n + r];
+          if (s == 0.0 || s < emin) continue;
+          const uint64_t t = (str & ~(1ull << p)) | (1ull << r);
+          add(spin ? Det{d.up, t} : Det{t, d.dn}, 1);
+        }
+      for (size_t a = 0; a < oc.size(); a++)
+        for (size_t b = a + 1; b < oc.size(); b++)
+          for (size_t x = 0; x < vi.size(); x++)
+            for (size_t y = x + 1; y < vi.size(); y++) {
+              const unsigned p = oc[a], q = oc[b], r = vi[x], s = vi[y];
+              const double H = std::abs(I.get_2b(p, r, q, s) - I.get_2b(p, s, q, r));
+              if (H == 0.0 || H < emin) continue;
+              const uint64_t t = (str & ~(1ull << p) & ~(1ull << q)) | (1ull << r) | (1ull << s);
+              add(spin ? Det{d.up, t} : Det{t, d.dn}, 2);
+            }
+    }
+    const auto ou = occ(d.up), vu = occ(all & ~d.up), od = occ(d.dn), vd = occ(all & ~d.dn);
+    for (unsigned p : ou)
+      for (unsigned r : vu)
+        for (unsigned q : od)
+          for (unsigned s : vd) {
+            const double H = std::abs(I.get_2b(p, r, q, s));
+            if (H == 0.0 || H < emin) continue;
+            add(Det{(d.up & ~(1ull << p)) | (1ull << r), (d.dn & ~(1ull << q)) | (1ull << s)}, 2);
+          }
+  }
+  std::vector<Term> all_terms;
+  for (auto& v : found) all_terms.insert(all_terms.end(), v.begin(), v.end());
+  std::sort(all_terms.begin(), all_terms.end(), [](const Term& x, const Term& y) {
+    if (x.a.up != y.a.up) return x.a.up < y.a.up;
+    if (x.a.dn != y.a.dn) return x.a.dn < y.a.dn;
+    return x.i < y.i;
+  });
+  double e = 0.0;
+  size_t np = 0;
+  for (size_t k = 0; k < all_terms.size();) {
+    size_t k2 = k;
+    double s = 0.0;
+    while (k2 < all_terms.size() && all_terms[k2].a == all_terms[k].a) s += all_terms[k2++].hc;
+    const double haa = chem.elem(all_terms[k].a, all_terms[k].a, 0);
+    e += s * s / (energy_var - haa);
+    np++;
+    k = k2;
+  }
+  *n_pt_dets = np;
+  return e;
+}
+
 struct OrcSystem {
   Integrals ints;
   Chem chem;
@@ -1522,6 +1617,20 @@ int orc_select_result(void* h, uint64_t* up, uint64_t* dn) {
     dn[i] = s->selected[i].dn;
   }
   return 0;
+}
+
+int orc_pt_dtm(void* h, const uint64_t* up, const uint64_t* dn, size_t n_dets, const double* coefs, double eps_pt,
+               double energy_var, double* energy_pt, size_t* n_pt_dets) {
+  try {
+    auto* s = static_cast<OrcSystem*>(h);
+    std::vector<Det> dets(n_dets);
+    for (size_t i = 0; i < n_dets; i++) dets[i] = Det{up[i], dn[i]};
+    *energy_pt = pt_dtm(s->ints, s->chem, dets, coefs, eps_pt, energy_var, n_pt_dets);
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
 }
 
 int orc_rdm(void* h, unsigned n_states, const double* coefs, double* rdm1, double* rdm2) {
