@@ -5,9 +5,11 @@
  *   SparseMatrix::mul               src/solver/sparse_matrix.cc:88-116
  *   Davidson::diagonalize           src/solver/davidson.cc:37-225
  *   RDM::get_1rdm / get_2rdm        src/chem/rdm.cc:12-131, 374-395
- * and the step before it, the heat-bath selection sweep
+ * the step before it, the heat-bath selection sweep
  *   Solver<S>::run_variation loop   src/solver/solver.h:435-495
  *   ChemSystem::find_connected_dets src/chem/chem_system.cc:289-402
+ * and the step after it, the deterministic PT2 correction
+ *   Solver<S>::get_energy_pt_dtm    src/solver/solver.h:637-764
  * Each entry point below names the reference interface it replaces.  All
  * arguments are plain pointers and sizes; host pointers unless a flag says
  * otherwise.  Every call returns SHCI_OK (0) or a negative status; the message
@@ -159,6 +161,19 @@ int shci_select(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t 
                 uint32_t n_states, double eps_var, double* eps_tried_prev, uint64_t* n_new);
 /* The last sweep's new dets: n must equal its *n_new. */
 int shci_select_result(shci_ctx* ctx, uint64_t* new_up, uint64_t* new_dn, uint64_t n);
+/* Deterministic PT2 correction of one state (Solver::get_energy_pt_dtm,
+ * solver.h:637-764) on the device.  up/dn, coefs: the n_dets variational dets
+ * (host, caller order) and that state's coefficients.  For every det i, the
+ * dets a that find_connected_dets yields with eps_min = eps_pt_dtm / |c_i|
+ * and no upper cut (chem_system.cc:289-402), outside the variational space
+ * and with |H_ai c_i| >= eps_pt_dtm, add H_ai c_i to hc_sums[a]; then
+ * *energy_pt = sum_a hc_sums[a]^2 / (energy_var - H_aa) and *n_pt_dets the
+ * number of such a.  Each hc_sums[a] adds its terms in increasing i.
+ * n_batches: partitions of the a by hash (0 = as few as device memory
+ * allows); the result does not depend on it beyond the summation order. */
+int shci_pt_dtm(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n_dets, const double* coefs,
+                double eps_pt_dtm, double energy_var, uint32_t n_batches, double* energy_pt, uint64_t* n_pt_dets);
+
 /* Phases of the last sweep (no reference counterpart; diagnostics), HIP
  * events on the context's stream, ms[4]: upload, checks, queues and the
  * variational-space hash set; the walk (one append pass); any repeated or

@@ -42,6 +42,8 @@ EXPORTS = {
     "shci_select": (C.c_int, [_vp, _u64p, _u64p, C.c_uint64, _dp, C.c_uint32, C.c_double, _dp, _u64p]),
     "shci_select_result": (C.c_int, [_vp, _u64p, _u64p, C.c_uint64]),
     "shci_select_timings": (C.c_int, [_vp, _fp]),
+    "shci_pt_dtm": (C.c_int, [_vp, _u64p, _u64p, C.c_uint64, _dp, C.c_double, C.c_double, C.c_uint32, _dp,
+                             _u64p]),
     "shci_get_build_timings": (C.c_int, [_vp, _fp]),
     "shci_time_spmv": (C.c_int, [_vp, C.c_uint32, C.c_int, _fp, _dp]),
     "shci_time_spmm": (C.c_int, [_vp, C.c_uint32, C.c_int, _fp, _dp]),

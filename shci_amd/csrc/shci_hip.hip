@@ -127,7 +127,8 @@ struct shci_ctx {
   DevBuf hq_H, hq_rs, hq_len, hq_max;   // n*n double lists of n*n entries: |H| descending, (r, s)
   DevBuf sel_up, sel_dn;                // last sweep's new dets, (up, dn) ascending
   uint64_t sel_n = 0;
-  float sel_ms[4] = {};                 // last sweep: set-up, count walk, fill walk + unique, output
+  float sel_ms[4] = {};                 // last sweep: set-up, walk, repeats + unique, output
+  uint32_t pt_batches = 0;              // batches of the last PT2 pass
   // Released device blocks kept for reuse (builds allocate and drop the same
   // temporaries every time; hipMalloc/hipFree of GB blocks per build would
   // cost more than the kernels).  Freed on memory pressure, clear and destroy.
@@ -4048,6 +4049,8 @@ struct SelParams {
   uint32_t* __restrict__ bad;        // fill pass: set when a det's count differs from the count pass
   unsigned long long* __restrict__ total;  // append pass: candidates found (also past cap)
   uint64_t cap;                      // append pass: capacity of out
+  double* __restrict__ outv;         // PT fill pass: H_ai c_i of each written candidate
+  uint32_t n_batches, batch;         // PT: only candidates whose hash falls in this batch
   uint64_t i0, i1;                   // det range
 };
 
@@ -4089,7 +4092,7 @@ __device__ __forceinline__ void sel_flush(const SelParams& S, const DetKey* stag
   scnt = 0;
 }
 
-template <int MODE>
+template <int MODE, bool PT = false>
 __global__ __launch_bounds__(64) void k_sel_walk(SelParams S) {
   constexpr bool FILL = MODE != kSelCount;
   __shared__ uint8_t occ[128];
@@ -4105,7 +4108,7 @@ __global__ __launch_bounds__(64) void k_sel_walk(SelParams S) {
       if (fabs(c) > fabs(mc)) mc = c;
     }
     const double emin = S.eps_var / fabs(mc);
-    const double emax = S.eps_tried[i];
+    const double emax = PT ? __builtin_inf() : S.eps_tried[i];  // PT: no upper cut (solver.h:639-642)
     uint64_t cur = MODE == kSelFill ? S.off[i] - S.base : 0;
     const uint64_t lim = MODE == kSelFill ? S.off[i + 1] - S.base : 0;  // the det's range from the count pass
     if (emin >= emax) {
@@ -4208,12 +4211,29 @@ __global__ __launch_bounds__(64) void k_sel_walk(SelParams S) {
             }
           }
           if (ok) ok = !sel_in_var(S, cu, cd);
-          if (ok && bk == 1) {
-            const double h = bp < n ? single_elem(S.I, u, cu, d) : single_elem(S.I, d, cd, u);
-            ok = fabs(h) >= emin;
+          double hc = 0.0;
+          if (!PT) {
+            if (ok && bk == 1) {
+              const double h = bp < n ? single_elem(S.I, u, cu, d) : single_elem(S.I, d, cd, u);
+              ok = fabs(h) >= emin;
+            }
+          } else if (ok) {
+            // PT: H_ai c_i with the build's elements (det i on the left), kept
+            // when |H_ai c_i| >= eps_pt_dtm (solver.h:710-717)
+            double h;
+            if (bk == 1) h = bp < n ? single_elem(S.I, u, cu, d) : single_elem(S.I, d, cd, u);
+            else if (bq < n) h = double_same(S.I, u, cu);
+            else if (bp >= n) h = double_same(S.I, d, cd);
+            else h = double_mixed(S.I, u, cu, d, cd);
+            hc = h * mc;
+            ok = fabs(hc) >= S.eps_var;
+            if (ok && S.n_batches > 1) ok = (uint32_t)((det_hash(cu, cd) >> 20) % S.n_batches) == S.batch;
           }
           const uint64_t em = __ballot(ok);
-          if (MODE == kSelFill && ok && cur + mbcnt(em) < lim) S.out[cur + mbcnt(em)] = pack_det(cu, cd, n);
+          if (MODE == kSelFill && ok && cur + mbcnt(em) < lim) {
+            S.out[cur + mbcnt(em)] = pack_det(cu, cd, n);
+            if (PT) S.outv[cur + mbcnt(em)] = hc;
+          }
           if (MODE == kSelAppend && em) {
             if (ok) stage[scnt + mbcnt(em)] = pack_det(cu, cd, n);
             scnt += (uint32_t)__popcll(em);
@@ -4225,7 +4245,7 @@ __global__ __launch_bounds__(64) void k_sel_walk(SelParams S) {
       }
     }
     if (lane == 0) {
-      if (FILL) S.eps_tried[i] = 1.000000001 * emin;
+      if (FILL && !PT) S.eps_tried[i] = 1.000000001 * emin;
       if (MODE == kSelFill && cur != lim) atomicOr(S.bad, 1u);
       if (MODE == kSelCount) S.cnt[i] = cur;
     }
@@ -4325,10 +4345,11 @@ __global__ void k_sel_check(const uint64_t* __restrict__ up, const uint64_t* __r
     atomicOr(bad, 1u);
 }
 
-int select_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n, const double* coefs,
-                 uint32_t n_states, double eps_var, double* eps_tried, uint64_t* n_new, DevBuf* bufs) {
-  DevBuf &dU = bufs[0], &dD = bufs[1], &dC = bufs[2], &dE = bufs[3], &tab = bufs[4], &cnt = bufs[5],
-         &off = bufs[6], &cand = bufs[7], &srt = bufs[8], &acc = bufs[9];
+// Uploads the dets, coefficients and (if given) eps_tried_prev, checks the
+// dets, builds the queues and the set of the dets; fills S for the walks.
+int sel_prepare(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n, const double* coefs,
+                uint32_t n_states, const double* eps_tried, DevBuf* bufs, SelParams& S) {
+  DevBuf &dU = bufs[0], &dD = bufs[1], &dC = bufs[2], &dE = bufs[3], &tab = bufs[4];
   const hipStream_t st = ctx->stream;
   const uint32_t n_orb = ctx->n_orb;
   const uint64_t omask = n_orb < 64 ? (1ull << n_orb) - 1ull : ~0ull;
@@ -4336,13 +4357,13 @@ int select_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t
   CHECK_RC(ensure(ctx, dU, n * 8));
   CHECK_RC(ensure(ctx, dD, n * 8));
   CHECK_RC(ensure(ctx, dC, (size_t)n_states * n * 8));
-  CHECK_RC(ensure(ctx, dE, n * 8));
+  if (eps_tried) CHECK_RC(ensure(ctx, dE, n * 8));
   CHECK_RC(ensure(ctx, ctx->scal, 64));
   HIPC(hipEventRecord(ctx->ev[0], st));
   HIPC(hipMemcpyAsync(dU.p, up, n * 8, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(dD.p, dn, n * 8, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(dC.p, coefs, (size_t)n_states * n * 8, hipMemcpyHostToDevice, st));
-  HIPC(hipMemcpyAsync(dE.p, eps_tried, n * 8, hipMemcpyHostToDevice, st));
+  if (eps_tried) HIPC(hipMemcpyAsync(dE.p, eps_tried, n * 8, hipMemcpyHostToDevice, st));
   HIPC(hipMemsetAsync(ctx->scal.p, 0, 8, st));
   k_sel_check<<<div_up(n, 256), 256, 0, st>>>(P<uint64_t>(dU), P<uint64_t>(dD), n, omask, ne, P<uint32_t>(ctx->scal));
   uint32_t bad = 0;
@@ -4356,14 +4377,13 @@ int select_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t
   CHECK_RC(ensure(ctx, tab, H * 16));
   HIPC(hipMemsetAsync(tab.p, 0xFF, H * 16, st));
   k_sel_hash_insert<<<div_up(n, 256), 256, 0, st>>>(P<uint64_t>(dU), P<uint64_t>(dD), n, P<uint64_t>(tab), H - 1);
-  SelParams S;
+  S = SelParams{};
   S.up = P<uint64_t>(dU);
   S.dn = P<uint64_t>(dD);
   S.n = n;
   S.coefs = P<double>(dC);
   S.n_states = n_states;
-  S.eps_var = eps_var;
-  S.eps_tried = P<double>(dE);
+  S.eps_tried = eps_tried ? P<double>(dE) : nullptr;
   S.htab = P<uint64_t>(tab);
   S.hmask = H - 1;
   S.sqS = P<double>(ctx->sq_S);
@@ -4375,14 +4395,24 @@ int select_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t
   S.hqLen = P<uint32_t>(ctx->hq_len);
   S.hqMax = P<double>(ctx->hq_max);
   S.I = make_ints(ctx);
-  S.out = nullptr;
   S.bad = P<uint32_t>(ctx->scal) + 1;
   S.total = reinterpret_cast<unsigned long long*>(P<uint64_t>(ctx->scal) + 1);
-  S.cnt = nullptr;
-  S.off = nullptr;
-  S.base = 0;
+  S.n_batches = 1;
+  S.batch = 0;
   S.i0 = 0;
   S.i1 = n;
+  return SHCI_OK;
+}
+
+int select_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n, const double* coefs,
+                 uint32_t n_states, double eps_var, double* eps_tried, uint64_t* n_new, DevBuf* bufs) {
+  DevBuf &dE = bufs[3], &cnt = bufs[5], &off = bufs[6], &cand = bufs[7], &srt = bufs[8], &acc = bufs[9];
+  const hipStream_t st = ctx->stream;
+  const uint32_t n_orb = ctx->n_orb;
+  SelParams S;
+  CHECK_RC(sel_prepare(ctx, up, dn, n, coefs, n_states, eps_tried, bufs, S));
+  S.eps_var = eps_var;
+  uint32_t bad = 0;
   // Candidate budget: ~64 B each (keys, sorted keys, flags, offsets, sort
   // storage) within 3/4 of the free memory (plus the context's cached blocks).
   size_t free_b = 0, total_b = 0;
@@ -4485,6 +4515,157 @@ int select_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t
   for (int k = 0; k < 4; k++) HIPC(hipEventElapsedTime(&ctx->sel_ms[k], ctx->ev[k], ctx->ev[k + 1]));
   ctx->sel_n = n_acc;
   *n_new = n_acc;
+  return SHCI_OK;
+}
+
+
+// Deterministic PT2 reduction: run starts of the sorted (det a, H_ai c_i)
+// pairs, then per distinct a the sum of its terms in their sorted order (the
+// stable sort keeps the walk's det order i), and its contribution
+// hc_sum^2 / (E_var - H_aa) (solver.h:728-733).
+__global__ void k_pt_starts(const uint64_t* __restrict__ flag, const uint64_t* __restrict__ pos, uint64_t m,
+                            uint64_t* __restrict__ starts) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m && flag[i]) starts[pos[i]] = i;
+}
+
+__global__ void k_pt_contrib(Ints I, const DetKey* __restrict__ k, const double* __restrict__ v,
+                             const uint64_t* __restrict__ starts, uint64_t nu, double energy_var,
+                             double* __restrict__ contrib) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nu) return;
+  const uint64_t b = starts[r], e = starts[r + 1];
+  double sum = 0.0;
+  for (uint64_t t = b; t < e; t++) sum += v[t];
+  uint64_t au, ad;
+  unpack_det(k[b], (uint32_t)I.n, au, ad);
+  const double haa = diag_elem(I, au, ad);
+  contrib[r] = sum * sum / (energy_var - haa);
+}
+
+// Fixed-order sums: block b adds the contiguous chunk [b * chunk, (b + 1) *
+// chunk) (strided per thread, then an LDS tree); the host adds the block
+// sums in order.  Deterministic for a given length.
+__global__ __launch_bounds__(256) void k_sum_chunks(const double* __restrict__ x, uint64_t m, uint64_t chunk,
+                                                    double* __restrict__ part) {
+  __shared__ double sh[256];
+  const uint64_t b0 = (uint64_t)blockIdx.x * chunk, b1 = b0 + chunk < m ? b0 + chunk : m;
+  double a = 0.0;
+  for (uint64_t t = b0 + threadIdx.x; t < b1; t += 256) a += x[t];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (uint32_t w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+int sort_det_pairs(shci_ctx* ctx, const DetKey* kin, DetKey* kout, const double* vin, double* vout, uint64_t m,
+                   int end_bit) {
+  size_t tb = 0;
+  HIPC(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kin, kout, vin, vout, m, DetKeyDecomposer{}, 0, end_bit,
+                                          ctx->stream));
+  CHECK_RC(ensure(ctx, ctx->tmp, tb));
+  HIPC(hipcub::DeviceRadixSort::SortPairs(ctx->tmp.p, tb, kin, kout, vin, vout, m, DetKeyDecomposer{}, 0, end_bit,
+                                          ctx->stream));
+  return SHCI_OK;
+}
+
+// One deterministic PT2 pass (all batches) over the dets: bufs as for the
+// selection plus [10..14] values, sorted values, run flags, their offsets,
+// run starts.
+int pt_dtm_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n, const double* coefs,
+                 double eps_pt, double energy_var, uint32_t n_batches, double* energy_pt, uint64_t* n_pt,
+                 DevBuf* bufs) {
+  DevBuf &cnt = bufs[5], &off = bufs[6], &keys = bufs[7], &srt = bufs[8], &vals = bufs[10], &vsrt = bufs[11],
+         &fl = bufs[12], &pos = bufs[13], &runs = bufs[14], &sums = bufs[9];
+  const hipStream_t st = ctx->stream;
+  SelParams S;
+  CHECK_RC(sel_prepare(ctx, up, dn, n, coefs, 1, nullptr, bufs, S));
+  S.eps_var = eps_pt;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(n, 1u << 20);
+  CHECK_RC(ensure(ctx, cnt, (n + 1) * 8));
+  CHECK_RC(ensure(ctx, off, (n + 1) * 8));
+  S.cnt = P<uint64_t>(cnt);
+  S.off = P<uint64_t>(off);
+  size_t free_b = 0, total_b = 0;
+  HIPC(hipMemGetInfo(&free_b, &total_b));
+  for (const DevBuf& c : ctx->cache) free_b += c.bytes;
+  // ~104 B per pair: keys and values, their sorted copies, flags, offsets,
+  // run starts and contributions, sort storage
+  const uint64_t cap = std::max<uint64_t>(1ull << 20, (uint64_t)(0.75 * (double)free_b / 104.0));
+  const bool autob = n_batches == 0;
+  uint32_t nb = autob ? 1 : n_batches;
+  double e_sum = 0.0;
+  uint64_t np_sum = 0;
+  HIPC(hipEventRecord(ctx->ev[1], st));
+  for (uint32_t b = 0; b < nb; b++) {
+    S.n_batches = nb;
+    S.batch = b;
+    k_sel_walk<kSelCount, true><<<grid, 64, 0, st>>>(S);
+    HIPC(hipMemsetAsync(P<uint64_t>(cnt) + n, 0, 8, st));
+    CHECK_RC(excl_scan(ctx, P<uint64_t>(cnt), P<uint64_t>(off), n + 1));
+    uint64_t m = 0;
+    HIPC(hipMemcpyAsync(&m, P<uint64_t>(off) + n, 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    if (m > cap) {
+      if (!autob || b > 0) return fail(ctx, SHCI_ERR_OOM, "PT: a batch's terms exceed the memory budget");
+      nb = (uint32_t)std::min<uint64_t>((m + cap - 1) / cap * 2, 1u << 16);  // hash batches are uneven
+      b = (uint32_t)-1;
+      continue;
+    }
+    if (m == 0) continue;
+    CHECK_RC(ensure(ctx, keys, m * sizeof(DetKey)));
+    CHECK_RC(ensure(ctx, srt, m * sizeof(DetKey)));
+    CHECK_RC(ensure(ctx, vals, m * 8));
+    CHECK_RC(ensure(ctx, vsrt, m * 8));
+    CHECK_RC(ensure(ctx, fl, (m + 1) * 8));
+    CHECK_RC(ensure(ctx, pos, (m + 1) * 8));
+    S.out = P<DetKey>(keys);
+    S.outv = P<double>(vals);
+    S.base = 0;
+    HIPC(hipMemsetAsync(P<uint32_t>(ctx->scal) + 1, 0, 4, st));
+    k_sel_walk<kSelFill, true><<<grid, 64, 0, st>>>(S);
+    HIPC(hipGetLastError());
+    CHECK_RC(sort_det_pairs(ctx, P<DetKey>(keys), P<DetKey>(srt), P<double>(vals), P<double>(vsrt), m,
+                            2 * (int)ctx->n_orb));
+    k_sel_first_flags<<<div_up(m, 256), 256, 0, st>>>(P<DetKey>(srt), m, P<uint64_t>(fl));
+    HIPC(hipMemsetAsync(P<uint64_t>(fl) + m, 0, 8, st));
+    CHECK_RC(excl_scan(ctx, P<uint64_t>(fl), P<uint64_t>(pos), m + 1));
+    uint64_t nu = 0;
+    uint32_t bad = 0;
+    HIPC(hipMemcpyAsync(&nu, P<uint64_t>(pos) + m, 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(&bad, P<uint32_t>(ctx->scal) + 1, 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    if (bad) return fail(ctx, SHCI_ERR_STATE, "PT: fill pass differs from the count pass");
+    // run starts (nu + 1 of them, the last = m: nu may exceed the det count);
+    // the contributions (nu <= m) reuse the unsorted values
+    CHECK_RC(ensure(ctx, runs, (nu + 1) * 8));
+    uint64_t* starts = P<uint64_t>(runs);
+    k_pt_starts<<<div_up(m, 256), 256, 0, st>>>(P<uint64_t>(fl), P<uint64_t>(pos), m, starts);
+    HIPC(hipMemcpyAsync(starts + nu, &m, 8, hipMemcpyHostToDevice, st));
+    double* contrib = P<double>(vals);
+    k_pt_contrib<<<div_up(nu, 256), 256, 0, st>>>(S.I, P<DetKey>(srt), P<double>(vsrt), starts, nu, energy_var,
+                                                   contrib);
+    const uint64_t G = std::min<uint64_t>(1024, (nu + 255) / 256);
+    const uint64_t chunk = (nu + G - 1) / G;
+    CHECK_RC(ensure(ctx, sums, G * 8));
+    k_sum_chunks<<<(uint32_t)G, 256, 0, st>>>(contrib, nu, chunk, P<double>(sums));
+    std::vector<double> part(G);
+    HIPC(hipMemcpyAsync(part.data(), sums.p, G * 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    for (double x : part) e_sum += x;
+    np_sum += nu;
+  }
+  HIPC(hipEventRecord(ctx->ev[2], st));
+  HIPC(hipEventSynchronize(ctx->ev[2]));
+  HIPC(hipEventElapsedTime(&ctx->sel_ms[1], ctx->ev[1], ctx->ev[2]));
+  HIPC(hipEventElapsedTime(&ctx->sel_ms[0], ctx->ev[0], ctx->ev[1]));
+  ctx->sel_ms[2] = ctx->sel_ms[3] = 0.0f;
+  ctx->pt_batches = nb;
+  *energy_pt = e_sum;
+  *n_pt = np_sum;
   return SHCI_OK;
 }
 
@@ -5018,6 +5199,23 @@ int shci_select(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t 
   ctx->sel_n = 0;
   DevBuf bufs[10];
   const int rc = select_sweep(ctx, up, dn, n_dets, coefs, n_states, eps_var, eps_tried_prev, n_new, bufs);
+  for (DevBuf& b : bufs) release(ctx, b);
+  return rc;
+}
+
+int shci_pt_dtm(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n_dets, const double* coefs,
+                double eps_pt_dtm, double energy_var, uint32_t n_batches, double* energy_pt, uint64_t* n_pt_dets) {
+  if (!ctx) return SHCI_ERR_ARG;
+  if (!up || !dn || !coefs || !energy_pt || !n_pt_dets || n_dets == 0 || !(eps_pt_dtm > 0.0))
+    return fail(ctx, SHCI_ERR_ARG, "PT: empty or null arguments, or eps_pt_dtm <= 0");
+  if (!ctx->have_ints) return fail(ctx, SHCI_ERR_STATE, "PT: no integrals");
+  if (n_dets >= 0xFFFFFFFFull) return fail(ctx, SHCI_ERR_UNSUPPORTED, "PT: n_dets >= 2^32 - 1");
+  HIPC(hipSetDevice(ctx->device));
+  *energy_pt = 0.0;
+  *n_pt_dets = 0;
+  DevBuf bufs[15];
+  const int rc = pt_dtm_sweep(ctx, up, dn, n_dets, coefs, eps_pt_dtm, energy_var, n_batches, energy_pt, n_pt_dets,
+                              bufs);
   for (DevBuf& b : bufs) release(ctx, b);
   return rc;
 }

@@ -1,10 +1,12 @@
-"""Heat-bath determinant selection on the device.
+"""Heat-bath determinant selection and the deterministic PT2 correction on the
+device.
 
 ``Selector`` is one sweep of the variational loop (src/solver/solver.h:435-495
 with ChemSystem::find_connected_dets, src/chem/chem_system.cc:289-402) through
 ``shci_select``; ``GpuSystem`` keeps the det list and eps_tried_prev with the
 interface of ``shci_amd.host.HostSystem`` (``select``, ``dets``, ``reset``,
-``n_dets``), so the same variational loop runs with either."""
+``n_dets``), so the same variational loop runs with either.  ``pt_dtm`` is
+the deterministic PT2 correction (src/solver/solver.h:637-764)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -91,3 +93,26 @@ class GpuSystem:
         self._dn = np.concatenate([self._dn, nd])
         self._eps = np.concatenate([self._eps, np.full(len(nu), 1.0e100)])
         return len(nu)
+
+
+def pt_dtm(ham: Hamiltonian, up, dn, coefs, eps_pt_dtm, energy_var, n_batches=0):
+    """Deterministic PT2 correction of one state over the integrals of
+    ``ham`` (Solver::get_energy_pt_dtm, src/solver/solver.h:637-764): returns
+    (E_pt2, number of PT dets, device ms).  ``coefs``: that state's
+    coefficients; ``energy_var``: its variational energy."""
+    import time
+
+    up = np.ascontiguousarray(up, dtype=np.uint64)
+    dn = np.ascontiguousarray(dn, dtype=np.uint64)
+    c = np.ascontiguousarray(coefs, dtype=np.float64).ravel()
+    if len(c) != len(up) or len(dn) != len(up):
+        raise ShciError("coefs / dets length mismatch")
+    ham.ensure_integrals()
+    ctx = ham.ctx
+    e = C.c_double()
+    m = C.c_uint64()
+    t0 = time.perf_counter()
+    ctx.check(ctx.lib.shci_pt_dtm(ctx.handle, up.ctypes.data_as(_u64p), dn.ctypes.data_as(_u64p), len(up),
+                                  c.ctypes.data_as(_dp), float(eps_pt_dtm), float(energy_var), int(n_batches),
+                                  C.byref(e), C.byref(m)), "shci_pt_dtm")
+    return e.value, m.value, (time.perf_counter() - t0) * 1e3
