@@ -96,6 +96,43 @@ def selection_bench(w, ham, coefs):
                      "(PCIe included); phases are HIP-event times on the library's stream")
 
 
+def pt2_bench(w, ham, coefs, energies, sample_dets):
+    """Deterministic PT2 correction (shci_pt_dtm) of the ground state over the
+    whole det list at eps_pt_dtm = 0.1 eps_var (the reference's ratio,
+    solver.h:545-553, without its 2e-6 floor, which lies above this space's
+    eps_var and would leave no PT det); then a parity sample: the first
+    ``sample_dets`` dets and their coefficients through the device and the
+    oracle's brute-force restatement, both timed."""
+    import oracle
+    from shci_amd.selection import pt_dtm
+
+    spec = w.spec
+    eps_var = float(w.log[-1]["eps"])
+    eps_pt = 0.1 * eps_var
+    c = np.atleast_2d(coefs)[0]
+    ev = float(energies[0])
+    e, npt, ms = pt_dtm(ham, w.up, w.dn, c, eps_pt, ev)
+    m = min(sample_dets, len(w.up))
+    es, ns, ms_s = pt_dtm(ham, w.up[:m], w.dn[:m], c[:m], eps_pt, ev)
+    orc = oracle.System.from_arrays(w.n, spec.n_up, spec.n_dn, w.h1, w.eri, w.core)
+    t0 = time.time()
+    eo, no = orc.pt_dtm(w.up[:m], w.dn[:m], c[:m], eps_pt, ev)
+    t_o = time.time() - t0
+    same = ns == no and abs(es - eo) <= 1e-12 * abs(eo)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"[bench] PT2 dtm at eps_pt {eps_pt:.1e} over {len(w.up):,} dets: {npt:,} PT dets, E_pt2 {e:.10f} in "
+        f"{ms:.1f} ms; sample of {m:,} dets: device {ms_s:.1f} ms, oracle brute force {t_o * 1e3:.0f} ms on "
+        f"{threads} threads, {ns:,} PT dets, match={same}")
+    if not same:
+        raise SystemExit("PT2: device and oracle differ on the sample")
+    return dict(eps_var=eps_var, eps_pt_dtm=eps_pt, n_dets=len(w.up), n_pt_dets=int(npt), energy_pt2=e,
+                device_ms=ms, sample=dict(n_dets=m, n_pt_dets=int(ns), device_ms=ms_s, oracle_ms=t_o * 1e3,
+                                          oracle_threads=threads, match=bool(same),
+                                          note="the oracle enumerates every single and double of each det "
+                                               "(no queues): a parity check, not a tuned CPU code"),
+                note="device_ms is wall time of shci_pt_dtm, host arrays in (PCIe included)")
+
+
 def cpu_baseline(w, sample_dets, reps_spmv=2):
     """The oracle (C++/OpenMP restatement of the reference) on the first
     ``sample_dets`` dets of the same list: build + Davidson to the same
@@ -188,6 +225,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-selection", action="store_true",
                     help="skip the selection sweep (device vs the host's queue walk)")
+    ap.add_argument("--no-pt2", action="store_true", help="skip the deterministic PT2 record")
+    ap.add_argument("--pt2-sample-dets", type=int, default=3000,
+                    help="dets of the PT2 parity sample run by the oracle's brute force")
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--layout", choices=["segments", "csr"], default="segments")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -304,6 +344,7 @@ def main():
         f"SpMV32 {spmv32_ms:.3f} ms = {spmv32_bytes / spmv32_ms / 1e6:.0f} GB/s")
     ham.clear()
     selection = None if args.no_selection else selection_bench(w, ham, d.get_lowest_eigenvectors())
+    pt2 = None if args.no_pt2 else pt2_bench(w, ham, d.get_lowest_eigenvectors(), e_ref_all, args.pt2_sample_dets)
 
     # ---------------- CPU baseline ----------------
     cpu = None
@@ -365,6 +406,8 @@ def main():
     }
     if selection is not None:
         out["selection"] = selection
+    if pt2 is not None:
+        out["pt2_dtm"] = pt2
     if ns > 1:
         out["n_states"] = ns
         out["energies_ref_fp64"] = e_ref_all.tolist()
