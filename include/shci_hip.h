@@ -55,9 +55,19 @@ enum {
                                     (beta-2) cores, each connected pair stored once as a
                                     value slot of its owning segment, no column indices;
                                     faster SpMV, less memory; no row export */
-  SHCI_F64_VECTORS = 1u << 6   /* shci_time_spmv: FP64 vectors with the store SHCI_FP32
+  SHCI_F64_VECTORS = 1u << 6,  /* shci_time_spmv: FP64 vectors with the store SHCI_FP32
                                   selects (FP32 values and FP64 accumulation: the
                                   mixed Davidson's shift and energy re-evaluation) */
+  SHCI_TIME_SYM = 1u << 7      /* build: time-reversal symmetry (ChemSystem time_sym,
+                                  base_system.h:86-122).  Each det (host pointers, up <= dn,
+                                  n_up == n_dn, the reference's canonical form, solver.h:452)
+                                  stands for the state (|up,dn> + |dn,up>)/sqrt2, or |up,up>.
+                                  Every later call works in that basis: vectors and
+                                  coefficients of n_dets entries, H_ts elements of
+                                  base_system.h:86-101 (products, diagonal, Davidson);
+                                  shci_rdm unpacks like unpack_time_sym.  The matrix is built
+                                  over the dets and their reverses; shci_get_counts reports
+                                  that matrix's entries; shci_export_upper is unsupported. */
 };
 
 /* Davidson modes. */

@@ -30,6 +30,9 @@ Davidson::diagonalize                  DavidsonTest.HilbertSystem known answer
                                        examples/C per-iteration "Davidson #k" trajectory
 RDM 1/2 (src/chem/rdm.cc)              trace / partial-trace identities; energy identity
                                        rdm.cc:731-745 vs the eigenvalue; finite differences
+time_sym element                       tests/test_oracle_time_sym.py: equals P^T H P of the
+(src/base_system.h:86-101)             full space's dense H (first principles); its spectrum
+                                       is the spin-flip-even part of H's
 selection sweep                        tests/test_oracle_selection.py: examples/C per-
 (src/solver/solver.h:435-495,          iteration det counts of the reference's log with
 src/chem/chem_system.cc:115-402)       this selection driving the loop; hand-built
@@ -96,6 +99,7 @@ def _load():
     lib.orc_select.argtypes = [C.c_void_p, _u64p, _u64p, C.c_size_t, _dp, C.c_uint, C.c_double, _dp,
                                C.POINTER(C.c_size_t)]
     lib.orc_select_result.argtypes = [C.c_void_p, _u64p, _u64p]
+    lib.orc_elem_time_sym.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, _dp]
     lib.orc_pt_dtm.argtypes = [C.c_void_p, _u64p, _u64p, C.c_size_t, _dp, C.c_double, C.c_double, _dp,
                                C.POINTER(C.c_size_t)]
     _lib = lib
@@ -248,6 +252,12 @@ class System:
         _check(_load().orc_rdm(self._h, coefs.shape[0], _dptr(coefs), _dptr(r1),
                                _dptr(r2) if want_2rdm else C.cast(None, _dp)))
         return r1, r2
+
+    def elem_time_sym(self, up_i, dn_i, up_j, dn_j):
+        """H_ts between time-reversal-symmetrised states (base_system.h:86-101)."""
+        out = C.c_double()
+        _check(_load().orc_elem_time_sym(self._h, up_i, dn_i, up_j, dn_j, C.byref(out)))
+        return out.value
 
     def select(self, up, dn, coefs, eps_var, eps_tried_prev):
         """One selection sweep over the dets (up, dn): the new dets, (up, dn)

@@ -375,6 +375,23 @@ struct Chem {
     }
     return e;
   }
+  // base_system.h:86-101: the element between time-reversal-symmetrised
+  // states (dets with up != dn stand for (|up,dn> + |dn,up>) / sqrt2).
+  double elem_time_sym(const Det& di, const Det& dj, int n_excite) const {
+    constexpr double SQRT2 = 1.4142135623730951;  // src/util.h:58
+    double h = elem(di, dj, n_excite);
+    if (di.up == di.dn) {
+      if (dj.up != dj.dn) h *= SQRT2;
+    } else {
+      if (dj.up == dj.dn) {
+        h *= SQRT2;
+      } else {
+        const Det di_rev{di.dn, di.up};
+        h += elem(di_rev, dj, -1);
+      }
+    }
+    return h;
+  }
   // chem_system.cc:409-442
   double elem(const Det& di, const Det& dj, int n_excite) const {
     DiffResult du, dd;
@@ -1439,6 +1456,12 @@ int orc_elem(void* h, uint64_t up_i, uint64_t dn_i, uint64_t up_j, uint64_t dn_j
 
 // Append-only determinant update + Hamiltonian::update (reference semantics:
 // the first n_prev dets must equal the previous call's list).
+int orc_elem_time_sym(void* h, uint64_t up_i, uint64_t dn_i, uint64_t up_j, uint64_t dn_j, double* out) {
+  auto* s = static_cast<OrcSystem*>(h);
+  *out = s->chem.elem_time_sym(Det{up_i, dn_i}, Det{up_j, dn_j}, -1);
+  return 0;
+}
+
 int orc_ham_update(void* h, const uint64_t* up, const uint64_t* dn, size_t n_dets) {
   auto* s = static_cast<OrcSystem*>(h);
   ORC_TRY({

@@ -18,6 +18,7 @@ SHCI_DEVICE_PTRS = 1 << 3
 SHCI_FP32 = 1 << 4
 SHCI_LAYOUT_SEGMENTS = 1 << 5
 SHCI_F64_VECTORS = 1 << 6
+SHCI_TIME_SYM = 1 << 7
 SHCI_DAVIDSON_FP64 = 0
 SHCI_DAVIDSON_MIXED = 1
 

@@ -119,6 +119,15 @@ struct shci_ctx {
   DevBuf tile_tab;                      // tile det tables (internal det ids)
   DevBuf seg_loc;                       // membership -> index in its tile's table (m <= 64 only)
   DevBuf kept;                          // kept-pair counters of the segment fill
+  // Time-reversal symmetry (SHCI_TIME_SYM): the matrix is built over the
+  // expanded list E (each det, then its reverse when up != dn); calls take and
+  // return vectors in the reduced basis (one entry per det).
+  bool ts = false;
+  uint64_t n_red = 0;                   // dets of the reduced basis
+  std::vector<uint64_t> ts_e;           // reduced det -> position in E (caller order)
+  std::vector<uint8_t> ts_pair;         // 1: up != dn (E holds the reverse next)
+  DevBuf ts_rev;                        // internal E index -> internal index of its reverse
+  DevBuf ts_diag;                       // reduced diagonal of H_ts (caller order)
   // Heat-bath selection: the singles and double-excitation queues of the
   // current integrals (rebuilt when the integrals or the electron count
   // change) and the new determinants of the last sweep.
@@ -956,6 +965,41 @@ __global__ void k_to_caller(const TI* __restrict__ xi, const uint32_t* __restric
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) xc[perm[i]] = (TO)xi[i];
 }
+// Time-reversal symmetry: y <- (y + R y) / 2 on the expanded list (R swaps
+// each det with its reverse; rev[i] = i for up == dn).  The pair's smaller
+// index writes both.
+template <typename T>
+__global__ void k_ts_sym(T* __restrict__ y, const uint32_t* __restrict__ rev, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t r = rev[i];
+  if (r > i) {
+    const T m = (y[i] + y[r]) * (T)0.5;
+    y[i] = m;
+    y[r] = m;
+  }
+}
+
+// Diagonal of H_ts = P^T H P over the reduced basis (caller order):
+// (H_dd + H_rr) / 2 + H_dr for a det with up != dn (r its reverse; H_dr is
+// a mixed double when up and dn differ in one orbital, else 0), H_dd else.
+// base_system.h:86-101 gives H_dd + H_rd: the same number up to rounding.
+__global__ void k_ts_diag(Ints I, const uint64_t* __restrict__ up_c, const uint64_t* __restrict__ dn_c,
+                          const uint64_t* __restrict__ e, const uint8_t* __restrict__ pair, uint64_t n,
+                          double* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t u = up_c[e[i]], d = dn_c[e[i]];
+  const double hdd = diag_elem(I, u, d);
+  if (!pair[i]) {
+    out[i] = hdd;
+    return;
+  }
+  const double hrr = diag_elem(I, d, u);
+  const double hdr = __popcll(u ^ d) == 2 ? double_mixed(I, u, d, d, u) : 0.0;
+  out[i] = 0.5 * (hdd + hrr) + hdr;
+}
+
 // interleave k vectors (row-major k x n, caller order) into internal x[i*K+v].
 __global__ void k_interleave(const double* __restrict__ X, const uint32_t* __restrict__ perm, float* __restrict__ xi,
                              uint32_t K, uint32_t Kp, uint64_t n) {
@@ -3535,10 +3579,28 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
   return SHCI_OK;
 }
 
+template <typename VT, typename XT, typename AT>
+int launch_csr_spmv(shci_ctx* ctx, const VT* val, double shift, const XT* x, XT* y);
+
 // SpMV launch on internal-order device vectors.
 template <typename VT, typename XT, typename AT>
 int launch_spmv(shci_ctx* ctx, const VT* val, double shift, const XT* x, XT* y) {
-  if (ctx->segl) return launch_seg_spmv<1, VT, XT>(ctx, shift, x, y);
+  if (ctx->segl) {
+    CHECK_RC(launch_seg_spmv<1, VT, XT>(ctx, shift, x, y));
+  } else {
+    CHECK_RC(launch_csr_spmv<VT, XT, AT>(ctx, val, shift, x, y));
+  }
+  if (ctx->ts) {
+    // keep the products in the symmetric subspace (the Davidson's vectors
+    // start there; the projection stops rounding from drifting out of it)
+    k_ts_sym<XT><<<div_up(ctx->n, 256), 256, 0, ctx->stream>>>(y, P<uint32_t>(ctx->ts_rev), ctx->n);
+    HIPC(hipGetLastError());
+  }
+  return SHCI_OK;
+}
+
+template <typename VT, typename XT, typename AT>
+int launch_csr_spmv(shci_ctx* ctx, const VT* val, double shift, const XT* x, XT* y) {
   const uint64_t n = ctx->n;
   const int L = ctx->lanes;
   const uint32_t grid = div_up(n, 256 / L);
@@ -4702,7 +4764,7 @@ int shci_ctx_destroy(shci_ctx* ctx) {
                     &ctx->row_ptr, &ctx->row_end, &ctx->col, &ctx->v64, &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb,
                     &ctx->va, &ctx->vb, &ctx->scal, &ctx->part, &ctx->xv, &ctx->yv, &ctx->x32, &ctx->y32,
                     &ctx->dav, &ctx->sq_S, &ctx->sq_r, &ctx->sq_len, &ctx->sq_max, &ctx->hq_H, &ctx->hq_rs, &ctx->hq_len, &ctx->hq_max,
-                    &ctx->sel_up, &ctx->sel_dn};
+                    &ctx->sel_up, &ctx->sel_dn, &ctx->ts_rev, &ctx->ts_diag};
   for (DevBuf* b : bufs) release(ctx, *b);
   cache_trim(ctx);
   for (auto& e : ctx->ev)
@@ -4747,8 +4809,8 @@ int shci_set_integrals(shci_ctx* ctx, uint32_t n_orb, const double* h1, const do
   return SHCI_OK;
 }
 
-int shci_build_hamiltonian(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n_dets,
-                           uint64_t n_dets_prev, uint32_t flags) {
+static int build_full(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n_dets, uint64_t n_dets_prev,
+                      uint32_t flags) {
   if (!ctx) return SHCI_ERR_ARG;
   if (!up || !dn || n_dets == 0) return fail(ctx, SHCI_ERR_ARG, "empty determinant list");
   if (n_dets >= (1ull << 32)) return fail(ctx, SHCI_ERR_UNSUPPORTED, "n_dets >= 2^32");
@@ -4799,10 +4861,18 @@ int shci_rebuild_hamiltonian(shci_ctx* ctx, uint32_t flags) {
   return do_build(ctx, flags & ~SHCI_DEVICE_PTRS);
 }
 
+static int update_full(shci_ctx* ctx);
+static int ts_diag_update(shci_ctx* ctx);
+
 int shci_update_existing_elems(shci_ctx* ctx) {
   if (!ctx) return SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
   HIPC(hipSetDevice(ctx->device));
+  CHECK_RC(update_full(ctx));
+  return ctx->ts ? ts_diag_update(ctx) : SHCI_OK;
+}
+
+static int update_full(shci_ctx* ctx) {
   if (ctx->segl) {
     k_diag<<<div_up(ctx->n, 256), 256, 0, ctx->stream>>>(make_ints(ctx), P<uint64_t>(ctx->up_s), P<uint64_t>(ctx->dn_s),
                                                         P<double>(ctx->diag), ctx->n);
@@ -4823,7 +4893,7 @@ int shci_get_counts(shci_ctx* ctx, uint64_t* n_dets, uint64_t* n_stored_upper, u
   if (!ctx) return SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
   const uint64_t up = ctx->nnz / 2 + ctx->n;
-  if (n_dets) *n_dets = ctx->n;
+  if (n_dets) *n_dets = ctx->ts ? ctx->n_red : ctx->n;
   if (n_stored_upper) *n_stored_upper = up;
   if (n_elems) *n_elems = 2 * up - ctx->n;
   return SHCI_OK;
@@ -4833,6 +4903,10 @@ int shci_get_diag(shci_ctx* ctx, double* diag) {
   if (!ctx || !diag) return SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
   HIPC(hipSetDevice(ctx->device));
+  if (ctx->ts) {
+    HIPC(hipMemcpy(diag, ctx->ts_diag.p, ctx->n_red * 8, hipMemcpyDeviceToHost));
+    return SHCI_OK;
+  }
   CHECK_RC(ensure(ctx, ctx->xv, ctx->n * 8));
   k_to_caller<double, double><<<div_up(ctx->n, 256), 256, 0, ctx->stream>>>(P<double>(ctx->diag), P<uint32_t>(ctx->perm),
                                                                             P<double>(ctx->xv), ctx->n);
@@ -4841,7 +4915,7 @@ int shci_get_diag(shci_ctx* ctx, double* diag) {
   return SHCI_OK;
 }
 
-int shci_spmv(shci_ctx* ctx, const double* x, double* y, uint32_t flags) {
+static int spmv_full(shci_ctx* ctx, const double* x, double* y, uint32_t flags) {
   if (!ctx || !x || !y) return SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
   HIPC(hipSetDevice(ctx->device));
@@ -4884,6 +4958,11 @@ int shci_spmm(shci_ctx* ctx, const double* X, double* Y, uint32_t k, uint32_t fl
   if (!ctx || !X || !Y || k == 0 || k > 8) return SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
   HIPC(hipSetDevice(ctx->device));
+  if (ctx->ts) {  // reduced basis: one product per vector
+    for (uint32_t v = 0; v < k; v++)
+      CHECK_RC(shci_spmv(ctx, X + (size_t)v * ctx->n_red, Y + (size_t)v * ctx->n_red, flags));
+    return SHCI_OK;
+  }
   const uint64_t n = ctx->n;
   const hipStream_t st = ctx->stream;
   // The segment layout's block kernels stage the 4 vectors of x of a
@@ -4938,9 +5017,9 @@ int shci_spmm(shci_ctx* ctx, const double* X, double* Y, uint32_t k, uint32_t fl
   return SHCI_OK;
 }
 
-int shci_davidson(shci_ctx* ctx, uint32_t n_states, const double* init, double tol, uint32_t mode,
-                  uint32_t max_calls, double* evals, double* evecs, int* converged, double* traj_e, double* traj_t,
-                  int traj_cap, int* n_traj) {
+static int davidson_full(shci_ctx* ctx, uint32_t n_states, const double* init, double tol, uint32_t mode,
+                         uint32_t max_calls, double* evals, double* evecs, int* converged, double* traj_e,
+                         double* traj_t, int traj_cap, int* n_traj) {
   if (!ctx || !init || !evals || n_states == 0 || n_states > 4) return ctx ? fail(ctx, SHCI_ERR_ARG, "bad davidson args") : SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
   const bool mixed = mode == SHCI_DAVIDSON_MIXED;
@@ -5077,7 +5156,7 @@ int shci_davidson(shci_ctx* ctx, uint32_t n_states, const double* init, double t
   return SHCI_OK;
 }
 
-int shci_rdm(shci_ctx* ctx, uint32_t n_states, const double* coefs, double* rdm1, double* rdm2) {
+static int rdm_full(shci_ctx* ctx, uint32_t n_states, const double* coefs, double* rdm1, double* rdm2) {
   if (!ctx || !coefs || n_states == 0) return SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
   HIPC(hipSetDevice(ctx->device));
@@ -5143,6 +5222,7 @@ int shci_export_upper(shci_ctx* ctx, int64_t* row_ptr, int64_t* cols, double* va
   if (!ctx || !row_ptr) return SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
   if (ctx->segl) return fail(ctx, SHCI_ERR_STATE, "row export needs the CSR layout (build without SHCI_LAYOUT_SEGMENTS)");
+  if (ctx->ts) return fail(ctx, SHCI_ERR_UNSUPPORTED, "row export of a time_sym build (its rows are those of the expanded list)");
   HIPC(hipSetDevice(ctx->device));
   const uint64_t n = ctx->n;
   const uint64_t m = ctx->nnz / 2 + n;
@@ -5259,6 +5339,10 @@ int shci_clear(shci_ctx* ctx) {
   for (auto& c : ctx->seg_maxm) c = 0;
   ctx->segl = false;
   ctx->built = false;
+  ctx->ts = false;
+  ctx->n_red = 0;
+  ctx->ts_e.clear();
+  ctx->ts_pair.clear();
   return SHCI_OK;
 }
 
@@ -5415,6 +5499,149 @@ int shci_segment_ranges(shci_ctx* ctx, uint64_t* out) {
     out[3 * r + 2] = voff[r + 1] - voff[r];
   }
   return SHCI_OK;
+}
+
+// ---- time-reversal symmetry: reduced basis <-> expanded list (host) ----
+// P maps reduced det i to (e_d + e_rev) / sqrt2 (up != dn) or e_d; expand is
+// x_E = P x, contract y = P^T y_E (base_system.h:108-122 unpacks coefficients
+// the same way, c / sqrt2 on both).
+constexpr double kSqrt1_2 = 0.7071067811865475;  // Util::SQRT2_INV (src/util.h:60), as written there
+
+static void ts_expand(const shci_ctx* ctx, const double* x, double* xe) {
+  for (uint64_t i = 0; i < ctx->n_red; i++) {
+    const uint64_t e = ctx->ts_e[i];
+    if (ctx->ts_pair[i]) {
+      xe[e] = xe[e + 1] = x[i] * kSqrt1_2;
+    } else {
+      xe[e] = x[i];
+    }
+  }
+}
+
+static void ts_contract(const shci_ctx* ctx, const double* ye, double* y) {
+  for (uint64_t i = 0; i < ctx->n_red; i++) {
+    const uint64_t e = ctx->ts_e[i];
+    y[i] = ctx->ts_pair[i] ? (ye[e] + ye[e + 1]) * kSqrt1_2 : ye[e];
+  }
+}
+
+// The reduced diagonal of H_ts from the current integrals (k_ts_diag).
+static int ts_diag_update(shci_ctx* ctx) {
+  const uint64_t n = ctx->n_red;
+  DevBuf de, dp;
+  int rc = SHCI_OK;
+  if ((rc = ensure(ctx, de, n * 8)) || (rc = ensure(ctx, dp, n)) || (rc = ensure(ctx, ctx->ts_diag, n * 8))) {
+    release(ctx, de);
+    release(ctx, dp);
+    return rc;
+  }
+  if (hipMemcpy(de.p, ctx->ts_e.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dp.p, ctx->ts_pair.data(), n, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(ctx, SHCI_ERR_HIP, "time_sym maps");
+  if (rc == SHCI_OK) {
+    k_ts_diag<<<div_up(n, 256), 256, 0, ctx->stream>>>(make_ints(ctx), P<uint64_t>(ctx->up_c), P<uint64_t>(ctx->dn_c),
+                                                      P<uint64_t>(de), P<uint8_t>(dp), n, P<double>(ctx->ts_diag));
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
+      rc = fail(ctx, SHCI_ERR_HIP, "time_sym diagonal");
+  }
+  release(ctx, de);
+  release(ctx, dp);
+  return rc;
+}
+
+static int build_ts(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n, uint64_t n_prev,
+                    uint32_t flags) {
+  if (flags & SHCI_DEVICE_PTRS) return fail(ctx, SHCI_ERR_UNSUPPORTED, "time_sym: host determinant arrays only");
+  if (!up || !dn || n == 0) return fail(ctx, SHCI_ERR_ARG, "empty determinant list");
+  if (__builtin_popcountll(up[0]) != __builtin_popcountll(dn[0]))
+    return fail(ctx, SHCI_ERR_ARG, "time_sym needs n_up == n_dn");
+  for (uint64_t i = 0; i < n; i++)
+    if (up[i] > dn[i]) return fail(ctx, SHCI_ERR_ARG, "time_sym: every det needs up <= dn (the canonical form)");
+  if (ctx->ts && ctx->built && n == n_prev && n == ctx->n_red) return SHCI_OK;  // hamiltonian.h:91
+  if (!ctx->ts) n_prev = 0;
+  std::vector<uint64_t> eu, ed, e(n);
+  std::vector<uint8_t> pair(n);
+  eu.reserve(2 * n);
+  ed.reserve(2 * n);
+  for (uint64_t i = 0; i < n; i++) {
+    e[i] = eu.size();
+    eu.push_back(up[i]);
+    ed.push_back(dn[i]);
+    pair[i] = up[i] != dn[i];
+    if (pair[i]) {
+      eu.push_back(dn[i]);
+      ed.push_back(up[i]);
+    }
+  }
+  const uint64_t nE = eu.size();
+  const uint64_t nE_prev = n_prev == 0 ? 0 : (n_prev < n ? e[n_prev] : nE);
+  ctx->ts = false;
+  CHECK_RC(build_full(ctx, eu.data(), ed.data(), nE, nE_prev, flags));
+  // internal E index -> internal index of its reverse
+  std::vector<uint32_t> perm(nE), iperm(nE), partner(nE), rev(nE);
+  HIPC(hipMemcpy(perm.data(), ctx->perm.p, nE * 4, hipMemcpyDeviceToHost));
+  for (uint64_t k = 0; k < nE; k++) iperm[perm[k]] = (uint32_t)k;
+  for (uint64_t i = 0; i < n; i++) {
+    partner[e[i]] = (uint32_t)(pair[i] ? e[i] + 1 : e[i]);
+    if (pair[i]) partner[e[i] + 1] = (uint32_t)e[i];
+  }
+  for (uint64_t k = 0; k < nE; k++) rev[k] = iperm[partner[perm[k]]];
+  CHECK_RC(ensure(ctx, ctx->ts_rev, nE * 4));
+  HIPC(hipMemcpy(ctx->ts_rev.p, rev.data(), nE * 4, hipMemcpyHostToDevice));
+  ctx->n_red = n;
+  ctx->ts_e = std::move(e);
+  ctx->ts_pair = std::move(pair);
+  CHECK_RC(ts_diag_update(ctx));
+  ctx->ts = true;
+  return SHCI_OK;
+}
+
+int shci_build_hamiltonian(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n_dets,
+                           uint64_t n_dets_prev, uint32_t flags) {
+  if (!ctx) return SHCI_ERR_ARG;
+  if (flags & SHCI_TIME_SYM) return build_ts(ctx, up, dn, n_dets, n_dets_prev, flags & ~SHCI_TIME_SYM);
+  if (ctx->ts) {  // a plain build after a symmetric one starts afresh
+    ctx->ts = false;
+    n_dets_prev = 0;
+  }
+  return build_full(ctx, up, dn, n_dets, n_dets_prev, flags);
+}
+
+int shci_spmv(shci_ctx* ctx, const double* x, double* y, uint32_t flags) {
+  if (!ctx || !x || !y) return SHCI_ERR_ARG;
+  if (!ctx->ts) return spmv_full(ctx, x, y, flags);
+  if (flags & SHCI_DEVICE_PTRS) return fail(ctx, SHCI_ERR_UNSUPPORTED, "time_sym: host vectors only");
+  std::vector<double> xe(ctx->n), ye(ctx->n);
+  ts_expand(ctx, x, xe.data());
+  CHECK_RC(spmv_full(ctx, xe.data(), ye.data(), flags));
+  ts_contract(ctx, ye.data(), y);
+  return SHCI_OK;
+}
+
+int shci_davidson(shci_ctx* ctx, uint32_t n_states, const double* init, double tol, uint32_t mode,
+                  uint32_t max_calls, double* evals, double* evecs, int* converged, double* traj_e, double* traj_t,
+                  int traj_cap, int* n_traj) {
+  if (!ctx || !init || !evals) return ctx ? fail(ctx, SHCI_ERR_ARG, "bad davidson args") : SHCI_ERR_ARG;
+  if (!ctx->ts || !ctx->built)
+    return davidson_full(ctx, n_states, init, tol, mode, max_calls, evals, evecs, converged, traj_e, traj_t, traj_cap,
+                         n_traj);
+  const uint64_t n = ctx->n_red, nE = ctx->n;
+  std::vector<double> ie((size_t)n_states * nE), ee(evecs ? (size_t)n_states * nE : 0);
+  for (uint32_t s = 0; s < n_states; s++) ts_expand(ctx, init + (size_t)s * n, ie.data() + (size_t)s * nE);
+  CHECK_RC(davidson_full(ctx, n_states, ie.data(), tol, mode, max_calls, evals, evecs ? ee.data() : nullptr, converged,
+                         traj_e, traj_t, traj_cap, n_traj));
+  if (evecs)
+    for (uint32_t s = 0; s < n_states; s++) ts_contract(ctx, ee.data() + (size_t)s * nE, evecs + (size_t)s * n);
+  return SHCI_OK;
+}
+
+int shci_rdm(shci_ctx* ctx, uint32_t n_states, const double* coefs, double* rdm1, double* rdm2) {
+  if (!ctx || !coefs || n_states == 0) return SHCI_ERR_ARG;
+  if (!ctx->ts || !ctx->built) return rdm_full(ctx, n_states, coefs, rdm1, rdm2);
+  const uint64_t n = ctx->n_red, nE = ctx->n;
+  std::vector<double> ce((size_t)n_states * nE);  // unpack_time_sym, base_system.h:108-122
+  for (uint32_t s = 0; s < n_states; s++) ts_expand(ctx, coefs + (size_t)s * n, ce.data() + (size_t)s * nE);
+  return rdm_full(ctx, n_states, ce.data(), rdm1, rdm2);
 }
 
 int shci_memory(shci_ctx* ctx, uint64_t* bytes) {

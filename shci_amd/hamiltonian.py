@@ -205,9 +205,12 @@ class Hamiltonian:
     precision(s) kept: ``"fp64"``, ``"fp32"`` or ``"both"``.  ``layout``:
     ``"csr"`` (full-symmetric CSR; supports row export) or ``"segments"``
     (dets grouped by shared cores, each connected pair stored once as a value
-    slot without column index; faster SpMV, less memory)."""
+    slot without column index; faster SpMV, less memory).  ``time_sym``: the
+    dets (up <= dn) stand for time-reversal-symmetrised states and every
+    vector is in that basis (ChemSystem time_sym, base_system.h:86-122)."""
 
-    def __init__(self, n_orb, h1, eri, core=0.0, device=0, store="fp64", keep_zeros=False, layout="csr"):
+    def __init__(self, n_orb, h1, eri, core=0.0, device=0, store="fp64", keep_zeros=False, layout="csr",
+                 time_sym=False):
         self.ctx = Context.get(device)
         self.matrix = SparseMatrix(self.ctx)
         self.n_dets = 0
@@ -217,6 +220,7 @@ class Hamiltonian:
         if layout not in ("csr", "segments"):
             raise ShciError(f"unknown layout {layout!r}")
         self.layout = layout
+        self.time_sym = bool(time_sym)
         self.set_integrals(n_orb, h1, eri, core)
 
     def set_integrals(self, n_orb, h1, eri, core=0.0):
@@ -245,6 +249,8 @@ class Hamiltonian:
             f |= _abi.SHCI_KEEP_ZEROS
         if self.layout == "segments":
             f |= _abi.SHCI_LAYOUT_SEGMENTS
+        if self.time_sym:
+            f |= _abi.SHCI_TIME_SYM
         return f
 
     def update(self, up, dn):

@@ -1,0 +1,110 @@
+"""GPU time-reversal symmetry (SHCI_TIME_SYM) against the oracle's H_ts
+(base_system.h:86-101, pinned against P^T H P in test_oracle_time_sym.py) on
+selected canonical det lists: products (1e-12), the diagonal (1e-12), the
+lowest eigenvalue (FP64 1e-9, mixed 1 uHa), the RDMs of the unpacked
+wavefunction (base_system.h:108-122) against the oracle's (1e-12), append-only
+updates, and the canonical-form check."""
+import numpy as np
+import pytest
+
+import oracle
+from shci_amd import integrals as si
+
+
+def _ts_dets(n, ne, h1, eri, eps_list):
+    """Host selection, then the canonical form (up <= dn), first occurrence kept."""
+    from shci_amd.host import HostSystem
+
+    host = HostSystem.from_arrays(n, ne, ne, h1, eri, 0.0)
+    coefs = np.ones((1, 1))
+    for eps in eps_list:
+        host.select(coefs, eps)
+        m = host.n_dets
+        coefs = np.concatenate([[1.0], np.full(m - 1, 1e-3)])[None]
+    up, dn = host.dets()
+    seen, out = set(), []
+    for u, d in zip(up.tolist(), dn.tolist()):
+        c = (u, d) if u <= d else (d, u)
+        if c not in seen:
+            seen.add(c)
+            out.append(c)
+    return np.array([c[0] for c in out], dtype=np.uint64), np.array([c[1] for c in out], dtype=np.uint64)
+
+
+def _dense_ts(orc, up, dn):
+    m = len(up)
+    H = np.zeros((m, m))
+    for i in range(m):
+        for j in range(m):
+            H[i, j] = orc.elem_time_sym(int(up[i]), int(dn[i]), int(up[j]), int(dn[j]))
+    return H
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("layout", ["csr", "segments"])
+@pytest.mark.parametrize("n,ne,seed,eps", [(12, 3, 7, [1e-2, 3e-3]), (14, 4, 11, [2e-2, 8e-3])])
+def test_gpu_time_sym_vs_oracle(layout, n, ne, seed, eps):
+    from shci_amd.hamiltonian import Davidson, Hamiltonian
+
+    _, h1, eri, core = si.synthetic(n, seed=seed)
+    up, dn = _ts_dets(n, ne, h1, eri, eps)
+    m = len(up)
+    orc = oracle.System.from_arrays(n, ne, ne, h1, eri, core)
+    H = _dense_ts(orc, up, dn)
+    print(f"[ts] n={n} {ne}+{ne} {layout}: {m} symmetrised dets, {int(np.sum(up != dn))} pairs", flush=True)
+    ham = Hamiltonian(n, h1, eri, core, store="both", layout=layout, time_sym=True)
+    ham.update(up, dn)
+    assert ham.matrix.counts()[0] == m
+    np.testing.assert_allclose(ham.matrix.get_diag(), np.diag(H), rtol=0, atol=1e-12)
+    x = np.random.default_rng(seed).uniform(-1, 1, m)
+    np.testing.assert_allclose(ham.matrix.mul(x), H @ x, rtol=0, atol=1e-11)
+    w = np.linalg.eigvalsh(H)
+    init = np.zeros(m)
+    init[0] = 1.0
+    d = Davidson(1)
+    d.diagonalize(ham.matrix, init, 1e-11, max_calls=40)
+    assert abs(d.lowest_eigenvalues[0] - w[0]) < 1e-9, (d.lowest_eigenvalues[0], w[0])
+    c = np.atleast_2d(d.get_lowest_eigenvectors())[0]
+    # the reduced eigenvector: unit norm, Rayleigh quotient at the eigenvalue
+    # (second order in the vector error), small residual (first order; the
+    # Davidson stops on the eigenvalue change)
+    assert abs(c @ c - 1.0) < 1e-8
+    assert abs(c @ H @ c - w[0]) < 1e-9
+    assert np.linalg.norm(H @ c - w[0] * c) < 1e-3
+    dm = Davidson(1)
+    dm.diagonalize(ham.matrix, init, 1e-10, mode="mixed", max_calls=40)
+    assert abs(dm.lowest_eigenvalues[0] - w[0]) < 1e-6
+    # RDMs of the unpacked wavefunction (each pair's coefficient / sqrt2 on
+    # both dets, the reference's SQRT2_INV)
+    r1, r2 = ham.rdm(c[None])
+    eu, ed, ec = [], [], []
+    for u, dd, ci in zip(up.tolist(), dn.tolist(), c.tolist()):
+        if u == dd:
+            eu.append(u); ed.append(dd); ec.append(ci)
+        else:
+            eu += [u, dd]; ed += [dd, u]; ec += [ci * 0.7071067811865475] * 2
+    orc.update(np.array(eu, dtype=np.uint64), np.array(ed, dtype=np.uint64))
+    o1, o2 = orc.rdm(np.array(ec)[None])
+    np.testing.assert_allclose(r1, o1, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(r2, o2, rtol=0, atol=1e-12)
+
+
+@pytest.mark.gpu
+def test_gpu_time_sym_append_and_checks():
+    from shci_amd.hamiltonian import Hamiltonian, ShciError
+
+    n, ne = 12, 3
+    _, h1, eri, core = si.synthetic(n, seed=7)
+    up, dn = _ts_dets(n, ne, h1, eri, [1e-2, 3e-3])
+    m = len(up)
+    ham = Hamiltonian(n, h1, eri, core, store="fp64", layout="segments", time_sym=True)
+    ham.update(up[: m // 2], dn[: m // 2])
+    ham.update(up, dn)  # append-only growth (hamiltonian.h:87-112)
+    fresh = Hamiltonian(n, h1, eri, core, store="fp64", layout="csr", time_sym=True)
+    x = np.random.default_rng(1).uniform(-1, 1, m)
+    y1 = ham.matrix.mul(x)
+    fresh.update(up, dn)
+    np.testing.assert_allclose(y1, fresh.matrix.mul(x), rtol=0, atol=1e-11)
+    bad = Hamiltonian(n, h1, eri, core, store="fp64", time_sym=True)
+    with pytest.raises(ShciError):
+        bad.update(np.array([0b1110], dtype=np.uint64), np.array([0b0111], dtype=np.uint64))  # up > dn
