@@ -148,11 +148,13 @@ class Hamiltonian {
     n_dets_prev_ = n_dets_;
     n_dets_ = T::n_dets(system);
     if (n_dets_prev_ == n_dets_) return;  // hamiltonian.h:91
-    if (T::time_sym(system)) throw std::runtime_error("time_sym is not supported by the GPU path");
     upload_integrals(system);
     std::vector<uint64_t> up, dn;
     T::dets(system, up, dn);
-    check(matrix.context(), shci_build_hamiltonian(matrix.context(), up.data(), dn.data(), n_dets_, n_dets_prev_, flags_),
+    // time_sym: the dets are the reference's canonical symmetrised states
+    // (up <= dn); vectors stay in that basis (base_system.h:86-122)
+    const uint32_t f = flags_ | (T::time_sym(system) ? SHCI_TIME_SYM : 0u);
+    check(matrix.context(), shci_build_hamiltonian(matrix.context(), up.data(), dn.data(), n_dets_, n_dets_prev_, f),
           "shci_build_hamiltonian");
     matrix.mark_built();
   }
