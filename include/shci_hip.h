@@ -166,9 +166,12 @@ int shci_export_upper(shci_ctx* ctx, int64_t* row_ptr, int64_t* cols, double* va
  * eps_min <= |H| < eps_tried_prev, not in the variational space.  *n_new
  * receives the number of distinct new dets; shci_select_result copies them
  * out, (up, dn) ascending (the reference appends them in the order of its
- * hash set). */
+ * hash set).  flags: SHCI_TIME_SYM applies the reference's time_sym rules
+ * (solver.h:447-455): the dets are canonical (up <= dn), eps_min is multiplied
+ * by sqrt2 for a det with up != dn, and candidates are tested against the
+ * space and returned in their canonical form. */
 int shci_select(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n_dets, const double* coefs,
-                uint32_t n_states, double eps_var, double* eps_tried_prev, uint64_t* n_new);
+                uint32_t n_states, double eps_var, double* eps_tried_prev, uint64_t* n_new, uint32_t flags);
 /* The last sweep's new dets: n must equal its *n_new. */
 int shci_select_result(shci_ctx* ctx, uint64_t* new_up, uint64_t* new_dn, uint64_t n);
 /* Deterministic PT2 correction of one state (Solver::get_energy_pt_dtm,

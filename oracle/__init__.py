@@ -97,7 +97,7 @@ def _load():
                                     C.c_int, C.POINTER(C.c_int)]
     lib.orc_rdm.argtypes = [C.c_void_p, C.c_uint, _dp, _dp, _dp]
     lib.orc_select.argtypes = [C.c_void_p, _u64p, _u64p, C.c_size_t, _dp, C.c_uint, C.c_double, _dp,
-                               C.POINTER(C.c_size_t)]
+                               C.POINTER(C.c_size_t), C.c_int]
     lib.orc_select_result.argtypes = [C.c_void_p, _u64p, _u64p]
     lib.orc_elem_time_sym.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, _dp]
     lib.orc_pt_dtm.argtypes = [C.c_void_p, _u64p, _u64p, C.c_size_t, _dp, C.c_double, C.c_double, _dp,
@@ -259,7 +259,7 @@ class System:
         _check(_load().orc_elem_time_sym(self._h, up_i, dn_i, up_j, dn_j, C.byref(out)))
         return out.value
 
-    def select(self, up, dn, coefs, eps_var, eps_tried_prev):
+    def select(self, up, dn, coefs, eps_var, eps_tried_prev, time_sym=False):
         """One selection sweep over the dets (up, dn): the new dets, (up, dn)
         ascending; ``eps_tried_prev`` (float64, one per det) updated in place."""
         up = np.ascontiguousarray(up, dtype=np.uint64)
@@ -268,7 +268,8 @@ class System:
         assert eps_tried_prev.dtype == np.float64 and eps_tried_prev.flags.c_contiguous
         m = C.c_size_t()
         _check(_load().orc_select(self._h, up.ctypes.data_as(_u64p), dn.ctypes.data_as(_u64p), len(up),
-                                  _dptr(coefs), coefs.shape[0], eps_var, _dptr(eps_tried_prev), C.byref(m)))
+                                  _dptr(coefs), coefs.shape[0], eps_var, _dptr(eps_tried_prev), C.byref(m),
+                                  int(time_sym)))
         nu = np.zeros(m.value, dtype=np.uint64)
         nd = np.zeros(m.value, dtype=np.uint64)
         _check(_load().orc_select_result(self._h, nu.ctypes.data_as(_u64p), nd.ctypes.data_as(_u64p)))

@@ -1175,6 +1175,9 @@ static thread_local std::string g_err;
 //     value (chem_system.cc:115-196,261-287): same spin |(pr|qs) - (ps|qr)|,
 //     opposite spin |(pr|qs)| (p, r up; q, s down);
 //   never a det of the variational space.
+// time_sym (solver.h:447-455): eps_min times sqrt2 for a det with up != dn,
+// candidates tested and kept in their canonical form (up <= dn), the single's
+// element taken with the candidate as found.
 // ---------------------------------------------------------------------------
 static double selection_S(const Integrals& I, unsigned p, unsigned r, unsigned n_elecs) {
   const unsigned n = I.n_orbs;
@@ -1194,7 +1197,8 @@ static double selection_S(const Integrals& I, unsigned p, unsigned r, unsigned n
 }
 
 static std::vector<Det> select_sweep(const Integrals& I, const Chem& chem, const std::vector<Det>& dets,
-                                     const double* coefs, unsigned n_states, double eps_var, double* eps_tried) {
+                                     const double* coefs, unsigned n_states, double eps_var, double* eps_tried,
+                                     bool time_sym) {
   const unsigned n = I.n_orbs;
   const size_t nd = dets.size();
   const unsigned ne = (unsigned)(__builtin_popcountll(dets[0].up) + __builtin_popcountll(dets[0].dn));
@@ -1211,13 +1215,16 @@ static std::vector<Det> select_sweep(const Integrals& I, const Chem& chem, const
     double mc = coefs[i];
     for (unsigned s = 1; s < n_states; s++)
       if (std::abs(coefs[s * nd + i]) > std::abs(mc)) mc = coefs[s * nd + i];
-    const double emin = eps_var / std::abs(mc), emax = eps_tried[i];
-    if (emin >= emax) continue;
     const Det& d = dets[i];
+    double emin = eps_var / std::abs(mc);
+    if (time_sym && d.up != d.dn) emin *= 1.4142135623730951;  // Util::SQRT2
+    const double emax = eps_tried[i];
+    if (emin >= emax) continue;
     auto keep = [&](const Det& c, bool single) {
-      if (var.count(c)) return;
+      const Det reg = (time_sym && c.up > c.dn) ? Det{c.dn, c.up} : c;
+      if (var.count(reg)) return;
       if (single && std::abs(chem.elem(d, c, 1)) < emin) return;
-      out.push_back(c);
+      out.push_back(reg);
     };
     for (int spin = 0; spin < 2; spin++) {
       const uint64_t str = spin ? d.dn : d.up;
@@ -1619,12 +1626,12 @@ int orc_sm_davidson(void* mh, unsigned n_states, const double* init, double tol,
 // 1-RDM (n x n) and 2-RDM (packed, n^2(n^2+1)/2) of coefs (n_states x n_dets)
 // over the system's det list and stored Hamiltonian connections.
 int orc_select(void* h, const uint64_t* up, const uint64_t* dn, size_t n_dets, const double* coefs,
-               unsigned n_states, double eps_var, double* eps_tried, size_t* n_new) {
+               unsigned n_states, double eps_var, double* eps_tried, size_t* n_new, int time_sym) {
   try {
     auto* s = static_cast<OrcSystem*>(h);
     std::vector<Det> dets(n_dets);
     for (size_t i = 0; i < n_dets; i++) dets[i] = Det{up[i], dn[i]};
-    s->selected = select_sweep(s->ints, s->chem, dets, coefs, n_states, eps_var, eps_tried);
+    s->selected = select_sweep(s->ints, s->chem, dets, coefs, n_states, eps_var, eps_tried, time_sym != 0);
     *n_new = s->selected.size();
     return 0;
   } catch (const std::exception& e) {

@@ -40,7 +40,7 @@ EXPORTS = {
                                 C.c_int, _ip]),
     "shci_rdm": (C.c_int, [_vp, C.c_uint32, _dp, _dp, _dp]),
     "shci_export_upper": (C.c_int, [_vp, _i64p, _i64p, _dp]),
-    "shci_select": (C.c_int, [_vp, _u64p, _u64p, C.c_uint64, _dp, C.c_uint32, C.c_double, _dp, _u64p]),
+    "shci_select": (C.c_int, [_vp, _u64p, _u64p, C.c_uint64, _dp, C.c_uint32, C.c_double, _dp, _u64p, C.c_uint32]),
     "shci_select_result": (C.c_int, [_vp, _u64p, _u64p, C.c_uint64]),
     "shci_select_timings": (C.c_int, [_vp, _fp]),
     "shci_pt_dtm": (C.c_int, [_vp, _u64p, _u64p, C.c_uint64, _dp, C.c_double, C.c_double, C.c_uint32, _dp,

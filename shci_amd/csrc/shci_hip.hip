@@ -4113,6 +4113,7 @@ struct SelParams {
   uint64_t cap;                      // append pass: capacity of out
   double* __restrict__ outv;         // PT fill pass: H_ai c_i of each written candidate
   uint32_t n_batches, batch;         // PT: only candidates whose hash falls in this batch
+  uint32_t ts;                       // time_sym rules of the selection (solver.h:447-454)
   uint64_t i0, i1;                   // det range
 };
 
@@ -4169,7 +4170,8 @@ __global__ __launch_bounds__(64) void k_sel_walk(SelParams S) {
       const double c = S.coefs[(uint64_t)s * S.n + i];
       if (fabs(c) > fabs(mc)) mc = c;
     }
-    const double emin = S.eps_var / fabs(mc);
+    double emin = S.eps_var / fabs(mc);
+    if (S.ts && u != d) emin *= 1.4142135623730951;  // Util::SQRT2 (solver.h:447)
     const double emax = PT ? __builtin_inf() : S.eps_tried[i];  // PT: no upper cut (solver.h:639-642)
     uint64_t cur = MODE == kSelFill ? S.off[i] - S.base : 0;
     const uint64_t lim = MODE == kSelFill ? S.off[i + 1] - S.base : 0;  // the det's range from the count pass
@@ -4272,21 +4274,28 @@ __global__ __launch_bounds__(64) void k_sel_walk(SelParams S) {
               if (s < n) cu |= 1ull << s; else cd |= 1ull << (s - n);
             }
           }
+          // time_sym: the candidate is tested and kept in its canonical form
+          // (up <= dn, solver.h:452-455); the single's element uses it as found
+          const uint64_t fu = cu, fd = cd;
+          if (S.ts && cu > cd) {
+            cu = fd;
+            cd = fu;
+          }
           if (ok) ok = !sel_in_var(S, cu, cd);
           double hc = 0.0;
           if (!PT) {
             if (ok && bk == 1) {
-              const double h = bp < n ? single_elem(S.I, u, cu, d) : single_elem(S.I, d, cd, u);
+              const double h = bp < n ? single_elem(S.I, u, fu, d) : single_elem(S.I, d, fd, u);
               ok = fabs(h) >= emin;
             }
           } else if (ok) {
             // PT: H_ai c_i with the build's elements (det i on the left), kept
             // when |H_ai c_i| >= eps_pt_dtm (solver.h:710-717)
             double h;
-            if (bk == 1) h = bp < n ? single_elem(S.I, u, cu, d) : single_elem(S.I, d, cd, u);
-            else if (bq < n) h = double_same(S.I, u, cu);
-            else if (bp >= n) h = double_same(S.I, d, cd);
-            else h = double_mixed(S.I, u, cu, d, cd);
+            if (bk == 1) h = bp < n ? single_elem(S.I, u, fu, d) : single_elem(S.I, d, fd, u);
+            else if (bq < n) h = double_same(S.I, u, fu);
+            else if (bp >= n) h = double_same(S.I, d, fd);
+            else h = double_mixed(S.I, u, fu, d, fd);
             hc = h * mc;
             ok = fabs(hc) >= S.eps_var;
             if (ok && S.n_batches > 1) ok = (uint32_t)((det_hash(cu, cd) >> 20) % S.n_batches) == S.batch;
@@ -4461,19 +4470,21 @@ int sel_prepare(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t 
   S.total = reinterpret_cast<unsigned long long*>(P<uint64_t>(ctx->scal) + 1);
   S.n_batches = 1;
   S.batch = 0;
+  S.ts = 0;
   S.i0 = 0;
   S.i1 = n;
   return SHCI_OK;
 }
 
 int select_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n, const double* coefs,
-                 uint32_t n_states, double eps_var, double* eps_tried, uint64_t* n_new, DevBuf* bufs) {
+                 uint32_t n_states, double eps_var, double* eps_tried, uint64_t* n_new, bool ts, DevBuf* bufs) {
   DevBuf &dE = bufs[3], &cnt = bufs[5], &off = bufs[6], &cand = bufs[7], &srt = bufs[8], &acc = bufs[9];
   const hipStream_t st = ctx->stream;
   const uint32_t n_orb = ctx->n_orb;
   SelParams S;
   CHECK_RC(sel_prepare(ctx, up, dn, n, coefs, n_states, eps_tried, bufs, S));
   S.eps_var = eps_var;
+  S.ts = ts ? 1u : 0u;
   uint32_t bad = 0;
   // Candidate budget: ~64 B each (keys, sorted keys, flags, offsets, sort
   // storage) within 3/4 of the free memory (plus the context's cached blocks).
@@ -5268,7 +5279,7 @@ int shci_export_upper(shci_ctx* ctx, int64_t* row_ptr, int64_t* cols, double* va
 }
 
 int shci_select(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t n_dets, const double* coefs,
-                uint32_t n_states, double eps_var, double* eps_tried_prev, uint64_t* n_new) {
+                uint32_t n_states, double eps_var, double* eps_tried_prev, uint64_t* n_new, uint32_t flags) {
   if (!ctx) return SHCI_ERR_ARG;
   if (!up || !dn || !coefs || !eps_tried_prev || !n_new || n_dets == 0 || n_states == 0)
     return fail(ctx, SHCI_ERR_ARG, "selection: empty or null arguments");
@@ -5278,7 +5289,10 @@ int shci_select(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t 
   *n_new = 0;
   ctx->sel_n = 0;
   DevBuf bufs[10];
-  const int rc = select_sweep(ctx, up, dn, n_dets, coefs, n_states, eps_var, eps_tried_prev, n_new, bufs);
+  if ((flags & SHCI_TIME_SYM) && __builtin_popcountll(up[0]) != __builtin_popcountll(dn[0]))
+    return fail(ctx, SHCI_ERR_ARG, "selection: time_sym needs n_up == n_dn");
+  const int rc = select_sweep(ctx, up, dn, n_dets, coefs, n_states, eps_var, eps_tried_prev, n_new,
+                              (flags & SHCI_TIME_SYM) != 0, bufs);
   for (DevBuf& b : bufs) release(ctx, b);
   return rc;
 }

@@ -219,7 +219,7 @@ class Selector {
     uint64_t m = 0;
     check(ctx_,
           shci_select(ctx_, up.data(), dn.data(), n, c.data(), (uint32_t)coefs.size(), eps_var,
-                      eps_tried_prev.data(), &m),
+                      eps_tried_prev.data(), &m, T::time_sym(system) ? SHCI_TIME_SYM : 0u),
           "shci_select");
     new_up.resize(m);
     new_dn.resize(m);

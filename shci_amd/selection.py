@@ -13,6 +13,7 @@ import ctypes as C
 
 import numpy as np
 
+from . import _abi
 from .hamiltonian import Hamiltonian, ShciError
 
 _u64p = C.POINTER(C.c_uint64)
@@ -20,11 +21,15 @@ _dp = C.POINTER(C.c_double)
 
 
 class Selector:
-    """Selection over the integrals of ``ham`` (its device context)."""
+    """Selection over the integrals of ``ham`` (its device context).
+    ``time_sym`` (default: ``ham.time_sym``) applies the reference's time_sym
+    rules: canonical dets (up <= dn), sqrt2 on eps_min of a det with up != dn,
+    canonical candidates (solver.h:447-455)."""
 
-    def __init__(self, ham: Hamiltonian):
+    def __init__(self, ham: Hamiltonian, time_sym=None):
         self.ham = ham
         self.ctx = ham.ctx
+        self.time_sym = getattr(ham, "time_sym", False) if time_sym is None else bool(time_sym)
         self.last_ms = 0.0
         self.last_phases_ms = {}
 
@@ -47,7 +52,8 @@ class Selector:
         t0 = time.perf_counter()
         self.ctx.check(lib.shci_select(self.ctx.handle, up.ctypes.data_as(_u64p), dn.ctypes.data_as(_u64p), len(up),
                                        coefs.ctypes.data_as(_dp), coefs.shape[0], float(eps_var),
-                                       eps_tried_prev.ctypes.data_as(_dp), C.byref(n_new)), "shci_select")
+                                       eps_tried_prev.ctypes.data_as(_dp), C.byref(n_new),
+                                       _abi.SHCI_TIME_SYM if self.time_sym else 0), "shci_select")
         m = n_new.value
         nu = np.zeros(m, dtype=np.uint64)
         nd = np.zeros(m, dtype=np.uint64)
@@ -65,7 +71,7 @@ class GpuSystem:
     ``select(coefs, eps_var)``, ``dets()``, ``reset()`` and ``n_dets``."""
 
     def __init__(self, ham: Hamiltonian, hf_up: int, hf_dn: int):
-        self.selector = Selector(ham)
+        self.selector = Selector(ham)  # time_sym follows the Hamiltonian's
         self.hf_up, self.hf_dn = int(hf_up), int(hf_dn)
         self.reset()
 

@@ -108,3 +108,61 @@ def test_gpu_time_sym_append_and_checks():
     bad = Hamiltonian(n, h1, eri, core, store="fp64", time_sym=True)
     with pytest.raises(ShciError):
         bad.update(np.array([0b1110], dtype=np.uint64), np.array([0b0111], dtype=np.uint64))  # up > dn
+
+
+@pytest.mark.gpu
+def test_gpu_select_time_sym_vs_oracle():
+    from shci_amd.hamiltonian import Hamiltonian
+    from shci_amd.selection import Selector
+
+    n, ne = 14, 4
+    _, h1, eri, core = si.synthetic(n, seed=11)
+    up, dn = _ts_dets(n, ne, h1, eri, [2e-2, 5e-3])
+    rng = np.random.default_rng(5)
+    c = rng.choice([-1.0, 1.0], (2, len(up))) * 10.0 ** rng.uniform(-5, 0, (2, len(up)))
+    c[0, 0] = 0.95
+    orc = oracle.System.from_arrays(n, ne, ne, h1, eri, core)
+    sel = Selector(Hamiltonian(n, h1, eri, core, time_sym=True))
+    eps_tried = np.full(len(up), 1e100)
+    for eps in (3e-3, 1e-3, 1e-3):
+        eg, eo = eps_tried.copy(), eps_tried.copy()
+        gu, gd = sel.select(up, dn, c, eps, eg)
+        ou, od = orc.select(up, dn, c, eps, eo, time_sym=True)
+        print(f"[ts] selection eps {eps}: {len(gu)} new", flush=True)
+        assert np.array_equal(gu, ou) and np.array_equal(gd, od) and np.array_equal(eg, eo)
+        assert np.all(gu <= gd)
+        eps_tried = eg
+        c = c * 2.0
+
+
+@pytest.mark.gpu
+def test_gpu_time_sym_variational_loop():
+    """The variational loop with time_sym throughout on the device
+    (selection rules, symmetrised build, Davidson): the final energy is the
+    lowest eigenvalue of the oracle's dense H_ts over the final dets."""
+    from helpers import variation
+    from shci_amd.hamiltonian import Davidson, Hamiltonian
+    from shci_amd.selection import GpuSystem
+
+    n, ne = 12, 3
+    _, h1, eri, core = si.synthetic(n, seed=7)
+    ham = Hamiltonian(n, h1, eri, core, store="fp64", layout="segments", time_sym=True)
+    hf = (1 << ne) - 1
+    gsys = GpuSystem(ham, hf, hf)
+    dav = Davidson(1)
+    last = {}
+
+    def backend(up, dn, init, tol):
+        ham.update(up, dn)
+        ev = dav.diagonalize(ham.matrix, init, tol)
+        last.update(up=up.copy(), dn=dn.copy(), e=float(ev[0]))
+        return ev, dav.get_lowest_eigenvectors(), dav.converged, {}
+
+    recs = list(variation(gsys, backend, [2e-3], [1e-2, 5e-3], target_error=1e-6))
+    up, dn = last["up"], last["dn"]
+    assert np.all(up <= dn) and len(set(zip(up.tolist(), dn.tolist()))) == len(up)
+    orc = oracle.System.from_arrays(n, ne, ne, h1, eri, core)
+    H = _dense_ts(orc, up, dn)
+    w0 = np.linalg.eigvalsh(H)[0]
+    print(f"[ts] loop: {len(recs)} iterations, {len(up)} dets, E {last['e']:.10f} (dense {w0:.10f})", flush=True)
+    assert abs(last["e"] - w0) < 1e-9

@@ -164,3 +164,27 @@ def test_selection_max_coefficient_over_states():
     a = orc.select([det[0]], [det[1]], c, eps, np.full(1, 1e100))
     b = orc.select([det[0]], [det[1]], np.array([[0.5]]), eps, np.full(1, 1e100))
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_selection_time_sym_rules():
+    """time_sym (solver.h:447-455) from the plain sweep: one canonical det at
+    a time, eps_min times sqrt2 when up != dn, candidates in canonical form,
+    minus the space."""
+    n, ne = 8, 2
+    _, h1, eri, core = si.synthetic(n, seed=23)
+    orc = oracle.System.from_arrays(n, ne, ne, h1, eri, core)
+    dets = [(0b0011, 0b0011), (0b0101, 0b0110), (0b0011, 0b1001), (0b1001, 0b1100)]
+    up = np.array([d[0] for d in dets], dtype=np.uint64)
+    dn = np.array([d[1] for d in dets], dtype=np.uint64)
+    c = np.array([[0.9, -0.3, 0.05, 0.01]])
+    eps = 2e-3
+    got = orc.select(up, dn, c, eps, np.full(len(dets), 1e100), time_sym=True)
+    got = set(zip(got[0].tolist(), got[1].tolist()))
+    want = set()
+    for k, (u, d) in enumerate(dets):
+        e = eps * (1.4142135623730951 if u != d else 1.0)
+        nu, nd = orc.select(up[k:k + 1], dn[k:k + 1], c[:, k:k + 1], e, np.full(1, 1e100))
+        for a, b in zip(nu.tolist(), nd.tolist()):
+            want.add((a, b) if a <= b else (b, a))
+    want -= set(dets)
+    assert got == want and len(got) > 0
