@@ -5083,6 +5083,22 @@ static int davidson_full(shci_ctx* ctx, uint32_t n_states, const double* init, d
   }
   std::vector<double> result(ns, 0.0);
   int conv = 0;
+  // internal index of the only nonzero entry of the ground-state start vector
+  // (n if there are several)
+  uint64_t single_k = n;
+  {
+    uint64_t nz = 0, k = 0;
+    for (uint64_t i = 0; i < n && nz < 2; i++)
+      if (init[i] != 0.0) {
+        nz++;
+        k = i;
+      }
+    if (nz == 1) {
+      uint32_t ki = 0;
+      HIPC(hipMemcpy(&ki, P<uint32_t>(ctx->iperm) + k, 4, hipMemcpyDeviceToHost));
+      single_k = ki;
+    }
+  }
   auto run_mode = [&](auto tag) -> int {
     using T = decltype(tag);
     Davidson<T> D;
@@ -5102,7 +5118,11 @@ static int davidson_full(shci_ctx* ctx, uint32_t n_states, const double* init, d
     for (size_t i = 0; i < ns; i++) D.hw.push_back(vb + (2 * K + ns + i) * nst);
     D.tmp = vb + (2 * K + 2 * ns) * nst;
     D.shift = 0.0;
-    if (mixed) {
+    if (mixed && single_k < n) {
+      // A start vector with one nonzero entry k: its Rayleigh quotient is H_kk.
+      HIPC(hipMemcpy(&D.shift, P<double>(ctx->diag) + single_k, 8, hipMemcpyDeviceToHost));
+      D.report_shift = D.shift;
+    } else if (mixed) {
       // Shift = Rayleigh quotient of the initial ground-state vector: keeps
       // FP32 values of H - shift small so FP32 row sums lose no absolute
       // accuracy.  It is subtracted on the FP64 diagonal and added back to
