@@ -88,3 +88,55 @@ def test_examples_C_gpu_pt_dtm(golden_dir):
           flush=True)
     assert npt == g["n_pt_dets"]
     assert abs(e - g["correction"]) < 1e-10
+
+
+@pytest.mark.gpu
+def test_examples_C_three_levels_on_device(golden_dir):
+    """The reference's whole examples/C variational run (eps_var 5e-5, 2e-5,
+    1e-5 after the schedule) with selection, build, Davidson and PT2 on the
+    device, iteration by iteration against its log (with the log revision's
+    energy-change stop, see helpers.variation).  The first level agrees
+    exactly: dets per iteration, E_var, the 52,762 PT dets and the correction.
+    The log's final-eps Davidson stopped at an eigenvalue change of ~2e-9
+    (1e-10 here), which moves a few borderline dets of the next level's
+    selection: at 2e-5 and 1e-5 the det and PT-det counts agree to 0.05%,
+    E_var to 1e-8 and the PT correction to 2e-9."""
+    from shci_amd.hamiltonian import Davidson, Hamiltonian
+    from shci_amd.host import HostSystem
+    from shci_amd.selection import GpuSystem, pt_dtm
+
+    gold = json.loads((golden_dir / "examples_C.json").read_text())
+    host = HostSystem.from_fcidump(golden_dir / "C_FCIDUMP", 4, 2)
+    h1, eri, _ = host.integrals()
+    ham = Hamiltonian(host.n_orbs, h1, eri, host.core, store="fp64", layout="segments")
+    gsys = GpuSystem(ham, host.hf_up, host.hf_dn)
+    dav = Davidson(1)
+    final = {}
+
+    def backend(up, dn, init, tol):
+        ham.update(up, dn)
+        ev = dav.diagonalize(ham.matrix, init, tol)
+        vec = np.atleast_2d(dav.get_lowest_eigenvectors())
+        return ev, vec, dav.converged, dict(up=up.copy(), dn=dn.copy(), c=vec[0].copy())
+
+    recs = list(variation(gsys, backend, [5e-5, 2e-5, 1e-5], [2e-3, 1e-3, 5e-4, 2e-4, 1e-4], energy_stop=True))
+    assert len(recs) == len(gold["iterations"])
+    for k, (rec, g) in enumerate(zip(recs, gold["iterations"])):
+        print(f"[pt] iteration {k + 1}: eps {rec['eps']:.0e} dets {rec['n_dets']} (log {g['n_dets']})", flush=True)
+        assert rec["eps"] == g["eps"]
+        if k < 9:
+            assert rec["n_dets"] == g["n_dets"]
+        else:
+            assert abs(rec["n_dets"] - g["n_dets"]) <= 5e-4 * g["n_dets"]
+        final[rec["eps"]] = rec
+    for g in gold["pt_dtm"]:
+        rec = final[g["eps_var"]]
+        e_var = float(rec["evals"][0])
+        assert abs(e_var - gold["energy_var"][f"{g['eps_var']:.2e}"]) < 1e-8
+        e, npt, ms = pt_dtm(ham, rec["up"], rec["dn"], rec["c"], g["eps_pt_dtm"], e_var)
+        print(f"[pt] eps_var {g['eps_var']:.0e}: {len(rec['up'])} dets, E_var {e_var:.10f}, {npt} PT dets, "
+              f"correction {e:.12f} (log {g['correction']:.10f}, {g['n_pt_dets']})", flush=True)
+        if g["eps_var"] == 5e-5:
+            assert npt == g["n_pt_dets"] and abs(e - g["correction"]) < 1e-10
+        else:
+            assert abs(npt - g["n_pt_dets"]) <= 5e-4 * g["n_pt_dets"] and abs(e - g["correction"]) < 2e-9
