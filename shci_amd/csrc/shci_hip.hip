@@ -1015,6 +1015,22 @@ __global__ void k_deinterleave(const float* __restrict__ yi, const uint32_t* __r
   const uint32_t c = perm[i];
   for (uint32_t v = 0; v < K; v++) Y[(size_t)v * n + c] = (double)yi[i * Kp + v];
 }
+// Up to 4 internal-order FP32 vectors <-> one interleaved block (x[i*4+v]),
+// for the Davidson's block products.
+struct Vec4 {
+  float* p[4];
+};
+__global__ void k_pack4(Vec4 src, uint32_t k, uint64_t n, float* __restrict__ xi) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (uint32_t v = 0; v < 4; v++) xi[i * 4 + v] = v < k ? src.p[v][i] : 0.f;
+}
+__global__ void k_unpack4(const float* __restrict__ yi, uint32_t k, uint64_t n, Vec4 dst) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (uint32_t v = 0; v < k; v++) dst.p[v][i] = yi[i * 4 + v];
+}
 
 // ---------------------------------------------------------------------------
 // RDM adjoint kernel (src/chem/rdm.cc).  Each unordered connected pair (and
@@ -3709,7 +3725,7 @@ void jacobi_eigh(int n, const std::vector<double>& a_in, std::vector<double>& w,
 // reported in the shifted frame.
 template <typename T>
 struct Davidson {
-  shci_ctx* ctx;
+  shci_ctx* ctx = nullptr;
   uint32_t ns;    // n_states
   size_t K;       // ns * n_store
   uint64_t n;
@@ -3726,6 +3742,15 @@ struct Davidson {
   double report_shift = 0.0;
 
   int spmv(const T* x, T* y);
+  // H v[k] -> hv[k] for the listed slots (one block product where possible).
+  int products(const std::vector<size_t>& slots);
+  DevBuf bx, by;  // interleaved block of the FP32 block product
+  ~Davidson() {
+    if (ctx) {
+      release(ctx, bx);
+      release(ctx, by);
+    }
+  }
 
   void record(double e) {
     if (traj_e && *n_traj < traj_cap) {
@@ -3745,8 +3770,12 @@ struct Davidson {
     lowest.assign(ns, 0.0);
     size_t n_converged = 0;
     converged = false;
+    {
+      std::vector<size_t> first(ns);
+      for (uint32_t s = 0; s < ns; s++) first[s] = s;
+      CHECK_RC(products(first));
+    }
     for (uint32_t s = 0; s < ns; s++) {
-      CHECK_RC(spmv(v[s], hv[s]));
       const T* b[1] = {hv[s]};
       double e;
       CHECK_RC(dots<T>(ctx, v[s], b, 1, &e));
@@ -3758,11 +3787,16 @@ struct Davidson {
     record(lowest[0] + report_shift);
     lowest_prev = lowest;
     size_t i_state_precond = 0;
+    // Products of a round's corrections wait for its Rayleigh-Ritz step: a
+    // correction depends on the last Ritz pairs only, and its product only
+    // enters that step (the same arithmetic as multiplying each at once).
+    std::vector<size_t> pending;
     for (size_t it = ns; it < n_store * ns * 2; it++) {
       const size_t ic = it % K;
       if (it >= K) {
         if (ic < ns - 1) continue;
         if (ic == ns - 1) {
+          pending.clear();  // the restart drops this round's corrections so far
           for (uint32_t s = 0; s < ns; s++) {
             HIPC(hipMemcpyAsync(v[s], w[s], n * sizeof(T), hipMemcpyDeviceToDevice, st));
             HIPC(hipMemcpyAsync(hv[s], hw[s], n * sizeof(T), hipMemcpyDeviceToDevice, st));
@@ -3799,14 +3833,16 @@ struct Davidson {
         break;
       }
       k_scale<T><<<g, 256, 0, st>>>(v[ic], 1.0 / norm, n);
-      CHECK_RC(spmv(v[ic], hv[ic]));
-      {
-        std::vector<const T*> b(v.begin(), v.begin() + ic + 1);
-        std::vector<double> d(ic + 1);
-        CHECK_RC(dots<T>(ctx, hv[ic], b.data(), (int)ic + 1, d.data()));
-        for (size_t i = 0; i <= ic; i++) H(ic, i) = d[i];
-      }
+      pending.push_back(ic);
       if (i_state_precond + 1 == ns) {
+        CHECK_RC(products(pending));
+        for (size_t pc : pending) {
+          std::vector<const T*> b(v.begin(), v.begin() + pc + 1);
+          std::vector<double> d(pc + 1);
+          CHECK_RC(dots<T>(ctx, hv[pc], b.data(), (int)pc + 1, d.data()));
+          for (size_t i = 0; i <= pc; i++) H(pc, i) = d[i];
+        }
+        pending.clear();
         const int ks = (int)ic + 1;
         std::vector<double> sub((size_t)ks * ks);
         for (int i = 0; i < ks; i++)
@@ -3849,6 +3885,40 @@ int Davidson<double>::spmv(const double* x, double* y) {
 template <>
 int Davidson<float>::spmv(const float* x, float* y) {
   return launch_spmv<float, float, float>(ctx, P<float>(ctx->v32), shift, x, y);
+}
+
+template <>
+int Davidson<double>::products(const std::vector<size_t>& slots) {
+  for (size_t k : slots) CHECK_RC(spmv(v[k], hv[k]));
+  return SHCI_OK;
+}
+template <>
+int Davidson<float>::products(const std::vector<size_t>& slots) {
+  // The block product (values read once for up to 4 vectors) when it applies:
+  // two or more vectors, no time_sym projection, segment x staging in LDS.
+  const uint32_t seg_big = *std::max_element(ctx->seg_maxm, ctx->seg_maxm + 9);
+  const bool block_ok = !ctx->ts && (ctx->segl ? (size_t)seg_big * 4 * sizeof(float) <= 65536 : true);
+  if (slots.size() < 2 || !block_ok) {
+    for (size_t k : slots) CHECK_RC(spmv(v[k], hv[k]));
+    return SHCI_OK;
+  }
+  const hipStream_t st = ctx->stream;
+  CHECK_RC(ensure(ctx, bx, 4 * n * sizeof(float)));
+  CHECK_RC(ensure(ctx, by, 4 * n * sizeof(float)));
+  for (size_t s0 = 0; s0 < slots.size(); s0 += 4) {
+    const uint32_t k = (uint32_t)std::min<size_t>(4, slots.size() - s0);
+    Vec4 src{}, dst{};
+    for (uint32_t j = 0; j < k; j++) {
+      src.p[j] = v[slots[s0 + j]];
+      dst.p[j] = hv[slots[s0 + j]];
+    }
+    k_pack4<<<div_up(n, 256), 256, 0, st>>>(src, k, n, P<float>(bx));
+    CHECK_RC(ctx->segl ? launch_seg_spmv<4, float, float>(ctx, shift, P<float>(bx), P<float>(by))
+                       : launch_spmm4(ctx, shift, P<float>(bx), P<float>(by)));
+    k_unpack4<<<div_up(n, 256), 256, 0, st>>>(P<float>(by), k, n, dst);
+    HIPC(hipGetLastError());
+  }
+  return SHCI_OK;
 }
 
 // Normalise + orthogonalise the initial vectors (davidson.cc:73-88) into v[0..ns).
