@@ -39,6 +39,12 @@ src/chem/chem_system.cc:115-402)       this selection driving the loop; hand-bui
                                        cases (a det whose only connections are known);
                                        the criteria's limits (eps -> 0 gives every det
                                        within two excitations; eps_tried_prev skips)
+wavefunction file (wf_format.py;       tests/test_wf_files.py: the published varint
+src/solver/solver.h:1179-1224,         examples; a file assembled byte by byte from the
+src/base_system.h:125-133)             field list.  No file written by the reference
+                                       exists in its checkout and its serializer (hps)
+                                       is absent: beyond hps's published encoding the
+                                       bytes are parity unpinned
 =====================================  ===================================================
 
 The reference itself cannot be compiled here (its submodules fgpl, hps, eigen,

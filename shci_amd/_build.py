@@ -22,7 +22,7 @@ HOST_LIB = PKG / "libshci_host.so"
 ORACLE_LIB = ROOT / "oracle" / "liboracle.so"
 
 HIP_SOURCES = ["shci_hip.hip"]
-HOST_SOURCES = ["chem_host.cc", "host_capi.cc", "variation.cc", "rdm_io.cc"]
+HOST_SOURCES = ["chem_host.cc", "host_capi.cc", "variation.cc", "rdm_io.cc", "wf_io.cc"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

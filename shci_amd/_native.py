@@ -60,6 +60,12 @@ def host_lib():
     lib.shci_host_dump_1rdm.argtypes = [C.c_char_p, C.c_uint, C.POINTER(C.c_uint), _dp]
     lib.shci_host_dump_2rdm.argtypes = [C.c_char_p, C.c_uint, C.POINTER(C.c_uint), _dp, C.c_int]
     lib.shci_host_io_last_error.restype = C.c_char_p
+    lib.shci_host_save_wf.argtypes = [C.c_char_p, C.c_uint, C.c_uint, _u64p, _u64p, C.c_size_t, _dp, C.c_uint,
+                                      C.c_double, _dp, C.c_int]
+    lib.shci_host_wf_info.argtypes = [C.c_char_p, C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_size_t),
+                                      C.POINTER(C.c_uint), C.POINTER(C.c_int)]
+    lib.shci_host_load_wf.argtypes = [C.c_char_p, _u64p, _u64p, C.c_size_t, _dp, C.c_uint, _dp, _dp]
+    lib.shci_host_wf_last_error.restype = C.c_char_p
     lib.shci_host_run_variation.argtypes = [_vp, _dp, C.c_uint, _dp, C.c_uint, C.c_double, C.c_uint, C.c_int,
                                             C.c_int, C.c_int, _dp]
     _host = lib

@@ -89,15 +89,18 @@ class HostSystem:
         _lib().shci_host_reset_dets(self._h)
 
     def run_variation(self, eps_vars, schedule, target_error=5e-5, n_states=1, mixed=False, gpu_select=False,
-                      device=0, verbose=False):
+                      device=0, verbose=False, wf_files=False, force_var=False):
         """The reference's variational loop in C++ (Solver::run_all_variations,
         solver.h:338-397) through the drop-in adapter: GPU build and Davidson,
-        selection by the host queue walk or on the device.  Returns the final
-        variational energies; the det list stays here (``dets()``)."""
+        selection by the host queue walk or on the device.  ``wf_files``: load
+        wf_eps1_<eps>.dat from the working directory instead of running an
+        eps_var whose file exists (unless ``force_var``), and save it after
+        running one (solver.h:352-375).  Returns the final variational
+        energies; the det list stays here (``dets()``)."""
         ev = np.ascontiguousarray(eps_vars, dtype=np.float64)
         sc = np.ascontiguousarray(schedule, dtype=np.float64)
         out = np.zeros(n_states)
-        mode = (1 if mixed else 0) | (2 if gpu_select else 0)
+        mode = (1 if mixed else 0) | (2 if gpu_select else 0) | (4 if wf_files else 0) | (8 if force_var else 0)
         rc = _lib().shci_host_run_variation(self._h, ev.ctypes.data_as(_dp), len(ev), sc.ctypes.data_as(_dp),
                                             len(sc), target_error, n_states, mode, device, int(verbose),
                                             out.ctypes.data_as(_dp))
@@ -145,3 +148,48 @@ def dump_2rdm(path, rdm2, n_orbs, orb_order=None, csv=True):
     keep, op = _orb_order_arg(n_orbs, orb_order)
     if _lib().shci_host_dump_2rdm(str(path).encode(), n_orbs, op, r2.ctypes.data_as(_dp), 1 if csv else 0) != 0:
         raise RuntimeError("shci_host_dump_2rdm: " + _lib().shci_host_io_last_error().decode())
+
+
+def wf_filename(eps_var) -> str:
+    """Solver::get_wf_filename (solver.h:1415-1417)."""
+    return "wf_eps1_%#.2e.dat" % eps_var
+
+
+def save_wf(path, n_up, n_dn, up, dn, coefs, energy_hf, energy_var, time_sym=False):
+    """Solver::save_variation_result (solver.h:1217-1224): the reference's
+    wavefunction file (BaseSystem::serialize, base_system.h:125-133) of dets
+    (up, dn) and coefs (n_states x n_dets)."""
+    up = np.ascontiguousarray(up, dtype=np.uint64)
+    dn = np.ascontiguousarray(dn, dtype=np.uint64)
+    c = np.ascontiguousarray(np.atleast_2d(coefs), dtype=np.float64)
+    ev = np.ascontiguousarray(np.atleast_1d(energy_var), dtype=np.float64)
+    if len(dn) != len(up) or c.shape[1] != len(up) or len(ev) != c.shape[0]:
+        raise ValueError("dets, coefs and energy_var disagree in size")
+    rc = _lib().shci_host_save_wf(str(path).encode(), n_up, n_dn, up.ctypes.data_as(_u64p), dn.ctypes.data_as(_u64p),
+                                  len(up), c.ctypes.data_as(_dp), c.shape[0], float(energy_hf),
+                                  ev.ctypes.data_as(_dp), 1 if time_sym else 0)
+    if rc != 0:
+        raise RuntimeError("shci_host_save_wf: " + _lib().shci_host_wf_last_error().decode())
+
+
+def load_wf(path) -> dict:
+    """Solver::load_variation_result (solver.h:1179-1215): the fields of a
+    wavefunction file as a dict (n_up, n_dn, up, dn, coefs, energy_hf,
+    energy_var, time_sym)."""
+    lib = _lib()
+    nu, nd, ns = C.c_uint(), C.c_uint(), C.c_uint()
+    n = C.c_size_t()
+    ts = C.c_int()
+    p = str(path).encode()
+    if lib.shci_host_wf_info(p, C.byref(nu), C.byref(nd), C.byref(n), C.byref(ns), C.byref(ts)) != 0:
+        raise RuntimeError("shci_host_wf_info: " + lib.shci_host_wf_last_error().decode())
+    up = np.zeros(n.value, dtype=np.uint64)
+    dn = np.zeros(n.value, dtype=np.uint64)
+    coefs = np.zeros((ns.value, n.value))
+    ev = np.zeros(ns.value)
+    ehf = C.c_double()
+    if lib.shci_host_load_wf(p, up.ctypes.data_as(_u64p), dn.ctypes.data_as(_u64p), n.value,
+                             coefs.ctypes.data_as(_dp), ns.value, C.byref(ehf), ev.ctypes.data_as(_dp)) != 0:
+        raise RuntimeError("shci_host_load_wf: " + lib.shci_host_wf_last_error().decode())
+    return dict(n_up=nu.value, n_dn=nd.value, up=up, dn=dn, coefs=coefs, energy_hf=ehf.value, energy_var=ev,
+                time_sym=bool(ts.value))

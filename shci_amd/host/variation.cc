@@ -2,6 +2,7 @@
 // heat-bath selection (host queue walk, or the device through the adapter)
 // feeding the GPU Hamiltonian build and Davidson through the drop-in adapter
 // (include/shci_gpu/shci_gpu.h).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -48,6 +49,55 @@ struct SystemTraits<shci_host::VarSystem> {
 
 using namespace shci_host;
 
+namespace {
+
+// Solver::load_variation_result (solver.h:1179-1215): false when the file
+// cannot be opened (the variation then runs); a file that opens but does not
+// parse, or holds another system's shape, is an error.
+bool load_wf(const char* name, HostSystem& s, unsigned n_states, std::vector<std::vector<double>>& coefs,
+             std::vector<double>& energy_var) {
+  if (FILE* f = std::fopen(name, "rb")) {
+    std::fclose(f);
+  } else {
+    return false;
+  }
+  unsigned n_up, n_dn, ns;
+  size_t n;
+  int ts;
+  if (shci_host_wf_info(name, &n_up, &n_dn, &n, &ns, &ts)) throw std::runtime_error(shci_host_wf_last_error());
+  if (n_up != s.n_up || n_dn != s.n_dn || ns != n_states || ts || n == 0)
+    throw std::runtime_error(std::string(name) + ": holds another system (electrons, states, time_sym or no dets)");
+  std::vector<uint64_t> up(n), dn(n);
+  std::vector<double> c((size_t)ns * n), ev(ns);
+  double e_hf;
+  if (shci_host_load_wf(name, up.data(), dn.data(), n, c.data(), ns, &e_hf, ev.data()))
+    throw std::runtime_error(shci_host_wf_last_error());
+  s.dets.resize(n);
+  for (size_t i = 0; i < n; i++) s.dets[i] = Det{up[i], dn[i]};
+  s.eps_tried.assign(n, 1.0e100);
+  coefs.assign(ns, std::vector<double>(n));
+  for (unsigned k = 0; k < ns; k++) std::copy(c.begin() + (size_t)k * n, c.begin() + (size_t)(k + 1) * n, coefs[k].begin());
+  energy_var = ev;
+  return true;
+}
+
+// Solver::save_variation_result (solver.h:1217-1224).
+void save_wf(const char* name, const HostSystem& s, const std::vector<std::vector<double>>& coefs,
+             const std::vector<double>& energy_var) {
+  const size_t n = s.dets.size();
+  std::vector<uint64_t> up(n), dn(n);
+  for (size_t i = 0; i < n; i++) up[i] = s.dets[i].up, dn[i] = s.dets[i].dn;
+  std::vector<double> c(coefs.size() * n);
+  for (const auto& ck : coefs)
+    if (ck.size() != n) throw std::runtime_error("save_wf: coefficients and dets differ in length");
+  for (size_t k = 0; k < coefs.size(); k++) std::copy(coefs[k].begin(), coefs[k].end(), c.begin() + k * n);
+  if (shci_host_save_wf(name, s.n_up, s.n_dn, up.data(), dn.data(), n, c.data(), (unsigned)coefs.size(),
+                        s.sel->hf_energy(), energy_var.data(), 0))
+    throw std::runtime_error(shci_host_wf_last_error());
+}
+
+}  // namespace
+
 extern "C" int shci_host_run_variation(void* h, const double* eps_vars, unsigned n_eps, const double* schedule,
                                        unsigned n_sched, double target_error, unsigned n_states, int mode,
                                        int device, int verbose, double* energies) {
@@ -55,6 +105,7 @@ extern "C" int shci_host_run_variation(void* h, const double* eps_vars, unsigned
   try {
     shci_ctx* ctx = shci_gpu::default_context(device);
     const bool mixed = (mode & 1) != 0, gpu_select = (mode & 2) != 0;
+    const bool wf_files = (mode & 4) != 0, force_var = (mode & 8) != 0;
     const uint32_t store = mixed ? (SHCI_STORE_FP32 | SHCI_STORE_FP64) : SHCI_STORE_FP64;
     shci_gpu::Hamiltonian<VarSystem> hamiltonian(ctx, store);
     shci_gpu::Selector<VarSystem> selector(ctx);
@@ -70,6 +121,15 @@ extern "C" int shci_host_run_variation(void* h, const double* eps_vars, unsigned
     size_t iteration_global = 0;
     for (unsigned e = 0; e < n_eps; e++) {
       const double eps_var = eps_vars[e];
+      char wf_name[64];
+      std::snprintf(wf_name, sizeof wf_name, "wf_eps1_%#.2e.dat", eps_var);
+      if (wf_files && !force_var && load_wf(wf_name, *s, n_states, coefs, energy_var)) {
+        // solver.h:374-377: the loaded list restarts the selection bookkeeping.
+        hamiltonian.clear();
+        if (verbose) printf("Loaded %zu dets from: %s\n", s->dets.size(), wf_name);
+        eps_prev = eps_var;
+        continue;
+      }
       std::vector<std::pair<double, bool>> plan;
       while (it_sched < n_sched && schedule[it_sched] >= eps_prev) it_sched++;
       while (it_sched < n_sched && schedule[it_sched] > eps_var) plan.push_back({schedule[it_sched++], false});
@@ -106,6 +166,7 @@ extern "C" int shci_host_run_variation(void* h, const double* eps_vars, unsigned
           if (!until) break;
         }
       }
+      if (wf_files) save_wf(wf_name, *s, coefs, energy_var);
       eps_prev = eps_var;
     }
     for (unsigned k = 0; k < n_states; k++) energies[k] = energy_var[k];
