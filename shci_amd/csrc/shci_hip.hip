@@ -2364,11 +2364,11 @@ __global__ __launch_bounds__(256) void k_seg_spmv_tiles(const uint32_t* __restri
 // slots and the unused half-step term is masked.
 constexpr int kSegChunks = 4;
 template <int NC, typename VT, typename XT>
-__device__ __forceinline__ void seg_large_rows(const SegDesc& D, uint32_t j0, const VT* __restrict__ val,
-                                               const XT* sx, const uint32_t* __restrict__ mem, XT* __restrict__ y) {
+__device__ __forceinline__ void seg_large_rows(const SegDesc& D, uint32_t j0, uint32_t d_lo, uint32_t d_hi,
+                                               bool half_step, const VT* __restrict__ val, const XT* sx,
+                                               const uint32_t* __restrict__ mem, XT* __restrict__ y) {
   constexpr uint32_t U = sizeof(XT) == 8 ? 1 : 4;  // steps per batch of loads (FP64: register budget)
   const uint32_t m = D.m, mh = m >> 1, t = threadIdx.x;
-  const uint32_t fs = (m & 1) ? mh : mh - 1;  // steps with m slots
   XT acc[NC];
   uint32_t jj[NC];
 #pragma unroll
@@ -2378,8 +2378,8 @@ __device__ __forceinline__ void seg_large_rows(const SegDesc& D, uint32_t j0, co
     acc[c] = (XT)0;
   }
   const VT* __restrict__ vs = val + D.voff;
-  uint32_t d = 1;
-  for (; d + U - 1 <= fs; d += U) {
+  uint32_t d = d_lo;
+  for (; d + U - 1 <= d_hi; d += U) {
     XT vf[U][NC], vb[U][NC];
 #pragma unroll
     for (uint32_t u = 0; u < U; u++) {
@@ -2399,7 +2399,7 @@ __device__ __forceinline__ void seg_large_rows(const SegDesc& D, uint32_t j0, co
         acc[c] += vf[u][c] * sx[j + du] + vb[u][c] * sx[j + m - du];
       }
   }
-  for (; d <= fs; d++) {
+  for (; d <= d_hi; d++) {
     const VT* __restrict__ vd = vs + (uint64_t)(d - 1) * m;
 #pragma unroll
     for (int c = 0; c < NC; c++) {
@@ -2408,7 +2408,7 @@ __device__ __forceinline__ void seg_large_rows(const SegDesc& D, uint32_t j0, co
       acc[c] += f * sx[j + d] + bk * sx[j + m - d];
     }
   }
-  if ((m & 1) == 0) {
+  if (half_step) {
     // last step of an even m: mh slots, the pair (i, i + mh) at slot i, read
     // by both of its members (forward term only)
     const VT* __restrict__ vd = vs + (uint64_t)(mh - 1) * m;
@@ -2583,6 +2583,13 @@ __global__ __launch_bounds__(64) void k_seg_spmv_large(const SegDesc* __restrict
   const uint32_t m = D.m, mh = m >> 1, t = threadIdx.x;
   const uint32_t j0 = blockIdx.y * (kWave * T);
   if (j0 >= m) return;
+  // blockIdx.z takes one share of the steps d = 1 .. fs (the last share also
+  // the half step of an even m); the shares' sums meet in the atomics on y.
+  const uint32_t fs = (m & 1) ? mh : mh - 1;  // steps with m slots
+  const uint32_t per = (fs + gridDim.z - 1) / gridDim.z;
+  const uint32_t d_lo = 1 + blockIdx.z * per, d_hi = min(fs, (blockIdx.z + 1) * per);
+  const bool half_step = (m & 1) == 0 && blockIdx.z + 1 == gridDim.z;
+  if (d_lo > d_hi && !half_step) return;
   XT* sx = reinterpret_cast<XT*>(seg_lds);
   for (uint32_t j = t; j < m; j += kWave) {
     const XT v = x[mem[D.moff + j]];
@@ -2593,10 +2600,10 @@ __global__ __launch_bounds__(64) void k_seg_spmv_large(const SegDesc* __restrict
   const uint32_t nc = min((uint32_t)T, (m - j0 + kWave - 1) / kWave);  // chunks holding members
   static_assert(T == 4, "chunk dispatch below");
   switch (nc) {
-    case 1: seg_large_rows<1>(D, j0, val, sx, mem, y); break;
-    case 2: seg_large_rows<2>(D, j0, val, sx, mem, y); break;
-    case 3: seg_large_rows<3>(D, j0, val, sx, mem, y); break;
-    default: seg_large_rows<4>(D, j0, val, sx, mem, y); break;
+    case 1: seg_large_rows<1>(D, j0, d_lo, d_hi, half_step, val, sx, mem, y); break;
+    case 2: seg_large_rows<2>(D, j0, d_lo, d_hi, half_step, val, sx, mem, y); break;
+    case 3: seg_large_rows<3>(D, j0, d_lo, d_hi, half_step, val, sx, mem, y); break;
+    default: seg_large_rows<4>(D, j0, d_lo, d_hi, half_step, val, sx, mem, y); break;
   }
 }
 
@@ -3399,8 +3406,13 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
       }
       const size_t lds = (size_t)ctx->seg_maxm[r] * 2 * sizeof(XT);
       if (lds > 65536) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment too large for the one-wavefront kernel");
-      // one wavefront per (segment, 256 members)
-      const dim3 grid(s1 - s0, div_up(ctx->seg_maxm[r], kWave * kSegChunks));
+      // one wavefront per (segment, 256 members, share of the steps): FP64
+      // reads its rows one step at a time, so a long segment is one long chain
+      // of loads, cut into shares of about 128 steps; FP32 reads 4 steps per
+      // batch and keeps one share (the split measured no gain there)
+      const uint32_t nz = sizeof(XT) == 8 ? std::min<uint32_t>(16, std::max<uint32_t>(1, div_up(ctx->seg_maxm[r] / 2, 128)))
+                                          : 1u;
+      const dim3 grid(s1 - s0, div_up(ctx->seg_maxm[r], kWave * kSegChunks), nz);
       k_seg_spmv_large<kSegChunks, VT, XT><<<grid, kWave, lds, st>>>(desc, s0, s1, mem, val, xr, yr);
     }
   }

@@ -210,6 +210,46 @@ def test_segments_counts_and_update_existing(system, keep_zeros):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n,ne,cls", [(34, 1, 0), (20, 2, 1)])
+def test_large_segments_vs_csr(n, ne, cls):
+    """Full spaces whose C segments exceed 1024 members (34 orbitals, 1+1
+    electrons: one segment of 1156) and 256 members (20 orbitals, 2+2: 361):
+    the one-wavefront kernel with each segment's steps split over several
+    workgroups.  FP64 and FP32 products and the FP64 Davidson energy against
+    the CSR layout."""
+    import itertools
+
+    from shci_amd.hamiltonian import Davidson, Hamiltonian
+
+    _, h1, eri, _ = si.synthetic(n, seed=31 + n)
+    strs = [sum(1 << k for k in c) for c in itertools.combinations(range(n), ne)]
+    hf = (1 << ne) - 1
+    dets = [(hf, hf)] + [(u, d) for u in strs for d in strs if (u, d) != (hf, hf)]
+    up = np.array([d[0] for d in dets], dtype=np.uint64)
+    dn = np.array([d[1] for d in dets], dtype=np.uint64)
+    x = np.random.default_rng(9).uniform(-1, 1, len(up))
+    init = np.zeros(len(up))
+    init[0] = 1.0
+    res = {}
+    for layout in ("csr", "segments"):
+        ham = Hamiltonian(n, h1, eri, 0.0, store="both", layout=layout)
+        ham.update(up, dn)
+        if layout == "segments":
+            k = sum(r["segments"] for r in ham.matrix.segment_ranges() if r["class"] == cls)
+            print(f"n={n} {ne}+{ne}: {len(up)} dets, {k} segments of class {cls}", flush=True)
+            assert k > 0
+        res[layout] = ham.matrix.mul(x)
+        res[layout + "32"] = ham.matrix.mul(x, fp32=True)
+        d = Davidson(1)
+        d.diagonalize(ham.matrix, init, 1e-10, max_calls=40)
+        res[layout + "_e"] = d.lowest_eigenvalues[0]
+        ham.clear()
+    np.testing.assert_allclose(res["segments"], res["csr"], rtol=1e-12, atol=1e-11)
+    np.testing.assert_allclose(res["segments32"], res["csr"], rtol=0, atol=2e-4 * np.abs(res["csr"]).max())
+    assert abs(res["segments_e"] - res["csr_e"]) < 1e-9
+
+
+@pytest.mark.gpu
 def test_segments_with_split_tiles_vs_csr():
     """Full space of 12 orbitals, 3+3 electrons (48,400 dets): the spin groups
     hold thousands of memberships, so the segment SpMV runs on groups cut into
