@@ -2364,11 +2364,21 @@ __global__ __launch_bounds__(256) void k_seg_spmv_tiles(const uint32_t* __restri
 // slots and the unused half-step term is masked.
 constexpr int kSegChunks = 4;
 template <int NC, typename VT, typename XT>
-__device__ __forceinline__ void seg_large_rows(const SegDesc& D, uint32_t j0, uint32_t d_lo, uint32_t d_hi,
-                                               bool half_step, const VT* __restrict__ val, const XT* sx,
-                                               const uint32_t* __restrict__ mem, XT* __restrict__ y) {
+__device__ __forceinline__ void seg_large_rows(const SegDesc& D, uint32_t j0, const VT* __restrict__ val,
+                                               const XT* sx, const uint32_t* __restrict__ mem, XT* __restrict__ y) {
   constexpr uint32_t U = sizeof(XT) == 8 ? 1 : 4;  // steps per batch of loads (FP64: register budget)
   const uint32_t m = D.m, mh = m >> 1, t = threadIdx.x;
+  uint32_t d_hi = (m & 1) ? mh : mh - 1;  // steps with m slots
+  uint32_t d_lo = 1;
+  bool half_step = (m & 1) == 0;
+  if constexpr (sizeof(XT) == 8) {
+    // FP64: blockIdx.z's share of the steps (the last share also the half
+    // step of an even m); the shares' sums meet in the atomics on y
+    const uint32_t per = (d_hi + gridDim.z - 1) / gridDim.z;
+    d_lo = 1 + blockIdx.z * per;
+    d_hi = min(d_hi, (blockIdx.z + 1) * per);
+    half_step = half_step && blockIdx.z + 1 == gridDim.z;
+  }
   XT acc[NC];
   uint32_t jj[NC];
 #pragma unroll
@@ -2583,13 +2593,12 @@ __global__ __launch_bounds__(64) void k_seg_spmv_large(const SegDesc* __restrict
   const uint32_t m = D.m, mh = m >> 1, t = threadIdx.x;
   const uint32_t j0 = blockIdx.y * (kWave * T);
   if (j0 >= m) return;
-  // blockIdx.z takes one share of the steps d = 1 .. fs (the last share also
-  // the half step of an even m); the shares' sums meet in the atomics on y.
-  const uint32_t fs = (m & 1) ? mh : mh - 1;  // steps with m slots
-  const uint32_t per = (fs + gridDim.z - 1) / gridDim.z;
-  const uint32_t d_lo = 1 + blockIdx.z * per, d_hi = min(fs, (blockIdx.z + 1) * per);
-  const bool half_step = (m & 1) == 0 && blockIdx.z + 1 == gridDim.z;
-  if (d_lo > d_hi && !half_step) return;
+  if constexpr (sizeof(XT) == 8) {
+    // an FP64 share with no steps (short segments of a split launch) is done
+    const uint32_t fs = (m & 1) ? mh : mh - 1;
+    const uint32_t per = (fs + gridDim.z - 1) / gridDim.z;
+    if (1 + blockIdx.z * per > fs && !((m & 1) == 0 && blockIdx.z + 1 == gridDim.z)) return;
+  }
   XT* sx = reinterpret_cast<XT*>(seg_lds);
   for (uint32_t j = t; j < m; j += kWave) {
     const XT v = x[mem[D.moff + j]];
@@ -2600,10 +2609,10 @@ __global__ __launch_bounds__(64) void k_seg_spmv_large(const SegDesc* __restrict
   const uint32_t nc = min((uint32_t)T, (m - j0 + kWave - 1) / kWave);  // chunks holding members
   static_assert(T == 4, "chunk dispatch below");
   switch (nc) {
-    case 1: seg_large_rows<1>(D, j0, d_lo, d_hi, half_step, val, sx, mem, y); break;
-    case 2: seg_large_rows<2>(D, j0, d_lo, d_hi, half_step, val, sx, mem, y); break;
-    case 3: seg_large_rows<3>(D, j0, d_lo, d_hi, half_step, val, sx, mem, y); break;
-    default: seg_large_rows<4>(D, j0, d_lo, d_hi, half_step, val, sx, mem, y); break;
+    case 1: seg_large_rows<1>(D, j0, val, sx, mem, y); break;
+    case 2: seg_large_rows<2>(D, j0, val, sx, mem, y); break;
+    case 3: seg_large_rows<3>(D, j0, val, sx, mem, y); break;
+    default: seg_large_rows<4>(D, j0, val, sx, mem, y); break;
   }
 }
 
