@@ -100,13 +100,11 @@ def pt2_bench(w, ham, coefs, energies, sample_dets):
     """Deterministic PT2 correction (shci_pt_dtm) of the ground state over the
     whole det list at eps_pt_dtm = 0.1 eps_var (the reference's ratio,
     solver.h:545-553, without its 2e-6 floor, which lies above this space's
-    eps_var and would leave no PT det); then a parity sample: the first
-    ``sample_dets`` dets and their coefficients through the device and the
-    oracle's brute-force restatement, both timed."""
-    import oracle
+    eps_var and would leave no PT det), and on the first ``sample_dets`` dets
+    with their coefficients (the sample the CPU-baseline leg repeats on the
+    oracle)."""
     from shci_amd.selection import pt_dtm
 
-    spec = w.spec
     eps_var = float(w.log[-1]["eps"])
     eps_pt = 0.1 * eps_var
     c = np.atleast_2d(coefs)[0]
@@ -114,23 +112,36 @@ def pt2_bench(w, ham, coefs, energies, sample_dets):
     e, npt, ms = pt_dtm(ham, w.up, w.dn, c, eps_pt, ev)
     m = min(sample_dets, len(w.up))
     es, ns, ms_s = pt_dtm(ham, w.up[:m], w.dn[:m], c[:m], eps_pt, ev)
+    log(f"[bench] PT2 dtm at eps_pt {eps_pt:.1e} over {len(w.up):,} dets: {npt:,} PT dets, E_pt2 {e:.10f} in "
+        f"{ms:.1f} ms; sample of {m:,} dets: device {ms_s:.1f} ms, {ns:,} PT dets")
+    return dict(eps_var=eps_var, eps_pt_dtm=eps_pt, n_dets=len(w.up), n_pt_dets=int(npt), energy_pt2=e,
+                device_ms=ms, sample=dict(n_dets=m, n_pt_dets=int(ns), energy_pt2=es, device_ms=ms_s),
+                note="device_ms is wall time of shci_pt_dtm, host arrays in (PCIe included)"), (c, ev)
+
+
+def pt2_cpu_sample(w, pt2, state):
+    """The CPU-baseline leg of the PT2 record: the oracle's brute-force
+    restatement on the device sample's dets, timed, and compared with the
+    device result."""
+    import oracle
+
+    c, ev = state
+    m = pt2["sample"]["n_dets"]
+    spec = w.spec
     orc = oracle.System.from_arrays(w.n, spec.n_up, spec.n_dn, w.h1, w.eri, w.core)
     t0 = time.time()
-    eo, no = orc.pt_dtm(w.up[:m], w.dn[:m], c[:m], eps_pt, ev)
+    eo, no = orc.pt_dtm(w.up[:m], w.dn[:m], c[:m], pt2["eps_pt_dtm"], ev)
     t_o = time.time() - t0
+    es, ns = pt2["sample"]["energy_pt2"], pt2["sample"]["n_pt_dets"]
     same = ns == no and abs(es - eo) <= 1e-12 * abs(eo)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    log(f"[bench] PT2 dtm at eps_pt {eps_pt:.1e} over {len(w.up):,} dets: {npt:,} PT dets, E_pt2 {e:.10f} in "
-        f"{ms:.1f} ms; sample of {m:,} dets: device {ms_s:.1f} ms, oracle brute force {t_o * 1e3:.0f} ms on "
-        f"{threads} threads, {ns:,} PT dets, match={same}")
+    log(f"[bench] PT2 sample of {m:,} dets: oracle brute force {t_o * 1e3:.0f} ms on {threads} threads, "
+        f"match={same}")
     if not same:
         raise SystemExit("PT2: device and oracle differ on the sample")
-    return dict(eps_var=eps_var, eps_pt_dtm=eps_pt, n_dets=len(w.up), n_pt_dets=int(npt), energy_pt2=e,
-                device_ms=ms, sample=dict(n_dets=m, n_pt_dets=int(ns), device_ms=ms_s, oracle_ms=t_o * 1e3,
-                                          oracle_threads=threads, match=bool(same),
-                                          note="the oracle enumerates every single and double of each det "
-                                               "(no queues): a parity check, not a tuned CPU code"),
-                note="device_ms is wall time of shci_pt_dtm, host arrays in (PCIe included)")
+    pt2["sample"].update(oracle_ms=t_o * 1e3, oracle_threads=threads, match=bool(same),
+                         note="the oracle enumerates every single and double of each det (no queues): a parity "
+                              "check, not a tuned CPU code")
 
 
 def cpu_baseline(w, sample_dets, reps_spmv=2):
@@ -344,7 +355,8 @@ def main():
         f"SpMV32 {spmv32_ms:.3f} ms = {spmv32_bytes / spmv32_ms / 1e6:.0f} GB/s")
     ham.clear()
     selection = None if args.no_selection else selection_bench(w, ham, d.get_lowest_eigenvectors())
-    pt2 = None if args.no_pt2 else pt2_bench(w, ham, d.get_lowest_eigenvectors(), e_ref_all, args.pt2_sample_dets)
+    pt2, pt2_state = (None, None) if args.no_pt2 else pt2_bench(w, ham, d.get_lowest_eigenvectors(), e_ref_all,
+                                                                 args.pt2_sample_dets)
 
     # ---------------- CPU baseline ----------------
     cpu = None
@@ -357,6 +369,8 @@ def main():
                            f"upper entries), Davidson to |dE|<={args.tol:g} {c['t_dav']:.2f}s; reference-layout "
                            f"SpMV {c['t_spmv'] * 1e3:.0f} ms = {c['stored'] * 16 / c['t_spmv'] / 1e9:.2f} GB/s"))
         log(f"[bench] CPU: {cpu['value']:.0f} dets/s ({cpu['sample']})")
+        if pt2 is not None:
+            pt2_cpu_sample(w, pt2, pt2_state)
 
     out = {
         "metric": METRIC,
