@@ -3916,9 +3916,10 @@ int Davidson<double>::products(const std::vector<size_t>& slots) {
 template <>
 int Davidson<float>::products(const std::vector<size_t>& slots) {
   // The block product (values read once for up to 4 vectors) when it applies:
-  // two or more vectors, no time_sym projection, segment x staging in LDS.
+  // two or more vectors, segment x staging in LDS.  Under time_sym each
+  // product is then projected, as launch_spmv projects a single one.
   const uint32_t seg_big = *std::max_element(ctx->seg_maxm, ctx->seg_maxm + 9);
-  const bool block_ok = !ctx->ts && (ctx->segl ? (size_t)seg_big * 4 * sizeof(float) <= 65536 : true);
+  const bool block_ok = ctx->segl ? (size_t)seg_big * 4 * sizeof(float) <= 65536 : true;
   if (slots.size() < 2 || !block_ok) {
     for (size_t k : slots) CHECK_RC(spmv(v[k], hv[k]));
     return SHCI_OK;
@@ -3937,6 +3938,9 @@ int Davidson<float>::products(const std::vector<size_t>& slots) {
     CHECK_RC(ctx->segl ? launch_seg_spmv<4, float, float>(ctx, shift, P<float>(bx), P<float>(by))
                        : launch_spmm4(ctx, shift, P<float>(bx), P<float>(by)));
     k_unpack4<<<div_up(n, 256), 256, 0, st>>>(P<float>(by), k, n, dst);
+    if (ctx->ts)
+      for (uint32_t j = 0; j < k; j++)
+        k_ts_sym<float><<<div_up(n, 256), 256, 0, st>>>(dst.p[j], P<uint32_t>(ctx->ts_rev), n);
     HIPC(hipGetLastError());
   }
   return SHCI_OK;

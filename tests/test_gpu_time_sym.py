@@ -74,6 +74,13 @@ def test_gpu_time_sym_vs_oracle(layout, n, ne, seed, eps):
     dm = Davidson(1)
     dm.diagonalize(ham.matrix, init, 1e-10, mode="mixed", max_calls=40)
     assert abs(dm.lowest_eigenvalues[0] - w[0]) < 1e-6
+    # three roots: the mixed mode's block products, each projected
+    init3 = np.zeros((3, m))
+    for k in range(3):
+        init3[k, k] = 1.0
+    d3 = Davidson(3)
+    d3.diagonalize(ham.matrix, init3, 1e-10, mode="mixed", max_calls=100)
+    np.testing.assert_allclose(d3.lowest_eigenvalues, w[:3], rtol=0, atol=1e-6)
     # RDMs of the unpacked wavefunction (each pair's coefficient / sqrt2 on
     # both dets, the reference's SQRT2_INV)
     r1, r2 = ham.rdm(c[None])
