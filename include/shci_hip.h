@@ -2,8 +2,8 @@
  *
  * The reference ("Arrow", QMC-Cornell/shci) computes this path on the CPU in
  *   Hamiltonian<S>::update          src/solver/hamiltonian.h:87-598
- *   SparseMatrix::mul               src/solver/sparse_matrix.cc:88-116
- *   Davidson::diagonalize           src/solver/davidson.cc:37-225
+ *   SparseMatrix::mul               src/solver/sparse_matrix.cc:28-56
+ *   Davidson::diagonalize           src/solver/davidson.cc:8-196
  *   RDM::get_1rdm / get_2rdm        src/chem/rdm.cc:12-131, 374-395
  * the step before it, the heat-bath selection sweep
  *   Solver<S>::run_variation loop   src/solver/solver.h:435-495
@@ -100,16 +100,16 @@ int shci_build_hamiltonian(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn
 int shci_rebuild_hamiltonian(shci_ctx* ctx, uint32_t flags);
 
 /* Hamiltonian<S>::update_existing_elems (hamiltonian.h:114-126,
- * sparse_matrix.cc:191-201): recompute every stored value with the current
+ * sparse_matrix.cc:131-141): recompute every stored value with the current
  * integrals on the fixed connection pattern. */
 int shci_update_existing_elems(shci_ctx* ctx);
 
 /* Hamiltonian<S>::clear / SparseMatrix::clear (hamiltonian.h:128-155,
- * sparse_matrix.cc:153-160): release the determinants, the matrix and all
+ * sparse_matrix.cc:93-100): release the determinants, the matrix and all
  * work buffers (the integrals stay). */
 int shci_clear(shci_ctx* ctx);
 
-/* SparseMatrix::count_n_elems (sparse_matrix.cc:73-86) and row count:
+/* SparseMatrix::count_n_elems (sparse_matrix.cc:13-26) and row count:
  * n_stored_upper counts the diagonal + pairs i<j (the reference's storage);
  * n_elems = 2 * n_stored_upper - n_dets (its printed "nonzero elems"). */
 int shci_get_counts(shci_ctx* ctx, uint64_t* n_dets, uint64_t* n_stored_upper, uint64_t* n_elems);
@@ -117,7 +117,7 @@ int shci_get_counts(shci_ctx* ctx, uint64_t* n_dets, uint64_t* n_stored_upper, u
 /* SparseMatrix::get_diag() (sparse_matrix.h:16): FP64 diagonal, caller order. */
 int shci_get_diag(shci_ctx* ctx, double* diag);
 
-/* SparseMatrix::mul (sparse_matrix.cc:88-116): y = H x, caller order.
+/* SparseMatrix::mul (sparse_matrix.cc:28-56): y = H x, caller order.
  * FP64 by default; with SHCI_FP32 the FP32 store is used (values FP32, row
  * accumulation FP32, diagonal FP64).  SHCI_DEVICE_PTRS: x, y on the device. */
 int shci_spmv(shci_ctx* ctx, const double* x, double* y, uint32_t flags);
@@ -127,9 +127,9 @@ int shci_spmv(shci_ctx* ctx, const double* x, double* y, uint32_t flags);
  * but never defines it; this is the block form used by the multi-root path. */
 int shci_spmm(shci_ctx* ctx, const double* X, double* Y, uint32_t k, uint32_t flags);
 
-/* Davidson::diagonalize (src/solver/davidson.cc:37-225) on the built H.
+/* Davidson::diagonalize (src/solver/davidson.cc:8-196) on the built H.
  * init: n_states x n_dets (caller order); evals[n_states], evecs n_states x n.
- * tol: eigenvalue-change tolerance (davidson.cc:203-210).  mode: SHCI_DAVIDSON_*.
+ * tol: eigenvalue-change tolerance (davidson.cc:174-185).  mode: SHCI_DAVIDSON_*.
  * max_calls: the reference performs at most 2*5*n_states - n_states
  * iterations per call and its caller re-invokes with the Ritz vectors
  * (solver.h:430-530); max_calls > 1 repeats that warm restart here until

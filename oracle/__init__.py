@@ -13,20 +13,20 @@ independent of the oracle's own formulas):
 oracle part (reference)                pinned by
 =====================================  ===================================================
 diff / permutation_factor              tests/test_oracle_dets.py: half_det_test.cc vectors;
-(src/det/half_det.cc:201-287)          brute-force creation/annihilation sign on all pairs
+(src/det/half_det.cc:120-206)          brute-force creation/annihilation sign on all pairs
                                        of strings with n <= 8
 Slater-Condon elements                 tests/test_oracle_elements.py: Jordan-Wigner
 (src/chem/chem_system.cc:409-609)      second-quantised H (dense, <= 12 spin orbitals)
                                        element by element; dense FCI eigenvalue
-FCIDUMP read / keys / reorder          tests/test_oracle_integrals.py: synthetic round trip;
+FCIDUMP read / keys / reorder          tests/test_integrals.py: synthetic round trip;
 (src/chem/integrals.cc:37-422)         examples/C HF energy and orbital energies (log)
 Hamiltonian::update (connections)      tests/test_oracle_hamiltonian.py: brute-force O(N^2)
 (src/solver/hamiltonian.h:87-598)      pattern; examples/C nnz and absingles counts (log);
                                        incremental == from-scratch
 SparseMatrix::mul                      dense product of the symmetric expansion
-(src/solver/sparse_matrix.cc:88-116)
+(src/solver/sparse_matrix.cc:28-56)
 Davidson::diagonalize                  DavidsonTest.HilbertSystem known answer
-(src/solver/davidson.cc:37-225)        (src/solver/davidson_test.cc:26-50); scipy eigh;
+(src/solver/davidson.cc:8-196)          (src/solver/davidson_test.cc:26-50); scipy eigh;
                                        examples/C per-iteration "Davidson #k" trajectory
 RDM 1/2 (src/chem/rdm.cc)              trace / partial-trace identities; energy identity
                                        rdm.cc:731-745 vs the eigenvalue; finite differences

@@ -225,7 +225,7 @@ static void load_integrals(const std::string& path, unsigned n_up, unsigned n_dn
 }
 
 // ---------------------------------------------------------------------------
-// HalfDet::diff for one 64-bit chunk: src/det/half_det.cc:201-287
+// HalfDet::diff for one 64-bit chunk: src/det/half_det.cc:120-206
 // ---------------------------------------------------------------------------
 struct DiffResult {
   unsigned left_only[2] = {0, 0};
@@ -420,12 +420,12 @@ struct SparseMatrix {
   std::vector<std::vector<double>> val;
   std::vector<double> diag;
 
-  void set_dim(size_t dim) {  // sparse_matrix.cc:147-151
+  void set_dim(size_t dim) {  // sparse_matrix.cc:87-91
     idx.resize(dim);
     val.resize(dim);
     diag.resize(dim, 0.0);
   }
-  void append_elem(size_t i, size_t j, double v) {  // sparse_matrix.cc:68-71
+  void append_elem(size_t i, size_t j, double v) {  // sparse_matrix.cc:8-11
     idx[i].push_back(j);
     val[i].push_back(v);
     if (i == j) diag[i] = v;
@@ -435,8 +435,8 @@ struct SparseMatrix {
     for (const auto& r : idx) s += r.size();
     return s;
   }
-  size_t count_n_elems() const { return n_stored() * 2 - idx.size(); }  // sparse_matrix.cc:73-86
-  // sparse_matrix.cc:88-116 (one rank: the allreduce is the identity)
+  size_t count_n_elems() const { return n_stored() * 2 - idx.size(); }  // sparse_matrix.cc:13-26
+  // sparse_matrix.cc:28-56 (one rank: the allreduce is the identity)
   std::vector<double> mul(const std::vector<double>& vec) const {
     const size_t dim = idx.size();
     std::vector<double> res(dim, 0.0);
@@ -696,7 +696,7 @@ struct Hamiltonian {
     }
   }
 
-  // hamiltonian.h:114-126 + sparse_matrix.cc:191-201
+  // hamiltonian.h:114-126 + sparse_matrix.cc:131-141
   void update_existing_elems(const Chem& sys, const std::vector<Det>& dets) {
     const size_t dim = matrix.idx.size();
 #pragma omp parallel for schedule(dynamic, 4)
@@ -713,7 +713,7 @@ struct Hamiltonian {
 
 // ---------------------------------------------------------------------------
 // Dense symmetric eigensolver for the Rayleigh-Ritz step (stands in for
-// Eigen::SelfAdjointEigenSolver at src/solver/davidson.cc:174-177).  Cyclic
+// Eigen::SelfAdjointEigenSolver at src/solver/davidson.cc:145-148).  Cyclic
 // Jacobi; eigenvalues ascending, eigenvectors as columns.
 // ---------------------------------------------------------------------------
 static void jacobi_eigh(int n, const std::vector<double>& a_in, std::vector<double>& w,
@@ -778,7 +778,7 @@ static double dot_omp(const std::vector<double>& a, const std::vector<double>& b
 }
 
 // ---------------------------------------------------------------------------
-// Davidson::diagonalize: src/solver/davidson.cc:37-225
+// Davidson::diagonalize: src/solver/davidson.cc:8-196
 // ---------------------------------------------------------------------------
 struct DavidsonResult {
   std::vector<double> evals;

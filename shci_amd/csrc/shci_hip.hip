@@ -8,11 +8,11 @@
 //     reference's exact operation order (bit-identical values, so the
 //     |H| >= 1e-12 cut keeps the same pattern), and writes a full-symmetric CSR
 //     (off-diagonal; FP64 diagonal kept apart) in two passes: count -> scan -> fill.
-//   * SpMV (reference: src/solver/sparse_matrix.cc:88-116): full-symmetric CSR,
+//   * SpMV (reference: src/solver/sparse_matrix.cc:28-56): full-symmetric CSR,
 //     L lanes per row (L chosen from the mean row length), shuffle reductions, no
 //     atomics; FP64, FP32 (FP32 values + FP32 row accumulation, FP64 diagonal) and
 //     FP32-values/FP64-accumulation variants, plus a 4-vector block product.
-//   * Davidson (reference: src/solver/davidson.cc:37-225) with device vectors,
+//   * Davidson (reference: src/solver/davidson.cc:8-196) with device vectors,
 //     device-resident Gram-Schmidt scalars and a host Jacobi Rayleigh-Ritz, in
 //     FP64 or mixed precision.
 //   * RDM adjoint (reference: src/chem/rdm.cc:12-131, 374-395, 435-657) over the
@@ -922,7 +922,7 @@ __global__ void k_copy_scale(const TI* __restrict__ x, TO* __restrict__ v, doubl
   if (i < n) v[i] = (TO)((double)x[i] * inv);
 }
 
-// Diagonal preconditioner (davidson.cc:135-143): lam and diag in FP64.
+// Diagonal preconditioner (davidson.cc:106-114): lam and diag in FP64.
 template <typename T>
 __global__ void k_precond(T* __restrict__ v, const T* __restrict__ Hw, const T* __restrict__ w,
                           const double* __restrict__ diag, double lam, double shift, uint64_t n) {
@@ -938,7 +938,7 @@ struct RitzArgs {
   double c[kMaxDots];
 };
 
-// w = sum_i c_i v_i, Hw = sum_i c_i Hv_i (davidson.cc:184-194), FP64 accumulation.
+// w = sum_i c_i v_i, Hw = sum_i c_i Hv_i (davidson.cc:155-165), FP64 accumulation.
 template <typename T, typename TO>
 __global__ void k_ritz(RitzArgs R, int m, uint64_t n, TO* __restrict__ w, TO* __restrict__ hw) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1076,7 +1076,7 @@ __device__ __forceinline__ double permfac_ccaa(uint64_t h, int p, int q, int r, 
 }
 
 __device__ __forceinline__ double pf_of(uint64_t l, uint64_t rr) {
-  // diff(l, rr).permutation_factor (half_det.cc:201-287)
+  // diff(l, rr).permutation_factor (half_det.cc:120-206)
   int c = 0;
   for (uint64_t m = l & ~rr; m; m &= m - 1) c += below(l, ctz64(m));
   for (uint64_t m = rr & ~l; m; m &= m - 1) c += below(rr, ctz64(m));
@@ -3740,7 +3740,7 @@ void jacobi_eigh(int n, const std::vector<double>& a_in, std::vector<double>& w,
   v.swap(vs);
 }
 
-// One reference Davidson call (davidson.cc:37-225) on internal-order device
+// One reference Davidson call (davidson.cc:8-196) on internal-order device
 // vectors of type T.  For T = float the matrix is applied shifted by `shift`
 // (FP32 store, FP32 row accumulation, FP64 diagonal) and eigenvalues are
 // reported in the shifted frame.
@@ -3837,7 +3837,7 @@ struct Davidson {
       const double lam = lowest[i_state_precond];
       k_precond<T><<<g, 256, 0, st>>>(v[ic], hw[i_state_precond], w[i_state_precond], P<double>(ctx->diag), lam,
                                       shift, n);
-      // Modified Gram-Schmidt with device-resident scalars (davidson.cc:146-152).
+      // Modified Gram-Schmidt with device-resident scalars (davidson.cc:116-123).
       for (size_t i = 0; i < ic; i++) {
         const T* b[1] = {v[i]};
         CHECK_RC(dots<T>(ctx, v[ic], b, 1, nullptr));
@@ -3946,7 +3946,7 @@ int Davidson<float>::products(const std::vector<size_t>& slots) {
   return SHCI_OK;
 }
 
-// Normalise + orthogonalise the initial vectors (davidson.cc:73-88) into v[0..ns).
+// Normalise + orthogonalise the initial vectors (davidson.cc:44-59) into v[0..ns).
 template <typename T>
 int init_vectors(shci_ctx* ctx, Davidson<T>& D, const double* init_dev_internal) {
   const uint64_t n = ctx->n;
@@ -5140,7 +5140,7 @@ static int davidson_full(shci_ctx* ctx, uint32_t n_states, const double* init, d
   *n_traj = 0;
   const auto t0 = std::chrono::steady_clock::now();
   if (max_calls == 0) max_calls = 1;
-  if (n == 1) {  // davidson.cc:49-55
+  if (n == 1) {  // davidson.cc:20-26
     double d;
     HIPC(hipMemcpy(&d, ctx->diag.p, 8, hipMemcpyDeviceToHost));
     evals[0] = d;

@@ -100,7 +100,7 @@ class SparseMatrix:
         return self.counts()[0] if self._built else 0
 
     def count_n_elems(self) -> int:
-        """``2 * stored_upper - n`` (sparse_matrix.cc:73-86)."""
+        """``2 * stored_upper - n`` (sparse_matrix.cc:13-26)."""
         return self.counts()[2]
 
     def get_diag(self, i=None):
@@ -109,7 +109,7 @@ class SparseMatrix:
         self.ctx.check(self.ctx.lib.shci_get_diag(self.ctx.handle, d.ctypes.data_as(_dp)), "shci_get_diag")
         return d if i is None else float(d[i])
 
-    # -- products (sparse_matrix.cc:88-145) --
+    # -- products (sparse_matrix.cc:28-85) --
     def mul(self, vec, fp32: bool = False):
         vec = np.asarray(vec)
         if np.iscomplexobj(vec):

@@ -21,7 +21,7 @@ int io_fail(const std::string& m) {
   return -1;
 }
 
-// Element D_pqrs of the packed 2-RDM (RDM::combine4_2rdm, rdm.cc:238-246).
+// Element D_pqrs of the packed 2-RDM (RDM::combine4_2rdm, rdm.cc:233-241).
 double rdm2_elem(const double* rdm2, size_t n, unsigned p, unsigned q, unsigned r, unsigned s) {
   const size_t a = (size_t)p * n + s, b = (size_t)q * n + r;
   return a > b ? rdm2[a * (a + 1) / 2 + b] : rdm2[b * (b + 1) / 2 + a];

@@ -1,4 +1,4 @@
-"""Pins Davidson::diagonalize (src/solver/davidson.cc:37-225) with the
+"""Pins Davidson::diagonalize (src/solver/davidson.cc:8-196) with the
 reference's own known-answer test DavidsonTest.HilbertSystem
 (src/solver/davidson_test.cc:5-51) and scipy's dense eigensolver."""
 import numpy as np

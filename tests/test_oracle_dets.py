@@ -1,4 +1,4 @@
-"""Pins HalfDet::diff (src/det/half_det.cc:201-287): the reference's own unit
+"""Pins HalfDet::diff (src/det/half_det.cc:120-206): the reference's own unit
 test vectors (src/det/half_det_test.cc:30-46) and, for the fermionic sign the
 reference tests leave unchecked, brute-force operator application on every
 pair of strings of up to 8 orbitals."""
