@@ -66,8 +66,9 @@ enum {
                                   coefficients of n_dets entries, H_ts elements of
                                   base_system.h:86-101 (products, diagonal, Davidson);
                                   shci_rdm unpacks like unpack_time_sym.  The matrix is built
-                                  over the dets and their reverses; shci_get_counts reports
-                                  that matrix's entries; shci_export_upper is unsupported. */
+                                  over the dets and their reverses; shci_get_counts answers
+                                  n_dets only (SHCI_ERR_UNSUPPORTED when an element count
+                                  is asked for); shci_export_upper is unsupported. */
 };
 
 /* Davidson modes. */
@@ -111,7 +112,8 @@ int shci_clear(shci_ctx* ctx);
 
 /* SparseMatrix::count_n_elems (sparse_matrix.cc:13-26) and row count:
  * n_stored_upper counts the diagonal + pairs i<j (the reference's storage);
- * n_elems = 2 * n_stored_upper - n_dets (its printed "nonzero elems"). */
+ * n_elems = 2 * n_stored_upper - n_dets (its printed "nonzero elems").  Any
+ * pointer may be NULL.  Under SHCI_TIME_SYM only n_dets is answered. */
 int shci_get_counts(shci_ctx* ctx, uint64_t* n_dets, uint64_t* n_stored_upper, uint64_t* n_elems);
 
 /* SparseMatrix::get_diag() (sparse_matrix.h:16): FP64 diagonal, caller order. */

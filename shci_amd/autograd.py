@@ -93,12 +93,16 @@ class VariationalSolver:
     def rdm(self, coefs=None):
         return self.ham.rdm(self.coefs if coefs is None else coefs)
 
-    def grads(self, weights=None):
-        """(dE/dh, dE/deri) for E = sum_s w_s E_s (default: state average)."""
+    def grads(self, weights=None, coefs=None):
+        """(dE/dh, dE/deri) for E = sum_s w_s E_s (default: state average) at
+        the eigenvectors ``coefs`` (default: those of the latest solve).  The
+        connection pattern is fixed (every excitation-<=2 pair is kept), so
+        the RDMs depend on the coefficients only, whatever integrals the
+        solver holds now."""
         w = np.full(self.n_states, 1.0 / self.n_states) if weights is None else np.asarray(weights, float)
-        c = self.coefs * np.sqrt(np.abs(w))[:, None]
         if np.any(w < 0):
             raise ValueError("negative state weights are not supported")
+        c = (self.coefs if coefs is None else np.atleast_2d(coefs)) * np.sqrt(np.abs(w))[:, None]
         r1, r2 = self.rdm(c)
         g_h = 0.5 * (r1 + r1.T)
         return g_h, self.eri_grad_from_rdm(r2)
@@ -119,18 +123,23 @@ class VariationalSolver:
 
 
 class VariationalEnergy(torch.autograd.Function):
-    """E(h, eri) with the RDM adjoint; ``solver`` carries the determinant list."""
+    """E(h, eri) with the RDM adjoint; ``solver`` carries the determinant list.
+
+    The forward keeps its own eigenvectors on ``ctx``: one solver may serve
+    several forward calls (E(h_a) + E(h_b), a closure re-evaluated before the
+    backward), and each backward contracts the vectors of its own solve."""
 
     @staticmethod
     def forward(ctx, h1: torch.Tensor, eri: torch.Tensor, solver: VariationalSolver):
         e = solver.solve(h1.detach().cpu().double().numpy(), eri.detach().cpu().double().numpy())
         ctx.solver = solver
+        ctx.coefs = solver.coefs.copy()
         ctx.dev = h1.device
         return torch.tensor(e.mean(), dtype=torch.float64, device=h1.device)
 
     @staticmethod
     def backward(ctx, grad_e: torch.Tensor):
-        g_h, g_eri = ctx.solver.grads()
+        g_h, g_eri = ctx.solver.grads(coefs=ctx.coefs)
         g = grad_e.detach().cpu().double()
         return (torch.from_numpy(g_h).to(ctx.dev) * g.to(ctx.dev),
                 torch.from_numpy(g_eri).to(ctx.dev) * g.to(ctx.dev), None)

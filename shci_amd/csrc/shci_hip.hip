@@ -4998,6 +4998,12 @@ static int update_full(shci_ctx* ctx) {
 int shci_get_counts(shci_ctx* ctx, uint64_t* n_dets, uint64_t* n_stored_upper, uint64_t* n_elems) {
   if (!ctx) return SHCI_ERR_ARG;
   if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
+  // Under time_sym the stored matrix is the one over the dets and their
+  // reverses, not H_ts: its entry counts belong to neither the matrix the
+  // caller's vectors live in nor the reference's (which counts H_ts), so only
+  // the row count is answered.
+  if (ctx->ts && (n_stored_upper || n_elems))
+    return fail(ctx, SHCI_ERR_UNSUPPORTED, "time_sym: the element counts of H_ts are not available (n_dets only)");
   const uint64_t up = ctx->nnz / 2 + ctx->n;
   if (n_dets) *n_dets = ctx->ts ? ctx->n_red : ctx->n;
   if (n_stored_upper) *n_stored_upper = up;

@@ -97,7 +97,11 @@ class SparseMatrix:
         return n.value, up.value, el.value
 
     def count_n_rows(self) -> int:
-        return self.counts()[0] if self._built else 0
+        if not self._built:
+            return 0
+        n = C.c_uint64()
+        self.ctx.check(self.ctx.lib.shci_get_counts(self.ctx.handle, C.byref(n), None, None), "shci_get_counts")
+        return n.value
 
     def count_n_elems(self) -> int:
         """``2 * stored_upper - n`` (sparse_matrix.cc:13-26)."""

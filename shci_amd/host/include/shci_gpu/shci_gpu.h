@@ -59,12 +59,11 @@ class SparseMatrix {
  public:
   explicit SparseMatrix(shci_ctx* ctx = nullptr) : ctx_(ctx ? ctx : default_context()) {}
 
-  double get_diag(size_t i) const { return get_diag().at(i); }
-  std::vector<double> get_diag() const {
-    std::vector<double> d(count_n_rows());
-    if (!d.empty()) check(ctx_, shci_get_diag(ctx_, d.data()), "shci_get_diag");
-    return d;
-  }
+  // The diagonal is copied to the host once per build or update and served
+  // from there: get_diag(i) is O(1), as in the reference (sparse_matrix.h:14),
+  // whose callers read it element by element inside loops.
+  double get_diag(size_t i) const { return diag_cache().at(i); }
+  std::vector<double> get_diag() const { return diag_cache(); }
   size_t count_n_elems() const {
     uint64_t n, up, el;
     check(ctx_, shci_get_counts(ctx_, &n, &up, &el), "shci_get_counts");
@@ -72,8 +71,8 @@ class SparseMatrix {
   }
   size_t count_n_rows() const {
     if (!built_) return 0;
-    uint64_t n, up, el;
-    check(ctx_, shci_get_counts(ctx_, &n, &up, &el), "shci_get_counts");
+    uint64_t n;
+    check(ctx_, shci_get_counts(ctx_, &n, nullptr, nullptr), "shci_get_counts");
     return n;
   }
   std::vector<double> mul(const std::vector<double>& vec) const {
@@ -111,14 +110,26 @@ class SparseMatrix {
     rp_.clear();
     cols_.clear();
     vals_.clear();
+    diag_.clear();
+    diag_ok_ = false;
   }
   shci_ctx* context() const { return ctx_; }
+  // Called after every build and update_existing_elems: drops the host copies.
   void mark_built() {
     built_ = true;
     rp_.clear();
+    diag_ok_ = false;
   }
 
  private:
+  const std::vector<double>& diag_cache() const {
+    if (!diag_ok_) {
+      diag_.assign(count_n_rows(), 0.0);
+      if (!diag_.empty()) check(ctx_, shci_get_diag(ctx_, diag_.data()), "shci_get_diag");
+      diag_ok_ = true;
+    }
+    return diag_;
+  }
   void export_cache() const {
     if (!rp_.empty()) return;
     uint64_t n, up, el;
@@ -132,6 +143,8 @@ class SparseMatrix {
   bool built_ = false;
   mutable std::vector<int64_t> rp_, cols_;
   mutable std::vector<double> vals_;
+  mutable std::vector<double> diag_;
+  mutable bool diag_ok_ = false;
 };
 
 // hamiltonian.h:14-79.  store_flags: SHCI_STORE_FP64 and/or SHCI_STORE_FP32.

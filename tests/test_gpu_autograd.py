@@ -80,3 +80,41 @@ def test_finite_differences_cfg4_generator_20_directions():
     solver = VariationalSolver(w.n, w.up, w.dn, w.core)
     worst = _fd_check(solver, w.h1, w.eri, n_dirs=20, seed=2024)
     print(f"{len(w.up)} dets, worst relative error {worst:.2e}")
+
+
+@pytest.mark.gpu
+def test_two_forwards_on_one_solver():
+    """E(h_a, v_a) + E(h_b, v_b) through one solver, one backward: each
+    input's gradient is the RDM contraction of its own solve (the oracle's
+    RDMs of that solve's vector), not of the solver's latest state."""
+    import torch
+
+    from bruteforce import dets_all
+    from shci_amd.autograd import VariationalSolver, variational_energy
+    from test_oracle_rdm import grad_eri_from_rdm
+
+    n, nu, nd = 7, 3, 2
+    _, h_a, v_a, _ = si.synthetic(n, seed=41)
+    _, h_b, v_b, _ = si.synthetic(n, seed=43)
+    dets = dets_all(n, nu, nd)
+    hf = ((1 << nu) - 1, (1 << nd) - 1)
+    dets.remove(hf)
+    dets = [hf] + dets
+    up = np.array([d[0] for d in dets], dtype=np.uint64)
+    dn = np.array([d[1] for d in dets], dtype=np.uint64)
+    solver = VariationalSolver(n, up, dn)
+    ha, va = torch.tensor(h_a, requires_grad=True), torch.tensor(v_a, requires_grad=True)
+    hb, vb = torch.tensor(h_b, requires_grad=True), torch.tensor(v_b, requires_grad=True)
+    e = variational_energy(ha, va, solver) + variational_energy(hb, vb, solver)
+    e.backward()
+    for h1, eri, gh, gv in ((h_a, v_a, ha.grad, va.grad), (h_b, v_b, hb.grad, vb.grad)):
+        orc = oracle.System.from_arrays(n, nu, nd, h1, eri, 0.0)
+        orc.update(up, dn)
+        rp, cols, vals = orc.export_upper()
+        H = np.zeros((len(up), len(up)))
+        H[np.repeat(np.arange(len(up)), np.diff(rp)), cols] = vals
+        H = H + np.triu(H, 1).T
+        vec = np.linalg.eigh(H)[1][:, :1].T  # ground state of the dense matrix
+        r1, r2 = orc.rdm(vec)
+        np.testing.assert_allclose(gh.numpy(), 0.5 * (r1 + r1.T), atol=1e-8)
+        np.testing.assert_allclose(gv.numpy(), grad_eri_from_rdm(n, r2), atol=1e-8)

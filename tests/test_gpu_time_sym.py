@@ -54,7 +54,13 @@ def test_gpu_time_sym_vs_oracle(layout, n, ne, seed, eps):
     print(f"[ts] n={n} {ne}+{ne} {layout}: {m} symmetrised dets, {int(np.sum(up != dn))} pairs", flush=True)
     ham = Hamiltonian(n, h1, eri, core, store="both", layout=layout, time_sym=True)
     ham.update(up, dn)
-    assert ham.matrix.counts()[0] == m
+    assert ham.matrix.count_n_rows() == m
+    # the stored matrix is over the dets and their reverses: its entry
+    # counts are not H_ts's, so they are refused rather than misreported
+    from shci_amd.hamiltonian import ShciError
+
+    with pytest.raises(ShciError, match="time_sym"):
+        ham.matrix.count_n_elems()
     np.testing.assert_allclose(ham.matrix.get_diag(), np.diag(H), rtol=0, atol=1e-12)
     x = np.random.default_rng(seed).uniform(-1, 1, m)
     np.testing.assert_allclose(ham.matrix.mul(x), H @ x, rtol=0, atol=1e-11)
