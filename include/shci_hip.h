@@ -211,6 +211,21 @@ int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, doubl
  * it, sparse_matrix.h:28-32). */
 int shci_time_spmm(shci_ctx* ctx, uint32_t k, int reps, float* ms_avg, double* bytes);
 
+/* Ritz-vector capture (no reference counterpart; measurement).  With
+ * max_vectors > 0, every later shci_davidson call keeps, in device memory, the
+ * ground-state Ritz vector behind each of its trajectory entries (the start
+ * vector's and every Rayleigh-Ritz step's; not the final re-evaluation of
+ * MIXED mode), at most max_vectors of them, widened to FP64; each call starts
+ * afresh.  0 stops the capture and frees the vectors.  The capture's widening
+ * copy is queued after each entry's timestamp. */
+int shci_capture_ritz(shci_ctx* ctx, uint32_t max_vectors);
+/* FP64 energies <w|H|w> / <w|w> of the captured vectors with the FP64 store
+ * (the matrix rebuilt with SHCI_STORE_FP64 over the same dets): e[k] for
+ * k < min(cap, captured); *n_captured receives the number captured.  This is
+ * how a MIXED run's time to a given FP64 accuracy is measured: the
+ * trajectory's timestamps, the energies from here. */
+int shci_ritz_energies(shci_ctx* ctx, double* e, uint32_t cap, uint32_t* n_captured);
+
 /* Layout statistics (no reference counterpart; diagnostics): stats[14] =
  * n_dets, stored off-diagonal entries (both directions), allocated entries
  * (both directions), segment layout?, segments, segment memberships,

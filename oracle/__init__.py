@@ -316,6 +316,23 @@ def sparse_davidson(dim, entries, init, tol):
         lib.orc_sm_free(m)
 
 
+def upper_digest(rp, cols, vals):
+    """Order-free digest of the (row, column, FP64 value bits) triples of an
+    upper-triangle export (row_ptr, cols, vals): equal digests and equal
+    row pointers stand for the same pattern, signs and bit-identical values
+    in any within-row order; a wrong column, a dropped entry or a value one
+    ulp off changes it.  For exports too large to sort."""
+    rows = np.repeat(np.arange(len(rp) - 1, dtype=np.uint64), np.diff(rp).astype(np.int64))
+    with np.errstate(over="ignore"):
+        h = rows * np.uint64(0x9E3779B97F4A7C15)
+        h ^= np.asarray(cols).astype(np.uint64) * np.uint64(0xC2B2AE3D27D4EB4F)
+        h ^= np.ascontiguousarray(vals, dtype=np.float64).view(np.uint64) * np.uint64(0x165667B19E3779F9)
+        h ^= h >> np.uint64(31)
+        h *= np.uint64(0xD6E8FEB86659FD93)
+        h ^= h >> np.uint64(32)
+        return int(np.bitwise_xor.reduce(h)), int(h.sum(dtype=np.uint64))
+
+
 def unpack_2rdm(packed, n):
     """Full ``D[p, q, r, s]`` from the reference's packed layout (rdm.cc:233-241, 683-685)."""
     a = (np.arange(n)[:, None] * n + np.arange(n)[None, :])  # a[p, s] = p*n + s

@@ -48,6 +48,8 @@ EXPORTS = {
     "shci_get_build_timings": (C.c_int, [_vp, _fp]),
     "shci_time_spmv": (C.c_int, [_vp, C.c_uint32, C.c_int, _fp, _dp]),
     "shci_time_spmm": (C.c_int, [_vp, C.c_uint32, C.c_int, _fp, _dp]),
+    "shci_capture_ritz": (C.c_int, [_vp, C.c_uint32]),
+    "shci_ritz_energies": (C.c_int, [_vp, _dp, C.c_uint32, C.POINTER(C.c_uint32)]),
     "shci_memory": (C.c_int, [_vp, _u64p]),
     "shci_layout_stats": (C.c_int, [_vp, _u64p]),
     "shci_segment_ranges": (C.c_int, [_vp, _u64p]),

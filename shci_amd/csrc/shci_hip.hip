@@ -143,6 +143,12 @@ struct shci_ctx {
   // cost more than the kernels).  Freed on memory pressure, clear and destroy.
   std::vector<DevBuf> cache;
 
+  // Ritz-vector capture (shci_capture_ritz): the ground-state Ritz vector of
+  // every Rayleigh-Ritz step of the latest Davidson call, widened to FP64, in
+  // internal order.
+  DevBuf cap;
+  uint32_t cap_max = 0, cap_n = 0;
+
   hipEvent_t ev[9] = {};
   float phase_ms[8] = {};
 };
@@ -187,7 +193,11 @@ void cache_trim(shci_ctx* ctx) {
 
 // b holds at least `bytes`: kept if large enough, else replaced by the
 // smallest cached block of bytes .. 2 * bytes + 64 MiB, else a new block (the
-// cache is freed and the allocation retried once if memory runs out).
+// cache is freed and the allocation retried once if memory runs out).  A new
+// block of 1 GiB or more gets 1/16 headroom when memory allows: the builds of
+// a growing det list ask for slightly larger stores, sort buffers and tables
+// each time, and a reallocated block costs more than the build itself
+// (hipMalloc maps its pages: seconds for the ~100 GB of a 1e7-det build).
 int ensure(shci_ctx* ctx, DevBuf& b, size_t bytes) {
   if (bytes <= b.bytes && b.p) return SHCI_OK;
   release(ctx, b);
@@ -203,19 +213,33 @@ int ensure(shci_ctx* ctx, DevBuf& b, size_t bytes) {
     ctx->cache.erase(ctx->cache.begin() + best);
     return SHCI_OK;
   }
-  hipError_t e = hipMalloc(&b.p, bytes);
+  constexpr size_t kGranule = 2ull << 20;
+  size_t want = bytes;
+  if (bytes >= (1ull << 30)) {
+    // headroom only while an eighth of the device stays free after it
+    size_t free_b = 0, total_b = 0;
+    const size_t padded = (bytes + bytes / 16 + kGranule - 1) & ~(kGranule - 1);
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > padded + total_b / 8) want = padded;
+    (void)hipGetLastError();
+  }
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e == hipErrorOutOfMemory && want > bytes) {
+    (void)hipGetLastError();
+    want = bytes;
+    e = hipMalloc(&b.p, want);
+  }
   if (e == hipErrorOutOfMemory && !ctx->cache.empty()) {
     (void)hipGetLastError();
     cache_trim(ctx);
-    e = hipMalloc(&b.p, bytes);
+    e = hipMalloc(&b.p, want);
   }
   if (e != hipSuccess) {
     b.p = nullptr;
     return fail(ctx, e == hipErrorOutOfMemory ? SHCI_ERR_OOM : SHCI_ERR_HIP,
                 std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
   }
-  b.bytes = bytes;
-  ctx->mem_in_use += bytes;
+  b.bytes = want;
+  ctx->mem_in_use += want;
   return SHCI_OK;
 }
 
@@ -3780,6 +3804,17 @@ struct Davidson {
       (*n_traj)++;
     }
   }
+  // A Rayleigh-Ritz step's estimate: recorded, and its ground-state Ritz
+  // vector captured when capture is on (after the timestamp; the widening
+  // copy is queued on the stream).
+  void record_ritz(double e) {
+    record(e);
+    if (ctx->cap_n < ctx->cap_max) {
+      double* dst = P<double>(ctx->cap) + (size_t)ctx->cap_n * ((n + 63) & ~(uint64_t)63);
+      k_copy_scale<T, double><<<div_up(n, 256), 256, 0, ctx->stream>>>(w[0], dst, 1.0, n);
+      ctx->cap_n++;
+    }
+  }
 
   // init: ns device vectors (internal order, already normalised/orthogonal).
   int run(double tol) {
@@ -3805,7 +3840,7 @@ struct Davidson {
       HIPC(hipMemcpyAsync(w[s], v[s], n * sizeof(T), hipMemcpyDeviceToDevice, st));
       HIPC(hipMemcpyAsync(hw[s], hv[s], n * sizeof(T), hipMemcpyDeviceToDevice, st));
     }
-    record(lowest[0] + report_shift);
+    record_ritz(lowest[0] + report_shift);
     lowest_prev = lowest;
     size_t i_state_precond = 0;
     // Products of a round's corrections wait for its Rayleigh-Ritz step: a
@@ -3883,7 +3918,7 @@ struct Davidson {
           HIPC(hipMemcpyAsync(w[s], tmp, n * sizeof(T), hipMemcpyDeviceToDevice, st));
         }
         HIPC(hipStreamSynchronize(st));
-        record(lowest[0] + report_shift);
+        record_ritz(lowest[0] + report_shift);
         for (uint32_t s = (uint32_t)n_converged; s < ns; s++) {
           if (std::fabs(lowest[s] - lowest_prev[s]) > tol) break;
           n_converged++;
@@ -5144,6 +5179,8 @@ static int davidson_full(shci_ctx* ctx, uint32_t n_states, const double* init, d
   int nt_local = 0;
   if (!n_traj) n_traj = &nt_local;
   *n_traj = 0;
+  ctx->cap_n = 0;
+  if (ctx->cap_max) CHECK_RC(ensure(ctx, ctx->cap, (size_t)ctx->cap_max * ((n + 63) & ~(uint64_t)63) * 8));
   const auto t0 = std::chrono::steady_clock::now();
   if (max_calls == 0) max_calls = 1;
   if (n == 1) {  // davidson.cc:20-26
@@ -5539,6 +5576,43 @@ int shci_time_spmv(shci_ctx* ctx, uint32_t flags, int reps, float* ms_avg, doubl
       // values + columns, row start/end (int64 each), FP64 diagonal, x once, y once.
       *bytes = F * (vs + 4.0) + (double)n * 16.0 + (double)n * 8.0 + 2.0 * (double)n * xs;
     }
+  }
+  return SHCI_OK;
+}
+
+int shci_capture_ritz(shci_ctx* ctx, uint32_t max_vectors) {
+  if (!ctx) return SHCI_ERR_ARG;
+  HIPC(hipSetDevice(ctx->device));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  ctx->cap_max = max_vectors;
+  ctx->cap_n = 0;
+  if (!max_vectors) release(ctx, ctx->cap);
+  return SHCI_OK;
+}
+
+int shci_ritz_energies(shci_ctx* ctx, double* e, uint32_t cap, uint32_t* n_captured) {
+  if (!ctx) return SHCI_ERR_ARG;
+  if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
+  if (!has_fp64(ctx)) return fail(ctx, SHCI_ERR_STATE, "the FP64 store is needed for the FP64 energies");
+  HIPC(hipSetDevice(ctx->device));
+  const uint64_t n = ctx->n;
+  const size_t nst = (n + 63) & ~(uint64_t)63;
+  if (ctx->cap_n && ctx->cap.bytes < (size_t)ctx->cap_n * nst * 8)
+    return fail(ctx, SHCI_ERR_STATE, "the captured vectors belong to a smaller determinant list");
+  if (n_captured) *n_captured = ctx->cap_n;
+  const uint32_t m = std::min(cap, ctx->cap_n);
+  if (m == 0) return SHCI_OK;
+  if (!e) return fail(ctx, SHCI_ERR_ARG, "no output array");
+  CHECK_RC(ensure(ctx, ctx->yv, n * 8));
+  for (uint32_t k = 0; k < m; k++) {
+    const double* x = P<double>(ctx->cap) + (size_t)k * nst;
+    CHECK_RC(launch_spmv<double, double, double>(ctx, P<double>(ctx->v64), 0.0, x, P<double>(ctx->yv)));
+    const double* b1[1] = {P<double>(ctx->yv)};
+    const double* b2[1] = {x};
+    double num, den;
+    CHECK_RC(dots<double>(ctx, x, b1, 1, &num));
+    CHECK_RC(dots<double>(ctx, x, b2, 1, &den));
+    e[k] = num / den;
   }
   return SHCI_OK;
 }

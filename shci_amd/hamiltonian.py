@@ -181,6 +181,20 @@ class SparseMatrix:
                        "shci_time_spmm")
         return ms.value, by.value
 
+    def capture_ritz(self, max_vectors: int):
+        """Keep the ground-state Ritz vector behind each trajectory entry of
+        the following Davidson calls (0 stops and frees)."""
+        self.ctx.check(self.ctx.lib.shci_capture_ritz(self.ctx.handle, int(max_vectors)), "shci_capture_ritz")
+
+    def ritz_energies(self, cap: int = 4096):
+        """FP64 energies <w|H|w>/<w|w> of the captured Ritz vectors of the
+        latest Davidson call; needs the FP64 store over the same dets."""
+        e = np.zeros(cap)
+        m = C.c_uint32()
+        self.ctx.check(self.ctx.lib.shci_ritz_energies(self.ctx.handle, e.ctypes.data_as(_dp), cap, C.byref(m)),
+                       "shci_ritz_energies")
+        return e[: min(cap, m.value)].copy()
+
     def layout_stats(self):
         st = (C.c_uint64 * 14)()
         self.ctx.check(self.ctx.lib.shci_layout_stats(self.ctx.handle, st), "shci_layout_stats")

@@ -15,7 +15,7 @@ from .hamiltonian import Davidson, Hamiltonian
 from .host import HostSystem
 from .selection import GpuSystem
 
-__all__ = ["Workload", "grow_det_list", "SCHEDULES"]
+__all__ = ["Workload", "grow_det_list", "det_hash", "SCHEDULES"]
 
 # eps sweeps per config: every eps is applied once, the last one until the det
 # count grows by less than 1% (at most 3 sweeps) or the target is reached.
@@ -40,6 +40,15 @@ class Workload:
         self.log = []
 
 
+def det_hash(up, dn) -> str:
+    """Digest of a det list in its order (run-to-run reproducibility)."""
+    import hashlib
+
+    h = hashlib.sha256(np.ascontiguousarray(up, dtype=np.uint64).tobytes())
+    h.update(np.ascontiguousarray(dn, dtype=np.uint64).tobytes())
+    return h.hexdigest()[:16]
+
+
 def grow_det_list(cfg: int, target: int | None = None, device: int = 0, verbose: bool = True,
                   schedule=None, max_final_sweeps: int = 3, selection: str = "device") -> Workload:
     """Grow the variational space of config ``cfg`` until ``target`` dets.
@@ -50,11 +59,16 @@ def grow_det_list(cfg: int, target: int | None = None, device: int = 0, verbose:
     target = target or spec.target_dets
     schedule = schedule or SCHEDULES[cfg]
     host = HostSystem.from_arrays(w.n, spec.n_up, spec.n_dn, w.h1, w.eri, w.core, n_states=spec.n_states)
-    # The CSR path sums rows in a fixed order, so the grown list is
-    # reproducible; the ~1e7-det configs need the segment layout's memory
-    # (its atomic row sums may move a borderline det between runs).
+    # The selection compares |H_ij c_i| with eps, so the coefficients that
+    # drive it must not depend on the summation order of the products.  The
+    # CSR sums rows in a fixed order; the ~1e7-det configs need the segment
+    # layout's memory, whose products add member sums with atomics.  There
+    # the Davidson runs in FP64 (values and vectors): the order changes the
+    # coefficients by ~1e-16 relative, and no |H c| lies that close to eps
+    # (with FP32 vectors, ~1e-7, a few borderline dets moved between runs).
     layout = "csr" if cfg in (1, 2, 4) else "segments"
-    ham = Hamiltonian(w.n, w.h1, w.eri, w.core, device=device, store="fp32", layout=layout)
+    store, mode = ("fp32", "mixed") if layout == "csr" else ("fp64", "fp64")
+    ham = Hamiltonian(w.n, w.h1, w.eri, w.core, device=device, store=store, layout=layout)
     if selection == "device":
         host = GpuSystem(ham, host.hf_up, host.hf_dn)
     dav = Davidson(spec.n_states)
@@ -80,7 +94,7 @@ def grow_det_list(cfg: int, target: int | None = None, device: int = 0, verbose:
             ham.update(up, dn)
             t_build = time.time() - t0
             t0 = time.time()
-            dav.diagonalize(ham.matrix, coefs, 1e-6, mode="mixed", max_calls=2)
+            dav.diagonalize(ham.matrix, coefs, 1e-6, mode=mode, max_calls=2)
             t_dav = time.time() - t0
             coefs = np.atleast_2d(dav.get_lowest_eigenvectors())
             _, _, nel = ham.matrix.counts()
@@ -97,6 +111,7 @@ def grow_det_list(cfg: int, target: int | None = None, device: int = 0, verbose:
             break
     w.up, w.dn = host.dets()
     w.coefs = coefs
+    w.det_hash = det_hash(w.up, w.dn)
     ham.clear()
     del ham
     if verbose:

@@ -327,3 +327,41 @@ def test_multiroot_davidson_vs_dense(system, layout):
     r1o, r2o = s["orc"].rdm(c)
     np.testing.assert_allclose(r1g, r1o, rtol=0, atol=1e-11)
     np.testing.assert_allclose(r2g, r2o, rtol=0, atol=1e-11)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("layout", ["csr", "segments"])
+def test_ritz_capture_fp64_energies(system, layout):
+    """The captured Ritz vectors (shci_capture_ritz) and their FP64 energies
+    (shci_ritz_energies): in FP64 mode each trajectory entry is a Rayleigh
+    quotient of the FP64 matrix, so the captured vectors' FP64 energies
+    reproduce the trajectory; in mixed mode they are variational (never below
+    the dense ground state) and the last lies within 1 uHa of it."""
+    from shci_amd.hamiltonian import Davidson, Hamiltonian
+
+    s = system
+    n = len(s["up"])
+    ham = Hamiltonian(s["n"], s["h1"], s["eri"], 0.0, store="both", layout=layout)
+    ham.update(s["up"], s["dn"])
+    init = np.zeros(n)
+    init[0] = 1.0
+    e0, vec0, _, _, _ = s["orc"].davidson(init[None], 1e-12)
+    for _ in range(30):
+        e1, vec0, conv, _, _ = s["orc"].davidson(vec0, 1e-12)
+        if abs(e1[0] - e0[0]) < 1e-13:
+            break
+        e0 = e1
+    ham.matrix.capture_ritz(64)
+    d = Davidson(1)
+    d.diagonalize(ham.matrix, init, 1e-9, mode="fp64", max_calls=4)
+    e = ham.matrix.ritz_energies()
+    assert len(e) == min(64, len(d.trajectory))
+    np.testing.assert_allclose(e, d.trajectory[: len(e)], rtol=0, atol=1e-10)
+    dm = Davidson(1)
+    dm.diagonalize(ham.matrix, init, 1e-9, mode="mixed", max_calls=4)
+    em = ham.matrix.ritz_energies()
+    ham.matrix.capture_ritz(0)
+    assert len(em) == min(64, len(dm.trajectory) - 1)  # the final re-evaluation is not a Ritz step
+    assert np.all(em >= e1[0] - 1e-9)
+    assert abs(em[-1] - e1[0]) <= 1e-6
+    np.testing.assert_allclose(em, dm.trajectory[: len(em)], rtol=0, atol=2e-5 * max(1.0, abs(e1[0])))

@@ -91,3 +91,34 @@ def test_update_existing_elems_matches_rebuild():
         for k in range(rp[i], rp[i + 1]):
             j = cols[k]
             assert vals[k] == a.elem(int(up[i]), int(dn[i]), int(up[j]), int(dn[j]), -1 if i != j else 0)
+
+
+def test_upper_digest_detects_plausible_mistakes():
+    """oracle.upper_digest (the order-free comparison of large exports):
+    equal under any within-row order, different after a one-ulp value
+    change, a sign flip, a changed column, an entry moved to another row
+    or a dropped entry."""
+    rng = np.random.default_rng(5)
+    n = 50
+    rows = [np.sort(rng.choice(np.arange(i, n), size=min(n - i, 6), replace=False)) for i in range(n)]
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int64)
+    cols = np.concatenate(rows).astype(np.int64)
+    vals = rng.normal(size=len(cols))
+    d0 = oracle.upper_digest(rp, cols, vals)
+    perm = np.concatenate([rp[i] + rng.permutation(rp[i + 1] - rp[i]) for i in range(n)])
+    assert oracle.upper_digest(rp, cols[perm], vals[perm]) == d0
+    v1 = vals.copy()
+    v1[17] = np.nextafter(v1[17], np.inf)
+    assert oracle.upper_digest(rp, cols, v1) != d0
+    v2 = vals.copy()
+    v2[3] = -v2[3]
+    assert oracle.upper_digest(rp, cols, v2) != d0
+    c1 = cols.copy()
+    c1[40] += 1
+    assert oracle.upper_digest(rp, c1, vals) != d0
+    rp1 = rp.copy()
+    rp1[5] += 1  # the first entry of row 5 now belongs to row 4
+    assert oracle.upper_digest(rp1, cols, vals) != d0
+    rp2 = rp.copy()
+    rp2[-1] -= 1
+    assert oracle.upper_digest(rp2, cols[:-1], vals[:-1]) != d0
