@@ -240,6 +240,11 @@ int shci_layout_stats(shci_ctx* ctx, uint64_t* stats);
  * out[3 * r] = segments, out[3 * r + 1] = memberships, out[3 * r + 2] =
  * value slots.  SHCI_ERR_STATE unless the segment layout is built. */
 int shci_segment_ranges(shci_ctx* ctx, uint64_t* out);
+/* Segment-layout kept pairs (no reference counterpart; diagnostics): out[r]
+ * = pairs of range r (as above) that its segments own and keep (|H| >= 1e-12
+ * or SHCI_KEEP_ZEROS), as counted by the last fill; the range's other value
+ * slots hold zeros. */
+int shci_segment_kept(shci_ctx* ctx, uint64_t* out);
 
 /* Device memory in use by the context (bytes). */
 int shci_memory(shci_ctx* ctx, uint64_t* bytes);

@@ -53,6 +53,7 @@ EXPORTS = {
     "shci_memory": (C.c_int, [_vp, _u64p]),
     "shci_layout_stats": (C.c_int, [_vp, _u64p]),
     "shci_segment_ranges": (C.c_int, [_vp, _u64p]),
+    "shci_segment_kept": (C.c_int, [_vp, _u64p]),
 }
 
 

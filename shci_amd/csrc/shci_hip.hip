@@ -107,6 +107,7 @@ struct shci_ctx {
   DevBuf mA_id, mB_id;                  // [group * C(n_e, 2) + pair] -> minus-two string id
   uint64_t n_seg = 0, n_mem = 0, n_slots = 0, n_pairs = 0;
   uint64_t seg_cs[28] = {};             // segment ranges: size class x type (C, A, B)
+  uint64_t seg_kept[27] = {};           // kept (owned, |H| >= cut) pairs of each range, last fill
   uint32_t seg_maxm[27] = {};           // largest segment of each range
   DevBuf seg_desc, seg_type, seg_core, seg_mem, sv32, sv64;
   DevBuf dnpos;                         // internal det -> position in beta order
@@ -3313,11 +3314,14 @@ SegFillParams seg_fill_params(shci_ctx* ctx, int update) {
 int seg_fill(shci_ctx* ctx, int update) {
   const hipStream_t st = ctx->stream;
   CHECK_RC(ensure(ctx, ctx->scal, 64 * 8));
-  CHECK_RC(ensure(ctx, ctx->kept, kKeptLines * 128));
+  // kept-pair counters: kKeptLines lines per range
+  CHECK_RC(ensure(ctx, ctx->kept, (size_t)kSegRanges * kKeptLines * 128));
   SegFillParams F = seg_fill_params(ctx, update);
-  HIPC(hipMemsetAsync(F.kept, 0, kKeptLines * 128, st));
+  unsigned long long* const kept0 = F.kept;
+  HIPC(hipMemsetAsync(kept0, 0, (size_t)kSegRanges * kKeptLines * 128, st));
   for (int r = 0; r < kSegRanges; r++) {
     const int c = r / 3;
+    F.kept = kept0 + (size_t)r * kKeptLines * 16;
     F.s0 = (uint32_t)ctx->seg_cs[r];
     F.s1 = (uint32_t)ctx->seg_cs[r + 1];
     const uint32_t cnt = F.s1 - F.s0;
@@ -3352,10 +3356,15 @@ int seg_fill(shci_ctx* ctx, int update) {
     }
   }
   HIPC(hipGetLastError());
-  std::vector<unsigned long long> lines((size_t)kKeptLines * 16);
-  CHECK_RC(copy_back(ctx, lines.data(), F.kept, lines.size()));
+  std::vector<unsigned long long> lines((size_t)kSegRanges * kKeptLines * 16);
+  CHECK_RC(copy_back(ctx, lines.data(), kept0, lines.size()));
   unsigned long long kept = 0;
-  for (uint32_t i = 0; i < kKeptLines; i++) kept += lines[(size_t)i * 16];
+  for (int r = 0; r < kSegRanges; r++) {
+    unsigned long long kr = 0;
+    for (uint32_t i = 0; i < kKeptLines; i++) kr += lines[((size_t)r * kKeptLines + i) * 16];
+    ctx->seg_kept[r] = kr;
+    kept += kr;
+  }
   if (!update) ctx->n_pairs = kept;
   return SHCI_OK;
 }
@@ -5681,6 +5690,13 @@ int shci_layout_stats(shci_ctx* ctx, uint64_t* stats) {
     CHECK_RC(copy_back(ctx, &v, out));
     stats[13] = v;
   }
+  return SHCI_OK;
+}
+
+int shci_segment_kept(shci_ctx* ctx, uint64_t* out) {
+  if (!ctx || !out) return SHCI_ERR_ARG;
+  if (!ctx->segl || !ctx->seg_desc.p) return fail(ctx, SHCI_ERR_STATE, "segment layout not built");
+  for (int r = 0; r < kSegRanges; r++) out[r] = ctx->seg_kept[r];
   return SHCI_OK;
 }
 

@@ -208,8 +208,10 @@ class SparseMatrix:
         > 256, > 64, 33-64, 17-32, 9-16, 5-8, 3-4, 2; type C, A, B."""
         out = (C.c_uint64 * 81)()
         self.ctx.check(self.ctx.lib.shci_segment_ranges(self.ctx.handle, out), "shci_segment_ranges")
+        kept = (C.c_uint64 * 27)()
+        self.ctx.check(self.ctx.lib.shci_segment_kept(self.ctx.handle, kept), "shci_segment_kept")
         return [{"class": r // 3, "type": "CAB"[r % 3], "segments": out[3 * r], "memberships": out[3 * r + 1],
-                 "slots": out[3 * r + 2]} for r in range(27)]
+                 "slots": out[3 * r + 2], "kept_pairs": kept[r]} for r in range(27)]
 
     def build_timings(self):
         t = (C.c_float * 8)()

@@ -188,6 +188,10 @@ def test_segments_counts_and_update_existing(system, keep_zeros):
             assert sum(r["segments"] for r in rg) == st["segments"]
             assert sum(r["memberships"] for r in rg) == st["memberships"]
             assert sum(r["slots"] for r in rg) == st["value_slots"]
+            # the kept pairs of the ranges are the stored pairs (the product
+            # and the oracle's count agree on those), each in a slot of its range
+            assert 2 * sum(r["kept_pairs"] for r in rg) + len(s["up"]) == ys[layout + "_nnz"]
+            assert all(r["kept_pairs"] <= r["slots"] for r in rg)
             lo = [1025, 257, 65, 33, 17, 9, 5, 3, 2]
             hi = [4096, 1024, 256, 64, 32, 16, 8, 4, 2]
             for r in rg:
