@@ -1346,19 +1346,27 @@ static double pt_dtm(const Integrals& I, const Chem& chem, const std::vector<Det
     if (x.a.dn != y.a.dn) return x.a.dn < y.a.dn;
     return x.i < y.i;
   });
-  double e = 0.0;
+  // The per-det sums add their terms in increasing i (the order the device
+  // keeps, so each contribution is bit-identical); the total over the PT
+  // dets is a compensated (Neumaier) sum, so that a comparison of totals over
+  // millions of same-sign contributions measures the other side's rounding,
+  // not this one's (a plain running sum drifts by ~1e-12 relative there).
+  double e = 0.0, comp = 0.0;
   size_t np = 0;
   for (size_t k = 0; k < all_terms.size();) {
     size_t k2 = k;
     double s = 0.0;
     while (k2 < all_terms.size() && all_terms[k2].a == all_terms[k].a) s += all_terms[k2++].hc;
     const double haa = chem.elem(all_terms[k].a, all_terms[k].a, 0);
-    e += s * s / (energy_var - haa);
+    const double t = s * s / (energy_var - haa);
+    const double u = e + t;
+    comp += std::fabs(e) >= std::fabs(t) ? (e - u) + t : (t - u) + e;
+    e = u;
     np++;
     k = k2;
   }
   *n_pt_dets = np;
-  return e;
+  return e + comp;
 }
 
 struct OrcSystem {

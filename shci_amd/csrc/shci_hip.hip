@@ -217,10 +217,19 @@ int ensure(shci_ctx* ctx, DevBuf& b, size_t bytes) {
   constexpr size_t kGranule = 2ull << 20;
   size_t want = bytes;
   if (bytes >= (1ull << 30)) {
-    // headroom only while an eighth of the device stays free after it
-    size_t free_b = 0, total_b = 0;
+    // headroom only while an eighth of the device stays free after it (the
+    // cached blocks are freed first when that makes the room)
+    size_t free_b = 0, total_b = 0, cached = 0;
+    for (const DevBuf& c : ctx->cache) cached += c.bytes;
     const size_t padded = (bytes + bytes / 16 + kGranule - 1) & ~(kGranule - 1);
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > padded + total_b / 8) want = padded;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+      if (free_b > padded + total_b / 8) {
+        want = padded;
+      } else if (free_b + cached > padded + total_b / 8) {
+        cache_trim(ctx);
+        want = padded;
+      }
+    }
     (void)hipGetLastError();
   }
   hipError_t e = hipMalloc(&b.p, want);
@@ -3542,16 +3551,11 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
     ctx->rb = std::min(ctx->n_dn, 2u);
     CHECK_RC(build_seg_layout(ctx));
     HIPC(hipEventRecord(ctx->ev[4], st));
-    if (flags & SHCI_STORE_FP64) {
-      CHECK_RC(ensure(ctx, ctx->sv64, ctx->n_slots * 8));
-    } else {
-      release(ctx, ctx->sv64);
-    }
-    if (flags & SHCI_STORE_FP32) {
-      CHECK_RC(ensure(ctx, ctx->sv32, ctx->n_slots * 4));
-    } else {
-      release(ctx, ctx->sv32);
-    }
+    // drop the store that is not asked for before allocating the other
+    if (!(flags & SHCI_STORE_FP64)) release(ctx, ctx->sv64);
+    if (!(flags & SHCI_STORE_FP32)) release(ctx, ctx->sv32);
+    if (flags & SHCI_STORE_FP64) CHECK_RC(ensure(ctx, ctx->sv64, ctx->n_slots * 8));
+    if (flags & SHCI_STORE_FP32) CHECK_RC(ensure(ctx, ctx->sv32, ctx->n_slots * 4));
     HIPC(hipEventRecord(ctx->ev[5], st));
     CHECK_RC(seg_fill(ctx, 0));
     HIPC(hipEventRecord(ctx->ev[6], st));
@@ -3600,18 +3604,12 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
   // drops entries keep a short row_end (no compaction, no reallocation).
   const uint64_t cap = std::max<int64_t>(total, 1);
   ctx->nnz_alloc = (uint64_t)total;
+  if (!(flags & SHCI_STORE_FP64)) release(ctx, ctx->v64);
+  if (!(flags & SHCI_STORE_FP32)) release(ctx, ctx->v32);
   CHECK_RC(ensure(ctx, ctx->row_end, n * 8));
   CHECK_RC(ensure(ctx, ctx->col, cap * 4));
-  if (flags & SHCI_STORE_FP64) {
-    CHECK_RC(ensure(ctx, ctx->v64, cap * 8));
-  } else {
-    release(ctx, ctx->v64);
-  }
-  if (flags & SHCI_STORE_FP32) {
-    CHECK_RC(ensure(ctx, ctx->v32, cap * 4));
-  } else {
-    release(ctx, ctx->v32);
-  }
+  if (flags & SHCI_STORE_FP64) CHECK_RC(ensure(ctx, ctx->v64, cap * 8));
+  if (flags & SHCI_STORE_FP32) CHECK_RC(ensure(ctx, ctx->v32, cap * 4));
   HIPC(hipEventRecord(ctx->ev[5], st));
   // Pass 2: fill.
   {
@@ -5510,9 +5508,10 @@ int shci_clear(shci_ctx* ctx) {
                     &ctx->tile_u0, &ctx->tile_nu, &ctx->tile_tab, &ctx->seg_loc,
                     &ctx->cnt, &ctx->row_ptr, &ctx->row_end, &ctx->row_end, &ctx->col, &ctx->v64,
                     &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb, &ctx->va, &ctx->vb, &ctx->xv,
-                    &ctx->yv, &ctx->x32, &ctx->y32, &ctx->dav};
+                    &ctx->yv, &ctx->x32, &ctx->y32, &ctx->dav, &ctx->cap};
   for (DevBuf* b : bufs) release(ctx, *b);
   cache_trim(ctx);
+  ctx->cap_n = 0;  // the captured vectors belonged to the dropped det list
   ctx->n = 0;
   ctx->nnz = 0;
   ctx->n_seg = ctx->n_mem = ctx->n_slots = ctx->n_pairs = ctx->n_tiles = 0;
