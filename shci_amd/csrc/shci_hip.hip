@@ -1341,6 +1341,9 @@ __global__ __launch_bounds__(256) void k_export_fill(uint64_t n, const int64_t* 
 constexpr uint32_t kSegMaxM = 4096;  // C: (n_orb-n_up+1)(n_orb-n_dn+1) <= 63*63; A, B: C(n_orb-n+2, 2) <= 2016
 constexpr int kSegClasses = 9;       // m > 1024, > 256, > 64 (one wavefront each); lane groups of 64 .. 2
 constexpr int kSegRanges = kSegClasses * 3;  // x segment type; range index = class * 3 + type
+// Zero slots after the last value slot: the lane-group product reads a few
+// rows past a segment's last step instead of clamping every address.
+constexpr uint64_t kSlotPad = 4096;
 
 enum : uint32_t { kSegC = 0, kSegA = 1, kSegB = 2 };
 
@@ -2194,49 +2197,52 @@ __device__ __forceinline__ double lane_read(double v, int a) {
 // the segment per step (lane j reads lane j + 1 mod m, a fixed address), so
 // after T steps lane j's R holds what it owes member j + T mod m; one last
 // cross-lane read delivers it.  The last step of an even m (m/2 slots, the pair
-// (j, j + m/2) read by both of its lanes) adds the forward term only.  The step
-// loop runs for `steps` steps, the most of any segment of the wave (wave
-// uniform); lanes past their own steps re-read their last slot and add zero.
+// (j, j + m/2) read by both of its lanes) adds the forward term only.
+//
+// Segment b has mm[b] members and its slots start at base + soff[b] (base
+// wave-uniform, offsets 32-bit, so each load is a scalar base plus one vector
+// offset that one addition advances per step).  The step loop runs for
+// `steps` steps, the most of any segment of the wave (wave uniform); the first
+// `steps_all` steps belong to every segment of the wave and run without
+// masks; past them a lane whose segment has no such step adds zero.  Loads
+// past a segment's last step stay inside the store, which ends with
+// kSlotPad zero slots; lanes past m hold x = 0 and add nothing.
 // All lanes of the wave must call this.
 template <int S, int NB, int CH, typename VT, typename XT>
-__device__ __forceinline__ void seg_group_sums(const SegDesc (&D)[NB], const XT (&xj)[NB], const VT* __restrict__ val,
-                                               uint32_t steps, XT (&out)[NB]) {
+__device__ __forceinline__ void seg_group_sums(const uint32_t (&mm)[NB], const uint32_t (&soff)[NB], const XT (&xj)[NB],
+                                               const VT* __restrict__ base, uint32_t steps, uint32_t steps_all,
+                                               XT (&out)[NB]) {
   // CH: steps per batch of value loads
   using gptr = const __attribute__((address_space(1))) VT*;  // global loads, not flat
+  using gbyte = const __attribute__((address_space(1))) char*;
+  const gptr vb = (gptr)base;
+  const gbyte bb = (gbyte)base;
+  // a slot at byte offset o (32-bit): one scalar base plus one vector offset
+  auto at = [&](uint32_t o) { return __builtin_nontemporal_load((gptr)(bb + o)); };
   const uint32_t j = threadIdx.x & (S - 1);
   const uint32_t gbase = lane_id() & ~(uint32_t)(S - 1);
-  uint32_t full[NB];
+  uint32_t full[NB], off[NB], stride[NB];
   int nxt[NB];
-  gptr vp[NB];
   XT xp[NB], acc[NB], R[NB];
+  VT wh[NB];
 #pragma unroll
   for (int b = 0; b < NB; b++) {
-    const uint32_t m = D[b].m;
+    const uint32_t m = mm[b], mh = m >> 1;
     const bool valid = j < m;
     full[b] = valid ? (m - 1) / 2 : 0;  // steps with m slots
     const uint32_t jn = j + 1 == m ? 0 : j + 1;
     nxt[b] = (int)((gbase + (valid ? jn : j)) << 2);
-    vp[b] = (gptr)(val + D[b].voff + (full[b] ? j : 0));
+    off[b] = (soff[b] + j) * (uint32_t)sizeof(VT);
+    stride[b] = m * (uint32_t)sizeof(VT);
     xp[b] = xj[b];
     acc[b] = (XT)0;
     R[b] = (XT)0;
+    // the half-step value of an even m, loaded ahead of the steps
+    const bool half = valid && (m & 1u) == 0;
+    const uint32_t hj = j < mh ? j : j - mh;
+    wh[b] = vb[soff[b] + (half ? (mh - 1) * m + hj : 0u)];
   }
-  uint32_t T = 0;  // rotations done
-#pragma unroll 1
-  for (uint32_t c0 = 1; c0 <= steps; c0 += CH, T += CH) {
-    XT v[NB][CH];
-#pragma unroll
-    for (int b = 0; b < NB; b++) {
-      bool live = c0 <= full[b];
-#pragma unroll
-      for (int i = 0; i < CH; i++) {
-        const VT w = __builtin_nontemporal_load(vp[b]);
-        const bool next = c0 + i + 1 <= full[b];
-        v[b][i] = live ? (XT)w : (XT)0;
-        vp[b] += next ? D[b].m : 0u;  // stays on the last slot read
-        live = next;
-      }
-    }
+  auto apply = [&](const XT (&v)[NB][CH]) {
 #pragma unroll
     for (int i = 0; i < CH; i++) {
 #pragma unroll
@@ -2247,17 +2253,46 @@ __device__ __forceinline__ void seg_group_sums(const SegDesc (&D)[NB], const XT 
         R[b] = fma(v[b][i], xj[b], R[b]);
       }
     }
+  };
+  uint32_t T = 0;  // rotations done
+  uint32_t c0 = 1;
+#pragma unroll 1
+  for (; c0 + CH - 1 <= steps_all; c0 += CH, T += CH) {
+    XT v[NB][CH];
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+      for (int i = 0; i < CH; i++) {
+        v[b][i] = (XT)at(off[b]);
+        off[b] += stride[b];
+      }
+    apply(v);
+  }
+#pragma unroll 1
+  for (; c0 <= steps; c0 += CH, T += CH) {
+    XT v[NB][CH];
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+      for (int i = 0; i < CH; i++) {
+        const VT w = at(off[b]);
+        off[b] += stride[b];
+        if constexpr (sizeof(VT) == sizeof(XT)) {
+          v[b][i] = c0 + i <= full[b] ? w : (VT)0;
+        } else {
+          v[b][i] = keep_if<XT>(w, c0 + i <= full[b]);
+        }
+      }
+    apply(v);
   }
 #pragma unroll
   for (int b = 0; b < NB; b++) {
-    const uint32_t mb = D[b].m, mh = mb >> 1;
+    const uint32_t mb = mm[b], mh = mb >> 1;
     const bool valid = j < mb;
     const bool half = valid && (mb & 1u) == 0;
-    const uint32_t hj = j < mh ? j : j - mh;
-    const VT w = val[D[b].voff + (half ? (uint64_t)(mh - 1) * mb + hj : 0ull)];
     const uint32_t pj = j + mh >= mb ? j + mh - mb : j + mh;
     const XT xh = lane_read(xj[b], (int)((gbase + (valid ? pj : j)) << 2));
-    acc[b] = fma(keep_if<XT>(w, half), xh, acc[b]);
+    acc[b] = fma(keep_if<XT>(wh[b], half), xh, acc[b]);
     // (j - T) mod m; T < 2m for the segments of a lane-group class: T is
     // (S/2 - 1) rounded up to a multiple of CH <= S at most, and m > S/2
     // (or m = S = 2, T = 0)
@@ -2298,25 +2333,31 @@ __device__ __forceinline__ void tile_class_run(uint32_t n, uint32_t k0, uint32_t
     const uint32_t sw = r0 + wv * PW;
     if (sw >= n) break;
     const uint32_t steps = ((L.sm[k0 + sw] & 0xFFu) - 1) / 2;
-    SegDesc D[NB];
+    // the wave's smallest segment (its last) bounds the steps all of its segments have
+    const uint32_t slast = sw + PW - 1;
+    const uint32_t steps_all = slast < n ? ((L.sm[k0 + slast] & 0xFFu) - 1) / 2 : 0u;
+    uint32_t mm[NB], so[NB];
     XT xj[NB], out[NB];
     uint32_t l[NB];
 #pragma unroll
     for (int k = 0; k < NB; k++) {
       const uint32_t sk = sw + k * GW + gl;
-      D[k] = SegDesc{0u, 0u, 0ull};
+      mm[k] = so[k] = 0;
+      uint32_t moff = 0;
       if (sk < n) {
         const uint32_t pk = L.sm[k0 + sk];
-        D[k] = SegDesc{m0 + (pk >> 8), pk & 0xFFu, voff0 + L.sv[k0 + sk]};
+        mm[k] = pk & 0xFFu;
+        moff = m0 + (pk >> 8);
+        so[k] = L.sv[k0 + sk];
       }
-      const bool valid = j < D[k].m;
-      l[k] = valid ? L.sl[D[k].moff + j] : 0u;
+      const bool valid = j < mm[k];
+      l[k] = valid ? L.sl[moff + j] : 0u;
       xj[k] = valid ? L.xs[l[k]] : (XT)0;
     }
-    seg_group_sums<S, NB, CH, VT, XT>(D, xj, val, steps, out);
+    seg_group_sums<S, NB, CH, VT, XT>(mm, so, xj, val + voff0, steps, steps_all, out);
 #pragma unroll
     for (int k = 0; k < NB; k++)
-      if (j < D[k].m) atomicAdd(&L.ys[l[k]], out[k]);
+      if (j < mm[k]) atomicAdd(&L.ys[l[k]], out[k]);
   }
 }
 
@@ -3554,8 +3595,14 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
     // drop the store that is not asked for before allocating the other
     if (!(flags & SHCI_STORE_FP64)) release(ctx, ctx->sv64);
     if (!(flags & SHCI_STORE_FP32)) release(ctx, ctx->sv32);
-    if (flags & SHCI_STORE_FP64) CHECK_RC(ensure(ctx, ctx->sv64, ctx->n_slots * 8));
-    if (flags & SHCI_STORE_FP32) CHECK_RC(ensure(ctx, ctx->sv32, ctx->n_slots * 4));
+    if (flags & SHCI_STORE_FP64) {
+      CHECK_RC(ensure(ctx, ctx->sv64, (ctx->n_slots + kSlotPad) * 8));
+      HIPC(hipMemsetAsync(P<double>(ctx->sv64) + ctx->n_slots, 0, kSlotPad * 8, st));
+    }
+    if (flags & SHCI_STORE_FP32) {
+      CHECK_RC(ensure(ctx, ctx->sv32, (ctx->n_slots + kSlotPad) * 4));
+      HIPC(hipMemsetAsync(P<float>(ctx->sv32) + ctx->n_slots, 0, kSlotPad * 4, st));
+    }
     HIPC(hipEventRecord(ctx->ev[5], st));
     CHECK_RC(seg_fill(ctx, 0));
     HIPC(hipEventRecord(ctx->ev[6], st));
