@@ -2511,13 +2511,17 @@ __device__ __forceinline__ void seg_large_rows(const SegDesc& D, uint32_t j0, co
 }
 
 // Value of lane l + 1 (lane 63: lane 0), a wave-wide DPP rotate.
+// Every lane is written (full row and bank masks), so the "old" operand is
+// the value itself and the rotate needs no separate initialisation.
 __device__ __forceinline__ float wave_next(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, false));
+  const int i = __float_as_int(v);
+  return __int_as_float(__builtin_amdgcn_update_dpp(i, i, 0x134, 0xF, 0xF, false));
 }
 __device__ __forceinline__ double wave_next(double v) {
   const uint64_t u = (uint64_t)__double_as_longlong(v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x134, 0xF, 0xF, false);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x134, 0xF, 0xF, false);
+  const int l = (int)(uint32_t)u, h = (int)(uint32_t)(u >> 32);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(l, l, 0x134, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(h, h, 0x134, 0xF, 0xF, false);
   return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 __device__ __forceinline__ float lane63(float v) {
