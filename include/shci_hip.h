@@ -124,6 +124,19 @@ int shci_get_diag(shci_ctx* ctx, double* diag);
  * accumulation FP32, diagonal FP64).  SHCI_DEVICE_PTRS: x, y on the device. */
 int shci_spmv(shci_ctx* ctx, const double* x, double* y, uint32_t flags);
 
+/* y = H x like shci_spmv with SHCI_DEVICE_PTRS, allocating nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  x and y are device pointers (caller order, FP64); flags:
+ * SHCI_FP32 selects the FP32 store as in shci_spmv.  The working vectors (x
+ * and y in internal order and the beta-order copies of the A segments) live
+ * in a fixed device array of 256 MiB per GPU, held until the process ends, so
+ * n_dets is at most 8,388,608 (SHCI_ERR_UNSUPPORTED beyond; also under
+ * SHCI_TIME_SYM).  Every call on one GPU uses that GPU's one array: calls that
+ * overlap on a GPU (two contexts, two threads) race.  The matrix must be
+ * built; the call returns when y is written. */
+int shci_spmv_noalloc(shci_ctx* ctx, const double* x, double* y, uint32_t flags);
+
 /* Block product Y = H X for k vectors (k <= 8), X, Y row-major k x n, caller
  * order.  The reference declares a multi-vector mul (sparse_matrix.h:28-32)
  * but never defines it; this is the block form used by the multi-root path. */

@@ -35,6 +35,7 @@ EXPORTS = {
     "shci_get_counts": (C.c_int, [_vp, _u64p, _u64p, _u64p]),
     "shci_get_diag": (C.c_int, [_vp, _dp]),
     "shci_spmv": (C.c_int, [_vp, _vp, _vp, C.c_uint32]),
+    "shci_spmv_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_uint32]),
     "shci_spmm": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_uint32]),
     "shci_davidson": (C.c_int, [_vp, C.c_uint32, _dp, C.c_double, C.c_uint32, C.c_uint32, _dp, _dp, _ip, _dp, _dp,
                                 C.c_int, _ip]),

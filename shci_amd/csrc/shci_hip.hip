@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -149,6 +150,9 @@ struct shci_ctx {
   // internal order.
   DevBuf cap;
   uint32_t cap_max = 0, cap_n = 0;
+  // Device address of this device's copy of the allocation-free product's
+  // scratch array (hipGetSymbolAddress at context creation).
+  unsigned char* noalloc_scratch = nullptr;
 
   hipEvent_t ev[9] = {};
   float phase_ms[8] = {};
@@ -3450,7 +3454,7 @@ __global__ void k_perm_add(const uint32_t* __restrict__ dnord, const XT* __restr
 // y = (H - shift) x on internal-order device vectors, NV interleaved vectors.
 // A segments (members in beta order) run on xb/yb, C and B segments on x/y.
 template <int NV, typename VT, typename XT>
-int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
+int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y, XT* xb_ext = nullptr, XT* yb_ext = nullptr) {
   const hipStream_t st = ctx->stream;
   const uint64_t n = ctx->n;
   const VT* val = sizeof(VT) == 4 ? reinterpret_cast<const VT*>(ctx->sv32.p) : reinterpret_cast<const VT*>(ctx->sv64.p);
@@ -3463,10 +3467,15 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y) {
   XT* yb = nullptr;
   const uint32_t* dnord = P<uint32_t>(ctx->dnord_idx);
   if (have_a) {
-    CHECK_RC(ensure(ctx, ctx->xb, n * NV * sizeof(XT)));
-    CHECK_RC(ensure(ctx, ctx->yb, n * NV * sizeof(XT)));
-    xb = static_cast<XT*>(ctx->xb.p);
-    yb = static_cast<XT*>(ctx->yb.p);
+    if (xb_ext && yb_ext) {  // caller-provided (the allocation-free product)
+      xb = xb_ext;
+      yb = yb_ext;
+    } else {
+      CHECK_RC(ensure(ctx, ctx->xb, n * NV * sizeof(XT)));
+      CHECK_RC(ensure(ctx, ctx->yb, n * NV * sizeof(XT)));
+      xb = static_cast<XT*>(ctx->xb.p);
+      yb = static_cast<XT*>(ctx->yb.p);
+    }
     k_perm_in<NV, XT><<<div_up(n, 256), 256, 0, st>>>(dnord, x, xb, yb, n);
   }
   const SegDesc* desc = P<SegDesc>(ctx->seg_desc);
@@ -3703,9 +3712,10 @@ int launch_csr_spmv(shci_ctx* ctx, const VT* val, double shift, const XT* x, XT*
 
 // SpMV launch on internal-order device vectors.
 template <typename VT, typename XT, typename AT>
-int launch_spmv(shci_ctx* ctx, const VT* val, double shift, const XT* x, XT* y) {
+int launch_spmv(shci_ctx* ctx, const VT* val, double shift, const XT* x, XT* y, XT* xb_ext = nullptr,
+                XT* yb_ext = nullptr) {
   if (ctx->segl) {
-    CHECK_RC(launch_seg_spmv<1, VT, XT>(ctx, shift, x, y));
+    CHECK_RC(launch_seg_spmv<1, VT, XT>(ctx, shift, x, y, xb_ext, yb_ext));
   } else {
     CHECK_RC(launch_csr_spmv<VT, XT, AT>(ctx, val, shift, x, y));
   }
@@ -4932,6 +4942,40 @@ int pt_dtm_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free product (shci_spmv_noalloc): the internal-
+// order x and y and the beta-order copies of the A segments, four vectors of
+// n_dets FP64 slots on 256-byte boundaries, at most kNoAllocBytes.  Each GPU
+// holds its own copy for the life of the process.  In host code `scratch` is
+// only a stand-in for the symbol: its device address comes from
+// hipGetSymbolAddress, per device, and only that address reaches kernels.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocBytes = 256ull << 20;  // n_dets <= 8,388,608
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+
+// The current device's address of `scratch`, looked up once per device.
+int noalloc_scratch_address(shci_ctx* ctx) {
+  static std::mutex mu;
+  static unsigned char* addr[kMaxDevices] = {};
+  if (ctx->device < 0 || ctx->device >= kMaxDevices) return fail(ctx, SHCI_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[ctx->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(ctx, SHCI_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    addr[ctx->device] = static_cast<unsigned char*>(p);
+  }
+  ctx->noalloc_scratch = addr[ctx->device];
+  return SHCI_OK;
+}
+
+}  // namespace
+
 // ===========================================================================
 // C-ABI
 // ===========================================================================
@@ -4945,6 +4989,7 @@ int shci_ctx_create(shci_ctx** out, int device) {
   HIPC(hipSetDevice(device));
   HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   for (auto& e : ctx->ev) HIPC(hipEventCreate(&e));
+  CHECK_RC(noalloc_scratch_address(ctx));
   return SHCI_OK;
 }
 
@@ -5155,6 +5200,47 @@ static int spmv_full(shci_ctx* ctx, const double* x, double* y, uint32_t flags) 
     k_to_caller<double, double><<<g, 256, 0, st>>>(P<double>(ctx->yv), P<uint32_t>(ctx->perm), yo, n);
   }
   if (!(flags & SHCI_DEVICE_PTRS)) HIPC(hipMemcpyAsync(y, ctx->y32.p, n * 8, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  return SHCI_OK;
+}
+
+int shci_spmv_noalloc(shci_ctx* ctx, const double* x, double* y, uint32_t flags) {
+  if (!ctx) return SHCI_ERR_ARG;
+  if (!x || !y) return fail(ctx, SHCI_ERR_ARG, "shci_spmv_noalloc: null vector");
+  if (!ctx->built) return fail(ctx, SHCI_ERR_STATE, "no Hamiltonian built");
+  if (ctx->ts) return fail(ctx, SHCI_ERR_UNSUPPORTED, "shci_spmv_noalloc: time_sym (reduced basis) is not supported");
+  const bool fp32 = flags & SHCI_FP32;
+  if (fp32 && !has_fp32(ctx)) return fail(ctx, SHCI_ERR_STATE, "FP32 store not built");
+  if (!fp32 && !has_fp64(ctx)) return fail(ctx, SHCI_ERR_STATE, "FP64 store not built");
+  const uint64_t n = ctx->n;
+  const size_t nst = (n + 63) & ~(uint64_t)63;  // vector stride: 512-byte multiples
+  if (4 * nst * 8 > kNoAllocBytes)
+    return fail(ctx, SHCI_ERR_UNSUPPORTED,
+                "shci_spmv_noalloc: " + std::to_string(n) + " dets exceed its scratch (at most " +
+                    std::to_string(kNoAllocBytes / 32) + ")");
+  if (!ctx->noalloc_scratch) return fail(ctx, SHCI_ERR_STATE, "scratch address not set up");
+  HIPC(hipSetDevice(ctx->device));
+  const uint32_t g = div_up(n, 256);
+  const hipStream_t st = ctx->stream;
+  unsigned char* base = ctx->noalloc_scratch;
+  if (fp32) {
+    float* xv = reinterpret_cast<float*>(base);
+    float* yv = reinterpret_cast<float*>(base + nst * 8);
+    float* xb = reinterpret_cast<float*>(base + 2 * nst * 8);
+    float* yb = reinterpret_cast<float*>(base + 3 * nst * 8);
+    k_to_internal<double, float><<<g, 256, 0, st>>>(x, P<uint32_t>(ctx->perm), xv, n);
+    CHECK_RC(launch_spmv<float, float, float>(ctx, P<float>(ctx->v32), 0.0, xv, yv, xb, yb));
+    k_to_caller<float, double><<<g, 256, 0, st>>>(yv, P<uint32_t>(ctx->perm), y, n);
+  } else {
+    double* xv = reinterpret_cast<double*>(base);
+    double* yv = reinterpret_cast<double*>(base + nst * 8);
+    double* xb = reinterpret_cast<double*>(base + 2 * nst * 8);
+    double* yb = reinterpret_cast<double*>(base + 3 * nst * 8);
+    k_to_internal<double, double><<<g, 256, 0, st>>>(x, P<uint32_t>(ctx->perm), xv, n);
+    CHECK_RC(launch_spmv<double, double, double>(ctx, P<double>(ctx->v64), 0.0, xv, yv, xb, yb));
+    k_to_caller<double, double><<<g, 256, 0, st>>>(yv, P<uint32_t>(ctx->perm), y, n);
+  }
+  HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(st));
   return SHCI_OK;
 }
