@@ -187,17 +187,29 @@ void release(shci_ctx* ctx, DevBuf& b) {
   b.bytes = 0;
 }
 
+// SHCI_LOG_ALLOC=1: every device allocation and cache release on stderr.
+bool log_alloc() {
+  static const bool on = getenv("SHCI_LOG_ALLOC") && atoi(getenv("SHCI_LOG_ALLOC")) != 0;
+  return on;
+}
+
 // Frees the cached blocks.
 void cache_trim(shci_ctx* ctx) {
+  size_t total = 0;
   for (DevBuf& c : ctx->cache) {
     (void)hipFree(c.p);
     ctx->mem_in_use -= c.bytes;
+    total += c.bytes;
   }
+  if (log_alloc() && !ctx->cache.empty())
+    fprintf(stderr, "[shci] cache trim: %zu blocks, %.2f GB\n", ctx->cache.size(), total / 1e9);
   ctx->cache.clear();
 }
 
 // b holds at least `bytes`: kept if large enough, else replaced by the
-// smallest cached block of bytes .. 2 * bytes + 64 MiB, else a new block (the
+// smallest cached block of bytes .. 9/8 bytes + 64 MiB (a close fit, so that
+// the same sequence of requests finds the same blocks build after build),
+// else a new block (the
 // cache is freed and the allocation retried once if memory runs out).  A new
 // block of 1 GiB or more gets 1/16 headroom when memory allows: the builds of
 // a growing det list ask for slightly larger stores, sort buffers and tables
@@ -210,7 +222,8 @@ int ensure(shci_ctx* ctx, DevBuf& b, size_t bytes) {
   size_t best = ctx->cache.size();
   for (size_t i = 0; i < ctx->cache.size(); i++) {
     const size_t cb = ctx->cache[i].bytes;
-    if (cb >= bytes && cb <= 2 * bytes + (64ull << 20) && (best == ctx->cache.size() || cb < ctx->cache[best].bytes))
+    if (cb >= bytes && cb <= bytes + bytes / 8 + (64ull << 20) &&
+        (best == ctx->cache.size() || cb < ctx->cache[best].bytes))
       best = i;
   }
   if (best < ctx->cache.size()) {
@@ -221,21 +234,15 @@ int ensure(shci_ctx* ctx, DevBuf& b, size_t bytes) {
   constexpr size_t kGranule = 2ull << 20;
   size_t want = bytes;
   if (bytes >= (1ull << 30)) {
-    // headroom only while an eighth of the device stays free after it (the
-    // cached blocks are freed first when that makes the room)
-    size_t free_b = 0, total_b = 0, cached = 0;
-    for (const DevBuf& c : ctx->cache) cached += c.bytes;
+    // headroom only while an eighth of the device stays free after it
+    // (freeing cached blocks for it would make the next build allocate them
+    // again)
+    size_t free_b = 0, total_b = 0;
     const size_t padded = (bytes + bytes / 16 + kGranule - 1) & ~(kGranule - 1);
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
-      if (free_b > padded + total_b / 8) {
-        want = padded;
-      } else if (free_b + cached > padded + total_b / 8) {
-        cache_trim(ctx);
-        want = padded;
-      }
-    }
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > padded + total_b / 8) want = padded;
     (void)hipGetLastError();
   }
+  const auto t_alloc = std::chrono::steady_clock::now();
   hipError_t e = hipMalloc(&b.p, want);
   if (e == hipErrorOutOfMemory && want > bytes) {
     (void)hipGetLastError();
@@ -254,6 +261,9 @@ int ensure(shci_ctx* ctx, DevBuf& b, size_t bytes) {
   }
   b.bytes = want;
   ctx->mem_in_use += want;
+  if (log_alloc())
+    fprintf(stderr, "[shci] hipMalloc %.3f GB (asked %.3f GB) in %.1f ms\n", want / 1e9, bytes / 1e9,
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_alloc).count());
   return SHCI_OK;
 }
 
