@@ -119,7 +119,8 @@ struct shci_ctx {
   DevBuf tile_rb, tile_re;              // [tile * 6 + class - 3] -> segment range of the class run
   DevBuf tile_u0, tile_nu;              // first entry and size of the tile's det table
   DevBuf tile_tab;                      // tile det tables (internal det ids)
-  DevBuf seg_loc;                       // membership -> index in its tile's table (m <= 64 only)
+  DevBuf seg_loc;                       // membership -> index in its tile's table (build only)
+  DevBuf tile_roff, tile_rec;           // per-tile record offset (16-byte units) and the records
   DevBuf kept;                          // kept-pair counters of the segment fill
   // Time-reversal symmetry (SHCI_TIME_SYM): the matrix is built over the
   // expanded list E (each det, then its reverse when up != dn); calls take and
@@ -1588,9 +1589,12 @@ __global__ void k_seg_keep(const uint32_t* __restrict__ rstart, const uint32_t* 
 // once per segment.  A group's segments with m <= 64 are cut into tiles of
 // fewer than kTileMax memberships: a segment starts a new tile when the
 // memberships before it in its group reach the next multiple of kTileCap.
-constexpr uint32_t kTileCap = 1984;
-constexpr uint32_t kTileMax = 2048;   // > kTileCap + 64: memberships (and dets) of one tile
+constexpr uint32_t kTileCap = 1920;
+constexpr uint32_t kTileMax = 1984;   // > kTileCap + 64: memberships (and dets) of one tile
 constexpr uint32_t kTileHash = 4096;  // LDS hash slots of the table build
+constexpr uint32_t kTileSort = 2048;  // the table build's bitonic sort pads its keys to a power of two
+static_assert(kTileSort >= kTileMax && (kTileSort & (kTileSort - 1)) == 0, "sort buffer below the tile size");
+static_assert(kTileHash >= 2 * kTileMax, "hash table below twice the tile size");
 
 // Per kept run (index f in run order): spin-group key and the memberships it
 // brings to its tile (m <= 64) or 0 (larger segments have no tile).
@@ -1716,8 +1720,8 @@ __global__ __launch_bounds__(256) void k_tile_tables(const uint32_t* __restrict_
   __shared__ uint32_t hk[kTileHash];
   __shared__ uint16_t cid[kTileHash];
   __shared__ uint16_t sl[kTileMax];
-  __shared__ uint32_t key[kTileMax];
-  __shared__ uint16_t pay[kTileMax];
+  __shared__ uint32_t key[kTileSort];
+  __shared__ uint16_t pay[kTileSort];
   __shared__ uint16_t rank[kTileMax];
   __shared__ uint32_t wtot[4];
   const uint32_t t = blockIdx.x, tid = threadIdx.x;
@@ -1830,6 +1834,90 @@ __global__ __launch_bounds__(256) void k_tile_tables(const uint32_t* __restrict_
     const uint32_t len = (uint32_t)(me[k] - mb[k]);
     for (uint32_t i = tid; i < len; i += 256) loc[mb[k] + i] = rank[cid[sl[k0 + i]]];
     k0 += len;
+  }
+}
+
+// Per-tile records: what the tile product stages in LDS, contiguous and
+// 16-byte aligned so that one wide copy brings it in.
+//   header (kTileHead bytes): table size | (A tile << 31), first table entry,
+//     segments per class run (u16 x 6), memberships per class run (u16 x 6),
+//     first value slot of each class run (u64 x 6);
+//   body: per segment (class runs in order) a u32 (slot offset in its class
+//     run << 11 | member offset in its class run), then per segment its m
+//     (u8), then per membership its index in the tile table (u16).
+// A class run has fewer than kTileMax memberships, of at most 64 members per
+// segment: slot offsets < kTileMax * 63 / 2 < 2^16, member offsets < 2^11.
+constexpr uint32_t kTileHead = 80;
+__host__ __device__ __forceinline__ uint32_t tile_body_bytes(uint32_t k, uint32_t nm) {
+  return ((((5 * k + 1) & ~1u) + 2 * nm) + 15) & ~15u;
+}
+constexpr uint32_t kTileBodyMax = ((((5 * (kTileMax / 2) + 1) & ~1u) + 2 * kTileMax) + 15) & ~15u;
+
+// Record size of each tile, in 16-byte units.
+__global__ void k_tile_rec_size(const uint32_t* __restrict__ rb, const uint32_t* __restrict__ re,
+                                const SegDesc* __restrict__ desc, uint64_t T, uint32_t* __restrict__ units) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  uint32_t k = 0, nm = 0;
+  for (int c = 0; c < 6; c++) {
+    const uint32_t b = rb[t * 6 + c], e = re[t * 6 + c];
+    if (b < e) {
+      k += e - b;
+      nm += desc[e - 1].moff + desc[e - 1].m - desc[b].moff;
+    }
+  }
+  units[t] = (kTileHead + tile_body_bytes(k, nm)) / 16;
+}
+
+__global__ __launch_bounds__(256) void k_tile_records(const uint32_t* __restrict__ rb, const uint32_t* __restrict__ re,
+                                                      const uint32_t* __restrict__ u0, const uint32_t* __restrict__ nu,
+                                                      const SegDesc* __restrict__ desc,
+                                                      const uint16_t* __restrict__ loc,
+                                                      const uint32_t* __restrict__ roff, uint4* __restrict__ rec) {
+  const uint64_t t = blockIdx.x;
+  unsigned char* r = reinterpret_cast<unsigned char*>(rec + roff[t]);
+  uint32_t nseg[6], nmem[6], k0[6], m0[6];
+  uint64_t voff0[6], moff0[6];
+  uint32_t K = 0, NM = 0;
+  for (int c = 0; c < 6; c++) {
+    const uint32_t b = rb[t * 6 + c], e = re[t * 6 + c];
+    nseg[c] = e - b;
+    k0[c] = K;
+    m0[c] = NM;
+    nmem[c] = 0;
+    voff0[c] = moff0[c] = 0;
+    if (b < e) {
+      voff0[c] = desc[b].voff;
+      moff0[c] = desc[b].moff;
+      nmem[c] = desc[e - 1].moff + desc[e - 1].m - desc[b].moff;
+      K += e - b;
+      NM += nmem[c];
+    }
+  }
+  if (threadIdx.x == 0) {
+    uint32_t* h32 = reinterpret_cast<uint32_t*>(r);
+    uint16_t* h16 = reinterpret_cast<uint16_t*>(r + 8);
+    uint64_t* h64 = reinterpret_cast<uint64_t*>(r + 32);
+    h32[0] = nu[t];
+    h32[1] = u0[t];
+    for (int c = 0; c < 6; c++) {
+      h16[c] = (uint16_t)nseg[c];
+      h16[6 + c] = (uint16_t)nmem[c];
+      h64[c] = voff0[c];
+    }
+  }
+  unsigned char* body = r + kTileHead;
+  uint32_t* sp = reinterpret_cast<uint32_t*>(body);
+  uint8_t* sm = body + 4 * K;
+  uint16_t* sl = reinterpret_cast<uint16_t*>(body + ((5 * K + 1) & ~1u));
+  for (int c = 0; c < 6; c++) {
+    const uint32_t b = rb[t * 6 + c];
+    for (uint32_t k = threadIdx.x; k < nseg[c]; k += 256) {
+      const SegDesc D = desc[b + k];
+      sp[k0[c] + k] = ((uint32_t)(D.voff - voff0[c]) << 11) | (uint32_t)(D.moff - moff0[c]);
+      sm[k0[c] + k] = (uint8_t)D.m;
+    }
+    for (uint32_t i = threadIdx.x; i < nmem[c]; i += 256) sl[m0[c] + i] = loc[moff0[c] + i];
   }
 }
 
@@ -2317,18 +2405,21 @@ __device__ __forceinline__ void seg_group_sums(const uint32_t (&mm)[NB], const u
   }
 }
 
-// A tile in LDS: its x and y, and for each of its segments (numbered class
-// run by class run) the slot offset from the run's first slot and the member
-// offset from the run's first member and m (packed (moff << 8) | m), and each
-// membership's index into x and y.  A tile has fewer than kTileMax
-// memberships, hence fewer than kTileMax / 2 segments.
+// A tile in LDS: its x and y, and the body of its record (per segment the
+// packed slot and member offsets and m, per membership the index into x and
+// y; segments numbered class run by class run).
 template <typename XT>
 struct TileLds {
   XT xs[kTileMax];
   XT ys[kTileMax];
-  uint32_t sv[kTileMax / 2];
-  uint32_t sm[kTileMax / 2];
-  uint16_t sl[kTileMax];
+  uint4 body[kTileBodyMax / 16];
+};
+
+// Views of a staged record body with K segments.
+struct TileBody {
+  const uint32_t* sp;  // (slot offset in the class run << 11) | member offset in the class run
+  const uint8_t* sm;   // m
+  const uint16_t* sl;  // membership -> index into xs / ys
 };
 
 // One class run of a tile: its n segments (LDS entries k0 ..) with m <= S,
@@ -2336,7 +2427,7 @@ struct TileLds {
 // slot voff0 and first membership m0 (LDS).
 template <int S, int NB, int CH, typename VT, typename XT>
 __device__ __forceinline__ void tile_class_run(uint32_t n, uint32_t k0, uint32_t m0, uint64_t voff0,
-                                               const VT* __restrict__ val, TileLds<XT>& L) {
+                                               const VT* __restrict__ val, TileLds<XT>& L, const TileBody& B) {
   static_assert(CH <= S, "the delivery of the travelling sums assumes fewer than 2m rotations");
   constexpr uint32_t GW = kWave / S;  // lane groups per wave
   constexpr uint32_t PW = GW * NB;    // segments per wave and round
@@ -2346,10 +2437,10 @@ __device__ __forceinline__ void tile_class_run(uint32_t n, uint32_t k0, uint32_t
     // run lists its segments largest first); waves past the run idle
     const uint32_t sw = r0 + wv * PW;
     if (sw >= n) break;
-    const uint32_t steps = ((L.sm[k0 + sw] & 0xFFu) - 1) / 2;
+    const uint32_t steps = (B.sm[k0 + sw] - 1u) / 2;
     // the wave's smallest segment (its last) bounds the steps all of its segments have
     const uint32_t slast = sw + PW - 1;
-    const uint32_t steps_all = slast < n ? ((L.sm[k0 + slast] & 0xFFu) - 1) / 2 : 0u;
+    const uint32_t steps_all = slast < n ? (B.sm[k0 + slast] - 1u) / 2 : 0u;
     uint32_t mm[NB], so[NB];
     XT xj[NB], out[NB];
     uint32_t l[NB];
@@ -2359,13 +2450,13 @@ __device__ __forceinline__ void tile_class_run(uint32_t n, uint32_t k0, uint32_t
       mm[k] = so[k] = 0;
       uint32_t moff = 0;
       if (sk < n) {
-        const uint32_t pk = L.sm[k0 + sk];
-        mm[k] = pk & 0xFFu;
-        moff = m0 + (pk >> 8);
-        so[k] = L.sv[k0 + sk];
+        const uint32_t pk = B.sp[k0 + sk];
+        mm[k] = B.sm[k0 + sk];
+        moff = m0 + (pk & 0x7FFu);
+        so[k] = pk >> 11;
       }
       const bool valid = j < mm[k];
-      l[k] = valid ? L.sl[moff + j] : 0u;
+      l[k] = valid ? B.sl[moff + j] : 0u;
       xj[k] = valid ? L.xs[l[k]] : (XT)0;
     }
     seg_group_sums<S, NB, CH, VT, XT>(mm, so, xj, val + voff0, steps, steps_all, out);
@@ -2376,67 +2467,58 @@ __device__ __forceinline__ void tile_class_run(uint32_t n, uint32_t k0, uint32_t
 }
 
 // Off-diagonal part of the segments with m <= 64, one workgroup per tile: the
-// tile's x (one gather per distinct det), segment descriptors and membership
-// indices staged in LDS, its class runs with lane groups of 64, 32, 16, 8, 4,
-// 2 lanes, member sums into an LDS y, then one global atomic per distinct
-// det.  VT stored value type, XT vector type.
+// tile's record body copied into LDS with 16-byte loads (its header read
+// through scalar loads), its x gathered (one load per distinct det), its class
+// runs with lane groups of 64, 32, 16, 8, 4, 2 lanes, member sums into an LDS
+// y, then one global atomic per distinct det.  VT stored value type, XT
+// vector type.
 template <typename VT, typename XT>
-__global__ __launch_bounds__(256) void k_seg_spmv_tiles(const uint32_t* __restrict__ rb, const uint32_t* __restrict__ re,
-                                                        const uint32_t* __restrict__ u0, const uint32_t* __restrict__ nu,
-                                                        const uint32_t* __restrict__ tab, const SegDesc* __restrict__ desc,
-                                                        const uint16_t* __restrict__ loc, const VT* __restrict__ val,
+__global__ __launch_bounds__(256) void k_seg_spmv_tiles(const uint32_t* __restrict__ roff, const uint4* __restrict__ rec,
+                                                        const uint32_t* __restrict__ tab, const VT* __restrict__ val,
                                                         const XT* __restrict__ xi, XT* __restrict__ yi,
                                                         const XT* __restrict__ xb, XT* __restrict__ yb) {
   __shared__ TileLds<XT> L;
-  const uint64_t t = blockIdx.x;
-  const uint32_t nut = nu[t], U = nut & 0x7FFFFFFFu;
+  const uint4* r = rec + roff[blockIdx.x];
+  const uint32_t* h32 = reinterpret_cast<const uint32_t*>(r);
+  const uint16_t* h16 = reinterpret_cast<const uint16_t*>(h32 + 2);
+  const uint64_t* h64 = reinterpret_cast<const uint64_t*>(h32 + 8);
+  const uint32_t nut = h32[0], U = nut & 0x7FFFFFFFu;
   const XT* __restrict__ x = (nut >> 31) ? xb : xi;  // A tiles: beta-order copies
   XT* __restrict__ y = (nut >> 31) ? yb : yi;
-  const uint32_t* __restrict__ tb = tab + u0[t];
+  const uint32_t* __restrict__ tb = tab + h32[1];
   uint32_t nseg[6], k0[6], m0[6];
-  uint64_t voff0[6], moff0[6];
-  uint32_t ks = 0, ms = 0;
+  uint64_t voff0[6];
+  uint32_t K = 0, NM = 0;
 #pragma unroll
   for (int c = 0; c < 6; c++) {
-    const uint32_t b = rb[t * 6 + c], e = re[t * 6 + c];
-    nseg[c] = e - b;
-    k0[c] = ks;
-    m0[c] = ms;
-    voff0[c] = moff0[c] = 0;
-    if (b < e) {
-      const SegDesc F = desc[b], G = desc[e - 1];
-      voff0[c] = F.voff;
-      moff0[c] = F.moff;
-      ks += e - b;
-      ms += G.moff + G.m - F.moff;
-    }
+    nseg[c] = h16[c];
+    k0[c] = K;
+    m0[c] = NM;
+    voff0[c] = h64[c];
+    K += nseg[c];
+    NM += h16[6 + c];
   }
+  const uint4* body = r + kTileHead / 16;
+  const uint32_t nv = tile_body_bytes(K, NM) / 16;
+  for (uint32_t v = threadIdx.x; v < nv; v += 256) L.body[v] = body[v];
   for (uint32_t u = threadIdx.x; u < U; u += 256) {
     L.xs[u] = x[tb[u]];
     L.ys[u] = (XT)0;
   }
-#pragma unroll
-  for (int c = 0; c < 6; c++) {
-    const uint32_t b = rb[t * 6 + c];
-    for (uint32_t k = threadIdx.x; k < nseg[c]; k += 256) {
-      const SegDesc D = desc[b + k];
-      L.sv[k0[c] + k] = (uint32_t)(D.voff - voff0[c]);
-      L.sm[k0[c] + k] = ((uint32_t)(D.moff - moff0[c]) << 8) | D.m;
-    }
-    const uint32_t len = (c + 1 < 6 ? m0[c + 1] : ms) - m0[c];
-    for (uint32_t i = threadIdx.x; i < len; i += 256) L.sl[m0[c] + i] = loc[moff0[c] + i];
-  }
+  const unsigned char* lb = reinterpret_cast<const unsigned char*>(L.body);
+  const TileBody B{reinterpret_cast<const uint32_t*>(lb), lb + 4 * K,
+                   reinterpret_cast<const uint16_t*>(lb + ((5 * K + 1) & ~1u))};
   __syncthreads();
   // (lanes per segment, segments per lane group and round, steps per load
   // batch); the widening product (FP32 values, FP64 vectors) runs the wide
   // classes one segment per lane group with longer batches
   constexpr bool MX = sizeof(VT) != sizeof(XT);
-  tile_class_run<64, MX ? 1 : 2, MX ? 16 : 8, VT, XT>(nseg[0], k0[0], m0[0], voff0[0], val, L);
-  tile_class_run<32, MX ? 1 : 2, MX ? 16 : 8, VT, XT>(nseg[1], k0[1], m0[1], voff0[1], val, L);
-  tile_class_run<16, MX ? 1 : 2, 8, VT, XT>(nseg[2], k0[2], m0[2], voff0[2], val, L);
-  tile_class_run<8, 4, 4, VT, XT>(nseg[3], k0[3], m0[3], voff0[3], val, L);
-  tile_class_run<4, 4, 2, VT, XT>(nseg[4], k0[4], m0[4], voff0[4], val, L);
-  tile_class_run<2, 4, 1, VT, XT>(nseg[5], k0[5], m0[5], voff0[5], val, L);
+  tile_class_run<64, MX ? 1 : 2, MX ? 16 : 8, VT, XT>(nseg[0], k0[0], m0[0], voff0[0], val, L, B);
+  tile_class_run<32, MX ? 1 : 2, MX ? 16 : 8, VT, XT>(nseg[1], k0[1], m0[1], voff0[1], val, L, B);
+  tile_class_run<16, MX ? 1 : 2, 8, VT, XT>(nseg[2], k0[2], m0[2], voff0[2], val, L, B);
+  tile_class_run<8, 4, 4, VT, XT>(nseg[3], k0[3], m0[3], voff0[3], val, L, B);
+  tile_class_run<4, 4, 2, VT, XT>(nseg[4], k0[4], m0[4], voff0[4], val, L, B);
+  tile_class_run<2, 4, 1, VT, XT>(nseg[5], k0[5], m0[5], voff0[5], val, L, B);
   __syncthreads();
   for (uint32_t u = threadIdx.x; u < U; u += 256) atomicAdd(&y[tb[u]], L.ys[u]);
 }
@@ -3163,7 +3245,7 @@ int build_seg_layout(shci_ctx* ctx) {
     for (const DevBuf& c : ctx->cache) free_b += c.bytes;
     if ((double)free_b < 40.0 * (double)M) {
       DevBuf* big[] = {&ctx->sv64, &ctx->sv32, &ctx->seg_mem, &ctx->seg_desc, &ctx->seg_core, &ctx->seg_type,
-                       &ctx->seg_loc, &ctx->tile_tab, &ctx->dav, &ctx->tmp};
+                       &ctx->seg_loc, &ctx->tile_tab, &ctx->tile_rec, &ctx->dav, &ctx->tmp};
       for (DevBuf* b : big) release(ctx, *b);
       cache_trim(ctx);
     }
@@ -3353,6 +3435,24 @@ int build_seg_layout(shci_ctx* ctx) {
                                    P<uint32_t>(ctx->tile_tab), P<uint32_t>(ctx->tile_nu),
                                    P<uint16_t>(ctx->seg_loc));
   HIPC(hipGetLastError());
+  // the tile records the product stages
+  {
+    TmpBuf units(ctx);
+    CHECK_RC(ensure(ctx, units.b, T * 4));
+    CHECK_RC(ensure(ctx, ctx->tile_roff, T * 4));
+    k_tile_rec_size<<<div_up(T, 256), 256, 0, st>>>(P<uint32_t>(ctx->tile_rb), P<uint32_t>(ctx->tile_re), desc, T,
+                                                     units.p<uint32_t>());
+    CHECK_RC(excl_scan(ctx, units.p<uint32_t>(), P<uint32_t>(ctx->tile_roff), T));
+    uint32_t lastr = 0, lastn = 0;
+    CHECK_RC(copy_back(ctx, &lastr, P<uint32_t>(ctx->tile_roff) + (T - 1)));
+    CHECK_RC(copy_back(ctx, &lastn, units.p<uint32_t>() + (T - 1)));
+    CHECK_RC(ensure(ctx, ctx->tile_rec, ((uint64_t)lastr + lastn) * 16));
+    k_tile_records<<<T, 256, 0, st>>>(P<uint32_t>(ctx->tile_rb), P<uint32_t>(ctx->tile_re), P<uint32_t>(ctx->tile_u0),
+                                      P<uint32_t>(ctx->tile_nu), desc, P<uint16_t>(ctx->seg_loc),
+                                      P<uint32_t>(ctx->tile_roff), P<uint4>(ctx->tile_rec));
+    HIPC(hipGetLastError());
+  }
+  release(ctx, ctx->seg_loc);  // copied into the records
   ctx->n_tiles = T;
   return SHCI_OK;
 }
@@ -3534,9 +3634,8 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y, XT* xb_ext 
   if constexpr (NV == 1) {
     // m <= 64: one workgroup per tile
     if (ctx->n_tiles)
-      k_seg_spmv_tiles<VT, XT><<<ctx->n_tiles, 256, 0, st>>>(
-          P<uint32_t>(ctx->tile_rb), P<uint32_t>(ctx->tile_re), P<uint32_t>(ctx->tile_u0), P<uint32_t>(ctx->tile_nu),
-          P<uint32_t>(ctx->tile_tab), desc, P<uint16_t>(ctx->seg_loc), val, x, y, xb, yb);
+      k_seg_spmv_tiles<VT, XT><<<ctx->n_tiles, 256, 0, st>>>(P<uint32_t>(ctx->tile_roff), P<uint4>(ctx->tile_rec),
+                                                             P<uint32_t>(ctx->tile_tab), val, x, y, xb, yb);
   }
   if (have_a) k_perm_add<NV, XT><<<div_up(n, 256), 256, 0, st>>>(dnord, yb, y, n);
   HIPC(hipGetLastError());
@@ -3642,7 +3741,7 @@ int do_build(shci_ctx* ctx, uint32_t flags) {
   {
     DevBuf* seg[] = {&ctx->seg_desc, &ctx->seg_type, &ctx->seg_core, &ctx->seg_mem, &ctx->sv32, &ctx->sv64,
                      &ctx->mA_id, &ctx->mB_id, &ctx->dnpos, &ctx->xb, &ctx->yb, &ctx->tile_rb, &ctx->tile_re,
-                     &ctx->tile_u0, &ctx->tile_nu, &ctx->tile_tab, &ctx->seg_loc};
+                     &ctx->tile_u0, &ctx->tile_nu, &ctx->tile_tab, &ctx->seg_loc, &ctx->tile_roff, &ctx->tile_rec};
     for (DevBuf* b : seg) release(ctx, *b);
     ctx->n_seg = ctx->n_mem = ctx->n_slots = ctx->n_pairs = ctx->n_tiles = 0;
   }
@@ -5012,7 +5111,7 @@ int shci_ctx_destroy(shci_ctx* ctx) {
                     &ctx->dnord_idx, &ctx->dnord_up, &ctx->m1a_id, &ctx->m1b_id, &ctx->dm1_vals, &ctx->dm1_rstart,
                     &ctx->dm1_seg, &ctx->flag, &ctx->mA_id, &ctx->mB_id, &ctx->seg_desc, &ctx->seg_type,
                     &ctx->seg_core, &ctx->seg_mem, &ctx->sv32, &ctx->sv64, &ctx->dnpos, &ctx->xb, &ctx->yb,
-                    &ctx->tile_rb, &ctx->tile_re, &ctx->tile_u0, &ctx->tile_nu, &ctx->tile_tab, &ctx->seg_loc,
+                    &ctx->tile_rb, &ctx->tile_re, &ctx->tile_u0, &ctx->tile_nu, &ctx->tile_tab, &ctx->seg_loc, &ctx->tile_roff, &ctx->tile_rec,
                     &ctx->diag,
                     &ctx->cnt,
                     &ctx->row_ptr, &ctx->row_end, &ctx->col, &ctx->v64, &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb,
@@ -5652,7 +5751,7 @@ int shci_clear(shci_ctx* ctx) {
                     &ctx->m1a_id, &ctx->m1b_id, &ctx->dm1_vals, &ctx->dm1_rstart, &ctx->dm1_seg, &ctx->diag,
                     &ctx->mA_id, &ctx->mB_id, &ctx->seg_desc, &ctx->seg_type, &ctx->seg_core, &ctx->seg_mem,
                     &ctx->sv32, &ctx->sv64, &ctx->dnpos, &ctx->xb, &ctx->yb, &ctx->tile_rb, &ctx->tile_re,
-                    &ctx->tile_u0, &ctx->tile_nu, &ctx->tile_tab, &ctx->seg_loc,
+                    &ctx->tile_u0, &ctx->tile_nu, &ctx->tile_tab, &ctx->seg_loc, &ctx->tile_roff, &ctx->tile_rec,
                     &ctx->cnt, &ctx->row_ptr, &ctx->row_end, &ctx->row_end, &ctx->col, &ctx->v64,
                     &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb, &ctx->va, &ctx->vb, &ctx->xv,
                     &ctx->yv, &ctx->x32, &ctx->y32, &ctx->dav, &ctx->cap};
