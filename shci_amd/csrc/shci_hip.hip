@@ -32,6 +32,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "shci_hip.h"
@@ -2473,10 +2474,10 @@ __device__ __forceinline__ void tile_class_run(uint32_t n, uint32_t k0, uint32_t
 // y, then one global atomic per distinct det.  VT stored value type, XT
 // vector type.
 template <typename VT, typename XT>
-__global__ __launch_bounds__(256) void k_seg_spmv_tiles(const uint32_t* __restrict__ roff, const uint4* __restrict__ rec,
-                                                        const uint32_t* __restrict__ tab, const VT* __restrict__ val,
-                                                        const XT* __restrict__ xi, XT* __restrict__ yi,
-                                                        const XT* __restrict__ xb, XT* __restrict__ yb) {
+__device__ __forceinline__ void seg_spmv_tile(const uint32_t* __restrict__ roff, const uint4* __restrict__ rec,
+                                              const uint32_t* __restrict__ tab, const VT* __restrict__ val,
+                                              const XT* __restrict__ xi, XT* __restrict__ yi,
+                                              const XT* __restrict__ xb, XT* __restrict__ yb) {
   __shared__ TileLds<XT> L;
   const uint4* r = rec + roff[blockIdx.x];
   const uint32_t* h32 = reinterpret_cast<const uint32_t*>(r);
@@ -2521,6 +2522,24 @@ __global__ __launch_bounds__(256) void k_seg_spmv_tiles(const uint32_t* __restri
   tile_class_run<2, 4, 1, VT, XT>(nseg[5], k0[5], m0[5], voff0[5], val, L, B);
   __syncthreads();
   for (uint32_t u = threadIdx.x; u < U; u += 256) atomicAdd(&y[tb[u]], L.ys[u]);
+}
+
+template <typename VT, typename XT>
+__global__ __launch_bounds__(256) void k_seg_spmv_tiles(const uint32_t* __restrict__ roff, const uint4* __restrict__ rec,
+                                                        const uint32_t* __restrict__ tab, const VT* __restrict__ val,
+                                                        const XT* __restrict__ xi, XT* __restrict__ yi,
+                                                        const XT* __restrict__ xb, XT* __restrict__ yb) {
+  seg_spmv_tile<VT, XT>(roff, rec, tab, val, xi, yi, xb, yb);
+}
+
+// The FP32 product's tile kernel: its LDS leaves room for six workgroups per
+// CU, so it is compiled for six waves per SIMD (at most 80 VGPRs).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+void k_seg_spmv_tiles_f32(const uint32_t* __restrict__ roff, const uint4* __restrict__ rec,
+                          const uint32_t* __restrict__ tab, const float* __restrict__ val,
+                          const float* __restrict__ xi, float* __restrict__ yi, const float* __restrict__ xb,
+                          float* __restrict__ yb) {
+  seg_spmv_tile<float, float>(roff, rec, tab, val, xi, yi, xb, yb);
 }
 
 
@@ -3634,8 +3653,12 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y, XT* xb_ext 
   if constexpr (NV == 1) {
     // m <= 64: one workgroup per tile
     if (ctx->n_tiles)
-      k_seg_spmv_tiles<VT, XT><<<ctx->n_tiles, 256, 0, st>>>(P<uint32_t>(ctx->tile_roff), P<uint4>(ctx->tile_rec),
-                                                             P<uint32_t>(ctx->tile_tab), val, x, y, xb, yb);
+      if constexpr (std::is_same<VT, float>::value && std::is_same<XT, float>::value)
+        k_seg_spmv_tiles_f32<<<ctx->n_tiles, 256, 0, st>>>(P<uint32_t>(ctx->tile_roff), P<uint4>(ctx->tile_rec),
+                                                            P<uint32_t>(ctx->tile_tab), val, x, y, xb, yb);
+      else
+        k_seg_spmv_tiles<VT, XT><<<ctx->n_tiles, 256, 0, st>>>(P<uint32_t>(ctx->tile_roff), P<uint4>(ctx->tile_rec),
+                                                               P<uint32_t>(ctx->tile_tab), val, x, y, xb, yb);
   }
   if (have_a) k_perm_add<NV, XT><<<div_up(n, 256), 256, 0, st>>>(dnord, yb, y, n);
   HIPC(hipGetLastError());
