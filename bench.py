@@ -267,6 +267,43 @@ def autograd_bench(w, args, n_dirs=20, step=1e-5, seed=2024):
     print(json.dumps(out))
 
 
+def copy_calibration():
+    """A 2 GiB elementwise copy (torch.mul by 1, 2 GiB read + 2 GiB written)
+    whose HBM counters (FETCH_SIZE, WRITE_SIZE) calibrate those of the
+    kernels profiled in the same run."""
+    import torch
+
+    a = torch.ones(1 << 29, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: 3 x torch.mul over {a.numel() * 4 / 2**30:.0f} GiB read + written")
+
+
+def spmv_bench(w, args):
+    """The products alone: a device-resident build of each store (FP32, then
+    FP64) and --reps timed products (HIP events on the library's stream,
+    after one warm-up product)."""
+    from shci_amd.hamiltonian import Hamiltonian
+
+    out = {"metric": "SpMV HBM GB/s (algorithmic bytes / kernel time)", "unit": "GB/s", "n_gpus": 1,
+           "higher_is_better": True, "config": {"workload": f"cfg{args.config}", "n_dets": len(w.up),
+                                                "det_list_digest": w.det_hash, "layout": args.layout}}
+    for store in ("fp32", "fp64"):
+        ham = Hamiltonian(w.n, w.h1, w.eri, w.core, store=store, layout=args.layout)
+        ham.update(w.up, w.dn)
+        ms, by = ham.matrix.time_spmv(fp32=store == "fp32", reps=args.reps)
+        log(f"[bench] {store} product {ms:.3f} ms, {by / 1e9:.2f} GB algorithmic, {by / ms / 1e6:.0f} GB/s")
+        out[store] = dict(ms=ms, algorithmic_bytes=by, GBps=by / ms / 1e6, frac=by / ms / 1e6 / HBM_PEAK_GBS)
+        ham.clear()
+    out["value"] = out["fp32"]["GBps"]
+    out["fp32_over_fp64"] = out["fp32"]["ms"] / out["fp64"]["ms"]
+    print(json.dumps(out))
+    if args.copy_calibration:
+        copy_calibration()
+
+
 def first_index_within(energies, e_ref, tol=1e-6):
     for k, e in enumerate(energies):
         if abs(e - e_ref) <= tol:
@@ -294,6 +331,10 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with a 2 GiB elementwise copy (torch.mul by 1) whose HBM counters "
                          "(FETCH_SIZE, WRITE_SIZE) calibrate the traffic of the kernels above")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="only the products: FP32 then FP64 store of the --layout, --reps timed products each "
+                         "(HIP events), one JSON line; the workload rocprofv3 profiles are taken on")
+    ap.add_argument("--reps", type=int, default=10, help="products per store in --spmv-only")
     ap.add_argument("--autograd", action="store_true",
                     help="time E(h, eri) forward/backward (RDM adjoint) and check 20 finite-difference "
                          "directions on the --config workload (use --config 4), one JSON line")
@@ -311,6 +352,9 @@ def main():
     t_grow = time.time() - t_grow
     if args.autograd:
         autograd_bench(w, args)
+        return
+    if args.spmv_only:
+        spmv_bench(w, args)
         return
     n_dets = len(w.up)
     ns = w.spec.n_states
@@ -515,14 +559,7 @@ def main():
                                 speedup_per_root=spmv32_ms / spmm["ms_per_root"])
     print(json.dumps(out))
     if args.copy_calibration:
-        import torch
-
-        a = torch.ones(1 << 29, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            torch.mul(a, 1.0, out=b)
-        torch.cuda.synchronize()
-        log(f"[bench] copy calibration: 3 x torch.mul over {a.numel() * 4 / 2**30:.0f} GiB read + written")
+        copy_calibration()
 
 
 if __name__ == "__main__":

@@ -2321,7 +2321,10 @@ __device__ __forceinline__ void seg_group_sums(const uint32_t (&mm)[NB], const u
   const gptr vb = (gptr)base;
   const gbyte bb = (gbyte)base;
   // a slot at byte offset o (32-bit): one scalar base plus one vector offset
-  auto at = [&](uint32_t o) { return __builtin_nontemporal_load((gptr)(bb + o)); };
+  // ordinary (cached) loads: rows of a few tens of bytes share cache lines
+  // with the next step's and the next segment's, which a streaming load would
+  // let go before they are read again
+  auto at = [&](uint32_t o) { return *((gptr)(bb + o)); };
   const uint32_t j = threadIdx.x & (S - 1);
   const uint32_t gbase = lane_id() & ~(uint32_t)(S - 1);
   uint32_t full[NB], off[NB], stride[NB];
@@ -2697,6 +2700,15 @@ __device__ __forceinline__ void seg_wave_rows(const SegDesc& D, const VT* __rest
     R[c] = acc[c] = (XT)0;
   }
   gptr vs = (gptr)(val + D.voff);
+  // FP32 rows are read through the caches (the line two rows share stays for
+  // the second), FP64 rows stream (measured faster for each)
+  auto row_load = [](gptr p) {
+    if constexpr (sizeof(VT) == 4) {
+      return *p;
+    } else {
+      return __builtin_nontemporal_load(p);
+    }
+  };
   uint32_t d = 1;
   for (; d + U - 1 <= fs; d += U) {
     XT v[U][NC];
@@ -2705,7 +2717,7 @@ __device__ __forceinline__ void seg_wave_rows(const SegDesc& D, const VT* __rest
 #pragma unroll
       for (int c = 0; c < NC; c++) {
         const uint32_t j = 64 * c + lane;
-        v[u][c] = (XT)__builtin_nontemporal_load(vs + (uint64_t)(d + u - 1) * m + (j < m ? j : 0));
+        v[u][c] = (XT)row_load(vs + (uint64_t)(d + u - 1) * m + (j < m ? j : 0));
       }
 #pragma unroll
     for (uint32_t u = 0; u < U; u++) {
@@ -2723,7 +2735,7 @@ __device__ __forceinline__ void seg_wave_rows(const SegDesc& D, const VT* __rest
 #pragma unroll
     for (int c = 0; c < NC; c++) {
       const uint32_t j = 64 * c + lane;
-      v[c] = (XT)__builtin_nontemporal_load(vs + (uint64_t)(d - 1) * m + (j < m ? j : 0));
+      v[c] = (XT)row_load(vs + (uint64_t)(d - 1) * m + (j < m ? j : 0));
     }
     seg_rotate<NC, XT>(xp, lane, cl, ll);
     seg_rotate<NC, XT>(R, lane, cl, ll);
