@@ -33,6 +33,7 @@
 #include <numeric>
 #include <string>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "shci_hip.h"
@@ -146,6 +147,16 @@ struct shci_ctx {
   // temporaries every time; hipMalloc/hipFree of GB blocks per build would
   // cost more than the kernels).  Freed on memory pressure, clear and destroy.
   std::vector<DevBuf> cache;
+  // Blocks of one chunk (1 GiB) or more: physical chunks mapped into a
+  // reserved address range, by start address; unmapped chunks wait in
+  // `pool` for the next block (see ensure).
+  struct ChunkMap {
+    size_t reserved = 0;                                // chunks of address range
+    std::vector<hipMemGenericAllocationHandle_t> used;  // mapped chunks, in address order
+  };
+  std::unordered_map<void*, ChunkMap> chunked;
+  std::vector<hipMemGenericAllocationHandle_t> pool;
+  size_t chunk = 1ull << 30;  // chunk size (SHCI_CHUNK_MIB at context creation, 1 GiB by default)
 
   // Ritz-vector capture (shci_capture_ritz): the ground-state Ritz vector of
   // every Rayleigh-Ritz step of the latest Davidson call, widened to FP64, in
@@ -195,30 +206,209 @@ bool log_alloc() {
   return on;
 }
 
-// Frees the cached blocks.
-void cache_trim(shci_ctx* ctx) {
+// ---------------------------------------------------------------------------
+// Large blocks: physical chunks kept by the context.
+//
+// Device memory that a process has used and returned to the driver is slow
+// to get back: on MI355X a hipMalloc or hipMemCreate that needs such memory
+// costs ~40 ms per GiB, against well under a millisecond for memory never
+// used.  A growing det list asks for larger blocks at every build (a 1e7-det
+// FP64 value store is 156 GB), so freeing and reallocating them cost seconds
+// per build.  Blocks of at least one chunk are therefore made of physical
+// chunks (hipMemCreate; 1 GiB, or SHCI_CHUNK_MIB at context creation) mapped
+// into a reserved address range.  A block that no longer fits is resized in
+// place: chunks are mapped at its end (the range reserves room to grow) or
+// unmapped from it into the context's pool; a block given up altogether
+// returns its chunks to the pool.  Mapping and unmapping cost ~0.01 and
+// ~0.25 ms per 1 GiB chunk.  The pool goes back to the driver only on
+// shci_clear, shci_ctx_destroy, or when a smaller allocation finds no memory.
+// ---------------------------------------------------------------------------
+
+bool is_chunked(const shci_ctx* ctx, const void* p) { return p && ctx->chunked.count(const_cast<void*>(p)) != 0; }
+
+// Unmaps the chunks of block p past the first `keep` into the pool.  Kernels
+// still queued on the stream may use them, so the stream is drained first.
+int chunks_unmap(shci_ctx* ctx, void* p, size_t keep) {
+  shci_ctx::ChunkMap& m = ctx->chunked.at(p);
+  if (m.used.size() <= keep) return SHCI_OK;
+  HIPC(hipStreamSynchronize(ctx->stream));
+  if (log_alloc())
+    fprintf(stderr, "[shci] unmapped %zu chunks; block %zu chunks, pool %zu chunks\n", m.used.size() - keep, keep,
+            ctx->pool.size() + m.used.size() - keep);
+  while (m.used.size() > keep) {
+    const size_t i = m.used.size() - 1;
+    HIPC(hipMemUnmap(static_cast<char*>(p) + i * ctx->chunk, ctx->chunk));
+    ctx->pool.push_back(m.used[i]);
+    m.used.pop_back();
+  }
+  return SHCI_OK;
+}
+
+// Unmaps block p entirely and frees its address range.
+int chunks_drop(shci_ctx* ctx, void* p) {
+  CHECK_RC(chunks_unmap(ctx, p, 0));
+  const size_t reserved = ctx->chunked.at(p).reserved;
+  ctx->chunked.erase(p);
+  HIPC(hipMemAddressFree(p, reserved * ctx->chunk));
+  return SHCI_OK;
+}
+
+// Gives up to `count` of the pool's chunks (all by default) back to the driver.
+void pool_release(shci_ctx* ctx, size_t count = SIZE_MAX) {
+  const size_t k = std::min(count, ctx->pool.size());
+  for (size_t i = 0; i < k; i++) {
+    (void)hipMemRelease(ctx->pool.back());
+    ctx->pool.pop_back();
+  }
+  if (log_alloc() && k) fprintf(stderr, "[shci] released %zu pooled chunks\n", k);
+  ctx->mem_in_use -= k * ctx->chunk;
+}
+
+// Frees the cached blocks: hipMalloc blocks go back to the driver, chunked
+// blocks to the pool.
+int cache_trim(shci_ctx* ctx) {
   size_t total = 0;
+  int rc = SHCI_OK;
   for (DevBuf& c : ctx->cache) {
-    (void)hipFree(c.p);
-    ctx->mem_in_use -= c.bytes;
+    if (is_chunked(ctx, c.p)) {
+      const int r = chunks_drop(ctx, c.p);
+      if (r != SHCI_OK) rc = r;
+    } else {
+      (void)hipFree(c.p);
+      ctx->mem_in_use -= c.bytes;
+    }
     total += c.bytes;
   }
   if (log_alloc() && !ctx->cache.empty())
     fprintf(stderr, "[shci] cache trim: %zu blocks, %.2f GB\n", ctx->cache.size(), total / 1e9);
   ctx->cache.clear();
+  return rc;
 }
 
-// b holds at least `bytes`: kept if large enough, else replaced by the
-// smallest cached block of bytes .. 9/8 bytes + 64 MiB (a close fit, so that
-// the same sequence of requests finds the same blocks build after build),
-// else a new block (the
-// cache is freed and the allocation retried once if memory runs out).  A new
-// block of 1 GiB or more gets 1/16 headroom when memory allows: the builds of
-// a growing det list ask for slightly larger stores, sort buffers and tables
-// each time, and a reallocated block costs more than the build itself
-// (hipMalloc maps its pages: seconds for the ~100 GB of a 1e7-det build).
+// Device memory the context could hand out: free memory, its cached blocks
+// and its unmapped chunks.
+size_t available_bytes(shci_ctx* ctx) {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    free_b = 0;
+  }
+  for (const DevBuf& c : ctx->cache) free_b += c.bytes;
+  return free_b + ctx->pool.size() * ctx->chunk;
+}
+
+// Maps k more chunks at the end of block p: chunks of the pool first, then
+// new ones; when the device has no memory left for them, the cached chunked
+// blocks are taken apart (largest first: a replaced store rather than the
+// working buffers the same build asks for again), then the cached hipMalloc
+// blocks freed.
+int chunks_map(shci_ctx* ctx, void* p, size_t k) {
+  if (k == 0) return SHCI_OK;
+  hipMemAllocationProp prop = {};
+  prop.type = hipMemAllocationTypePinned;
+  prop.location.type = hipMemLocationTypeDevice;
+  prop.location.id = ctx->device;
+  size_t created = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  bool trimmed = false;
+  while (ctx->pool.size() < k) {
+    hipMemGenericAllocationHandle_t h;
+    const hipError_t e = hipMemCreate(&h, ctx->chunk, &prop, 0);
+    if (e == hipSuccess) {
+      ctx->pool.push_back(h);
+      ctx->mem_in_use += ctx->chunk;
+      created++;
+      continue;
+    }
+    (void)hipGetLastError();
+    if (e == hipErrorOutOfMemory) {
+      size_t big = ctx->cache.size();
+      for (size_t i = 0; i < ctx->cache.size(); i++)
+        if (is_chunked(ctx, ctx->cache[i].p) && (big == ctx->cache.size() || ctx->cache[i].bytes > ctx->cache[big].bytes))
+          big = i;
+      if (big < ctx->cache.size()) {
+        void* q = ctx->cache[big].p;
+        ctx->cache.erase(ctx->cache.begin() + big);
+        CHECK_RC(chunks_drop(ctx, q));
+        continue;
+      }
+      if (!trimmed && !ctx->cache.empty()) {
+        CHECK_RC(cache_trim(ctx));
+        trimmed = true;
+        continue;
+      }
+    }
+    return fail(ctx, e == hipErrorOutOfMemory ? SHCI_ERR_OOM : SHCI_ERR_HIP,
+                std::string("hipMemCreate(chunk) with ") + std::to_string(ctx->chunked.at(p).used.size()) +
+                    " of a block's chunks mapped and " + std::to_string(k - ctx->pool.size()) +
+                    " more needed: " + hipGetErrorString(e));
+  }
+  shci_ctx::ChunkMap& m = ctx->chunked.at(p);
+  if (m.used.size() + k > m.reserved) return fail(ctx, SHCI_ERR_HIP, "chunked block: address range exceeded");
+  const size_t first = m.used.size();
+  for (size_t i = 0; i < k; i++) {
+    const hipMemGenericAllocationHandle_t h = ctx->pool.back();
+    HIPC(hipMemMap(static_cast<char*>(p) + (first + i) * ctx->chunk, ctx->chunk, 0, h, 0));
+    ctx->pool.pop_back();
+    m.used.push_back(h);
+  }
+  hipMemAccessDesc acc = {};
+  acc.location.type = hipMemLocationTypeDevice;
+  acc.location.id = ctx->device;
+  acc.flags = hipMemAccessFlagsProtReadWrite;
+  HIPC(hipMemSetAccess(static_cast<char*>(p) + first * ctx->chunk, k * ctx->chunk, &acc, 1));
+  if (log_alloc())
+    fprintf(stderr, "[shci] mapped %zu chunks (%zu new) in %.1f ms; block %zu chunks, pool %zu chunks\n", k, created,
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(), m.used.size(),
+            ctx->pool.size());
+  return SHCI_OK;
+}
+
+// b holds at least `bytes`: kept if large enough, else replaced.
+// Blocks of at least one chunk are chunked: b itself is extended in place if
+// its address range has room, else the smallest cached chunked block of
+// n .. n + n/8 + 1 chunks is taken as it is, else a new range of twice the
+// chunks (room to grow) is reserved and mapped.  Smaller blocks: the smallest cached block of
+// bytes .. 9/8 bytes + 64 MiB (a close fit, so that the same sequence of
+// requests finds the same blocks build after build), else hipMalloc (if the
+// device is full, the cache and then the pool are freed and it is retried).
 int ensure(shci_ctx* ctx, DevBuf& b, size_t bytes) {
   if (bytes <= b.bytes && b.p) return SHCI_OK;
+  if (bytes >= ctx->chunk) {
+    const size_t n = (bytes + ctx->chunk - 1) / ctx->chunk;
+    if (is_chunked(ctx, b.p) && ctx->chunked.at(b.p).reserved >= n) {
+      CHECK_RC(chunks_map(ctx, b.p, n - ctx->chunked.at(b.p).used.size()));
+      b.bytes = n * ctx->chunk;
+      return SHCI_OK;
+    }
+    release(ctx, b);
+    // the smallest cached chunked block of n .. n + n/8 + 1 chunks, as it is
+    size_t fit = ctx->cache.size();
+    for (size_t i = 0; i < ctx->cache.size(); i++) {
+      void* q = ctx->cache[i].p;
+      if (!is_chunked(ctx, q)) continue;
+      const size_t u = ctx->chunked.at(q).used.size();
+      if (u >= n && u <= n + n / 8 + 1 && (fit == ctx->cache.size() || ctx->cache[i].bytes < ctx->cache[fit].bytes))
+        fit = i;
+    }
+    if (fit < ctx->cache.size()) {
+      b = ctx->cache[fit];
+      ctx->cache.erase(ctx->cache.begin() + fit);
+      return SHCI_OK;
+    }
+    void* p = nullptr;
+    const size_t reserved = 2 * n;
+    HIPC(hipMemAddressReserve(&p, reserved * ctx->chunk, ctx->chunk, nullptr, 0));
+    ctx->chunked[p].reserved = reserved;
+    const int rc = chunks_map(ctx, p, n);
+    if (rc != SHCI_OK) {
+      (void)chunks_drop(ctx, p);
+      return rc;
+    }
+    b.p = p;
+    b.bytes = n * ctx->chunk;
+    return SHCI_OK;
+  }
   release(ctx, b);
   if (bytes == 0) return SHCI_OK;
   size_t best = ctx->cache.size();
@@ -233,38 +423,28 @@ int ensure(shci_ctx* ctx, DevBuf& b, size_t bytes) {
     ctx->cache.erase(ctx->cache.begin() + best);
     return SHCI_OK;
   }
-  constexpr size_t kGranule = 2ull << 20;
-  size_t want = bytes;
-  if (bytes >= (1ull << 30)) {
-    // headroom only while an eighth of the device stays free after it
-    // (freeing cached blocks for it would make the next build allocate them
-    // again)
-    size_t free_b = 0, total_b = 0;
-    const size_t padded = (bytes + bytes / 16 + kGranule - 1) & ~(kGranule - 1);
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > padded + total_b / 8) want = padded;
-    (void)hipGetLastError();
-  }
   const auto t_alloc = std::chrono::steady_clock::now();
-  hipError_t e = hipMalloc(&b.p, want);
-  if (e == hipErrorOutOfMemory && want > bytes) {
-    (void)hipGetLastError();
-    want = bytes;
-    e = hipMalloc(&b.p, want);
-  }
+  hipError_t e = hipMalloc(&b.p, bytes);
   if (e == hipErrorOutOfMemory && !ctx->cache.empty()) {
     (void)hipGetLastError();
-    cache_trim(ctx);
-    e = hipMalloc(&b.p, want);
+    CHECK_RC(cache_trim(ctx));
+    e = hipMalloc(&b.p, bytes);
+  }
+  if (e == hipErrorOutOfMemory && !ctx->pool.empty()) {
+    (void)hipGetLastError();
+    pool_release(ctx, bytes / ctx->chunk + 2);
+    e = hipMalloc(&b.p, bytes);
   }
   if (e != hipSuccess) {
+    (void)hipGetLastError();
     b.p = nullptr;
     return fail(ctx, e == hipErrorOutOfMemory ? SHCI_ERR_OOM : SHCI_ERR_HIP,
                 std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
   }
-  b.bytes = want;
-  ctx->mem_in_use += want;
+  b.bytes = bytes;
+  ctx->mem_in_use += bytes;
   if (log_alloc())
-    fprintf(stderr, "[shci] hipMalloc %.3f GB (asked %.3f GB) in %.1f ms\n", want / 1e9, bytes / 1e9,
+    fprintf(stderr, "[shci] hipMalloc %.3f GB in %.1f ms\n", bytes / 1e9,
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_alloc).count());
   return SHCI_OK;
 }
@@ -3268,18 +3448,12 @@ int build_seg_layout(shci_ctx* ctx) {
   if (M >= (1ull << 31)) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment layout: n_dets * keys per det >= 2^31");
   if (K.tag_shift + 2 > 64) return fail(ctx, SHCI_ERR_UNSUPPORTED, "segment keys exceed 64 bits");
   // The sort needs ~36 B per key; value stores of a previous build are
-  // reallocated after it anyway, so drop them first when memory is short
-  // (free memory plus the context's cached blocks).
-  {
-    size_t free_b = 0, total_b = 0;
-    HIPC(hipMemGetInfo(&free_b, &total_b));
-    for (const DevBuf& c : ctx->cache) free_b += c.bytes;
-    if ((double)free_b < 40.0 * (double)M) {
-      DevBuf* big[] = {&ctx->sv64, &ctx->sv32, &ctx->seg_mem, &ctx->seg_desc, &ctx->seg_core, &ctx->seg_type,
-                       &ctx->seg_loc, &ctx->tile_tab, &ctx->tile_rec, &ctx->dav, &ctx->tmp};
-      for (DevBuf* b : big) release(ctx, *b);
-      cache_trim(ctx);
-    }
+  // reallocated after it anyway, so give them up first when memory is short
+  // (their chunks then serve the sort's buffers).
+  if ((double)available_bytes(ctx) < 40.0 * (double)M) {
+    DevBuf* big[] = {&ctx->sv64, &ctx->sv32, &ctx->seg_mem, &ctx->seg_desc, &ctx->seg_core, &ctx->seg_type,
+                     &ctx->seg_loc, &ctx->tile_tab, &ctx->tile_rec, &ctx->dav, &ctx->tmp};
+    for (DevBuf* b : big) release(ctx, *b);
   }
   // Keys of every (det, core) membership, radix-sorted: one 64-bit sort with
   // the region tag on top, or, when each region's key fits 32 bits, three
@@ -4829,10 +5003,8 @@ int select_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t
   S.ts = ts ? 1u : 0u;
   uint32_t bad = 0;
   // Candidate budget: ~64 B each (keys, sorted keys, flags, offsets, sort
-  // storage) within 3/4 of the free memory (plus the context's cached blocks).
-  size_t free_b = 0, total_b = 0;
-  HIPC(hipMemGetInfo(&free_b, &total_b));
-  for (const DevBuf& c : ctx->cache) free_b += c.bytes;
+  // storage) within 3/4 of the memory the context can hand out.
+  const size_t free_b = available_bytes(ctx);
   const uint64_t mem_cap = std::max<uint64_t>(1ull << 20, (uint64_t)(0.75 * (double)free_b / 64.0));
   uint64_t cap = mem_cap;
   if (const char* e = std::getenv("SHCI_SELECT_CAP")) {  // smaller buffers (tests of the fallback paths)
@@ -5004,9 +5176,7 @@ int pt_dtm_sweep(shci_ctx* ctx, const uint64_t* up, const uint64_t* dn, uint64_t
   CHECK_RC(ensure(ctx, off, (n + 1) * 8));
   S.cnt = P<uint64_t>(cnt);
   S.off = P<uint64_t>(off);
-  size_t free_b = 0, total_b = 0;
-  HIPC(hipMemGetInfo(&free_b, &total_b));
-  for (const DevBuf& c : ctx->cache) free_b += c.bytes;
+  const size_t free_b = available_bytes(ctx);
   // ~104 B per pair: keys and values, their sorted copies, flags, offsets,
   // run starts and contributions, sort storage
   const uint64_t cap = std::max<uint64_t>(1ull << 20, (uint64_t)(0.75 * (double)free_b / 104.0));
@@ -5129,6 +5299,14 @@ int shci_ctx_create(shci_ctx** out, int device) {
   if (!out) return SHCI_ERR_ARG;
   auto* ctx = new shci_ctx();
   ctx->device = device;
+  if (const char* e = getenv("SHCI_CHUNK_MIB")) {
+    const unsigned long long mib = strtoull(e, nullptr, 10);
+    if (mib < 2 || mib % 2 || mib > (1ull << 20)) {
+      *out = ctx;
+      return fail(ctx, SHCI_ERR_ARG, "SHCI_CHUNK_MIB must be an even number of MiB in [2, 2^20]");
+    }
+    ctx->chunk = (size_t)mib << 20;
+  }
   *out = ctx;
   HIPC(hipSetDevice(device));
   HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -5152,9 +5330,10 @@ int shci_ctx_destroy(shci_ctx* ctx) {
                     &ctx->row_ptr, &ctx->row_end, &ctx->col, &ctx->v64, &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb,
                     &ctx->va, &ctx->vb, &ctx->scal, &ctx->part, &ctx->xv, &ctx->yv, &ctx->x32, &ctx->y32,
                     &ctx->dav, &ctx->sq_S, &ctx->sq_r, &ctx->sq_len, &ctx->sq_max, &ctx->hq_H, &ctx->hq_rs, &ctx->hq_len, &ctx->hq_max,
-                    &ctx->sel_up, &ctx->sel_dn, &ctx->ts_rev, &ctx->ts_diag};
+                    &ctx->sel_up, &ctx->sel_dn, &ctx->ts_rev, &ctx->ts_diag, &ctx->cap};
   for (DevBuf* b : bufs) release(ctx, *b);
-  cache_trim(ctx);
+  (void)cache_trim(ctx);
+  pool_release(ctx);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -5791,7 +5970,8 @@ int shci_clear(shci_ctx* ctx) {
                     &ctx->v32, &ctx->len, &ctx->tmp, &ctx->ka, &ctx->kb, &ctx->va, &ctx->vb, &ctx->xv,
                     &ctx->yv, &ctx->x32, &ctx->y32, &ctx->dav, &ctx->cap};
   for (DevBuf* b : bufs) release(ctx, *b);
-  cache_trim(ctx);
+  CHECK_RC(cache_trim(ctx));
+  pool_release(ctx);
   ctx->cap_n = 0;  // the captured vectors belonged to the dropped det list
   ctx->n = 0;
   ctx->nnz = 0;
