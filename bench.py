@@ -547,6 +547,9 @@ def main():
         "spmv_fp32_csr_equiv_GBps": (F * 8 + n_dets * 32) / spmv32_ms / 1e6,
         "spmv_fp64_csr_equiv_GBps": (F * 12 + n_dets * 40) / spmv64_ms / 1e6,
         "grow_s": t_grow,
+        # Hamiltonian::update calls of the growth (FP64 store, wall time with
+        # the dets' upload): the last is the append of the final sweep
+        "growth_updates": [dict(n_dets=r["n_dets"], new_dets=r["n_new"], update_s=r["t_build"]) for r in w.log],
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -107,7 +107,8 @@ int shci_update_existing_elems(shci_ctx* ctx);
 
 /* Hamiltonian<S>::clear / SparseMatrix::clear (hamiltonian.h:128-155,
  * sparse_matrix.cc:93-100): release the determinants, the matrix and all
- * work buffers (the integrals stay). */
+ * work buffers (the integrals stay); their device memory, cached blocks and
+ * pooled chunks included, goes back to the driver. */
 int shci_clear(shci_ctx* ctx);
 
 /* SparseMatrix::count_n_elems (sparse_matrix.cc:13-26) and row count:
@@ -259,7 +260,8 @@ int shci_segment_ranges(shci_ctx* ctx, uint64_t* out);
  * slots hold zeros. */
 int shci_segment_kept(shci_ctx* ctx, uint64_t* out);
 
-/* Device memory in use by the context (bytes). */
+/* Device memory held by the context (bytes): its buffers, its cached blocks
+ * and its pooled physical chunks. */
 int shci_memory(shci_ctx* ctx, uint64_t* bytes);
 
 #ifdef __cplusplus
