@@ -285,6 +285,32 @@ def copy_calibration():
     log(f"[bench] copy calibration: 3 x torch.mul over {a.numel() * 4 / 2**30:.0f} GiB read + written")
 
 
+def build_bench(w, args):
+    """The build alone: the first FP32 build of the list (with its upload),
+    then --reps device-resident rebuilds, each timed with HIP events by the
+    library (shci_get_build_timings) and on the host clock."""
+    from shci_amd.hamiltonian import Hamiltonian
+
+    ham = Hamiltonian(w.n, w.h1, w.eri, w.core, store="fp32", layout=args.layout)
+    t0 = time.time()
+    ham.update(w.up, w.dn)
+    first_s = time.time() - t0
+    walls, phases = [], []
+    for _ in range(args.reps):
+        t0 = time.time()
+        ham.rebuild()
+        walls.append((time.time() - t0) * 1e3)
+        phases.append(ham.matrix.build_timings())
+    ms = float(np.median([p["total"] for p in phases]))
+    log(f"[bench] FP32 build of {len(w.up):,} dets: first (with upload) {first_s:.3f} s; resident rebuild "
+        f"{ms:.1f} ms (median of {args.reps}; host clock {np.median(walls):.1f} ms); phases {phases[-1]}")
+    print(json.dumps({"metric": "H-build dets/s (device-resident FP32 rebuild)", "value": len(w.up) / (ms / 1e3),
+                      "unit": "dets/s", "n_gpus": 1, "higher_is_better": True, "build_ms": ms,
+                      "build_host_ms": float(np.median(walls)), "first_build_s": first_s, "phases_ms": phases[-1],
+                      "config": {"workload": f"cfg{args.config}", "n_dets": len(w.up),
+                                 "det_list_digest": w.det_hash, "layout": args.layout}}))
+
+
 def spmv_bench(w, args):
     """The products alone: a device-resident build of each store (FP32, then
     FP64) and --reps timed products (HIP events on the library's stream,
@@ -338,7 +364,11 @@ def main():
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the products: FP32 then FP64 store of the --layout, --reps timed products each "
                          "(HIP events), one JSON line; the workload rocprofv3 profiles are taken on")
-    ap.add_argument("--reps", type=int, default=10, help="products per store in --spmv-only")
+    ap.add_argument("--reps", type=int, default=10, help="products per store in --spmv-only, rebuilds in "
+                                                             "--build-only")
+    ap.add_argument("--build-only", action="store_true",
+                    help="only the Hamiltonian build: --reps device-resident FP32 rebuilds of the --layout after "
+                         "the first build, one JSON line; the workload of the build's kernel-trace profile")
     ap.add_argument("--autograd", action="store_true",
                     help="time E(h, eri) forward/backward (RDM adjoint) and check 20 finite-difference "
                          "directions on the --config workload (use --config 4), one JSON line")
@@ -359,6 +389,9 @@ def main():
         return
     if args.spmv_only:
         spmv_bench(w, args)
+        return
+    if args.build_only:
+        build_bench(w, args)
         return
     n_dets = len(w.up)
     ns = w.spec.n_states
