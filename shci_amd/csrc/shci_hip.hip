@@ -1773,8 +1773,7 @@ __global__ void k_seg_keep(const uint32_t* __restrict__ rstart, const uint32_t* 
 constexpr uint32_t kTileCap = 1920;
 constexpr uint32_t kTileMax = 1984;   // > kTileCap + 64: memberships (and dets) of one tile
 constexpr uint32_t kTileHash = 4096;  // LDS hash slots of the table build
-constexpr uint32_t kTileSort = 2048;  // the table build's bitonic sort pads its keys to a power of two
-static_assert(kTileSort >= kTileMax && (kTileSort & (kTileSort - 1)) == 0, "sort buffer below the tile size");
+static_assert(kTileMax % 4 == 0, "the table ranks read the keys four at a time");
 static_assert(kTileHash >= 2 * kTileMax, "hash table below twice the tile size");
 
 // Per kept run (index f in run order): spin-group key and the memberships it
@@ -1885,9 +1884,41 @@ __global__ void k_tile_check(const uint32_t* __restrict__ cnt, uint64_t T, unsig
   if (t < T && cnt[t] >= kTileMax) atomicOr(bad, 1u);
 }
 
+// Ranks of the keys i = threadIdx.x + 256 j (j < NK, i < total) among the
+// total distinct keys in LDS (padded with 0xFFFFFFFF to a multiple of 4):
+// rank[i] and out[rank] = key.
+template <int NK>
+__device__ __forceinline__ void tile_rank(const uint32_t* key, uint32_t total, uint16_t* rank,
+                                          uint32_t* __restrict__ out) {
+  uint32_t kk[NK], rr[NK];
+#pragma unroll
+  for (int j = 0; j < NK; j++) {
+    const uint32_t i = threadIdx.x + 256u * j;
+    kk[j] = i < total ? key[i] : 0u;
+    rr[j] = 0;
+  }
+  const uint4* k4 = reinterpret_cast<const uint4*>(key);
+  const uint32_t n4 = (total + 3) / 4;
+#pragma unroll 8
+  for (uint32_t q = 0; q < n4; q++) {
+    const uint4 v = k4[q];
+#pragma unroll
+    for (int j = 0; j < NK; j++)
+      rr[j] += (v.x < kk[j] ? 1u : 0u) + (v.y < kk[j] ? 1u : 0u) + (v.z < kk[j] ? 1u : 0u) + (v.w < kk[j] ? 1u : 0u);
+  }
+#pragma unroll
+  for (int j = 0; j < NK; j++) {
+    const uint32_t i = threadIdx.x + 256u * j;
+    if (i < total) {
+      rank[i] = (uint16_t)rr[j];
+      out[rr[j]] = kk[j];
+    }
+  }
+}
+
 // One workgroup per tile: the distinct dets of its memberships (LDS hash
 // table with linear probing; at most kTileMax keys in kTileHash slots), their
-// table tab[u0 + i] in ascending order (bitonic sort in LDS) and each
+// table tab[u0 + i] in ascending order (ranks by counting in LDS) and each
 // membership's table index.  A tiles list beta-order positions (the SpMV runs
 // them on the beta-order copies of x and y, where a beta group's dets are
 // adjacent), the others internal det ids; nu[t] = table size | (A tile << 31).
@@ -1901,8 +1932,7 @@ __global__ __launch_bounds__(256) void k_tile_tables(const uint32_t* __restrict_
   __shared__ uint32_t hk[kTileHash];
   __shared__ uint16_t cid[kTileHash];
   __shared__ uint16_t sl[kTileMax];
-  __shared__ uint32_t key[kTileSort];
-  __shared__ uint16_t pay[kTileSort];
+  __shared__ __align__(16) uint32_t key[kTileMax];  // read four at a time by tile_rank
   __shared__ uint16_t rank[kTileMax];
   __shared__ uint32_t wtot[4];
   const uint32_t t = blockIdx.x, tid = threadIdx.x;
@@ -1961,52 +1991,24 @@ __global__ __launch_bounds__(256) void k_tile_tables(const uint32_t* __restrict_
     if (hk[h] != 0xFFFFFFFFu) {
       cid[h] = (uint16_t)base;
       key[base] = hk[h];
-      pay[base] = (uint16_t)base;
       base++;
     }
   }
   __syncthreads();
-  if (total <= 512) {
-    // small tables: rank = number of smaller keys (the keys are distinct;
-    // every lane reads the same key per iteration, an LDS broadcast)
-    for (uint32_t i = tid; i < total; i += 256) {
-      const uint32_t ki = key[i];
-      uint32_t r = 0;
-      for (uint32_t q = 0; q < total; q++) r += key[q] < ki ? 1u : 0u;
-      rank[i] = (uint16_t)r;
-      tab[(uint64_t)u0[t] + r] = ki;
-    }
-  } else {
-    // bitonic sort of (key, dense index) pairs, padded to a power of two
-    uint32_t P = 2;
-    while (P < total) P <<= 1;
-    for (uint32_t i = total + tid; i < P; i += 256) {
-      key[i] = 0xFFFFFFFFu;
-      pay[i] = 0;
-    }
-    __syncthreads();
-    for (uint32_t k = 2; k <= P; k <<= 1) {
-      for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
-        for (uint32_t i = tid; i < P; i += 256) {
-          const uint32_t ixj = i ^ jj;
-          if (ixj > i) {
-            const uint32_t ka = key[i], kb = key[ixj];
-            if ((ka > kb) == ((i & k) == 0)) {
-              key[i] = kb;
-              key[ixj] = ka;
-              const uint16_t pa = pay[i];
-              pay[i] = pay[ixj];
-              pay[ixj] = pa;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
-    for (uint32_t i = tid; i < total; i += 256) {
-      rank[pay[i]] = (uint16_t)i;
-      tab[(uint64_t)u0[t] + i] = key[i];
-    }
+  // rank = number of smaller keys (the keys are distinct): each thread ranks
+  // its keys i = tid + 256 j against all keys, read four at a time (the key
+  // array padded with 0xFFFFFFFF, larger than any det id); every lane reads
+  // the same keys, an LDS broadcast
+  for (uint32_t i = total + tid; i < ((total + 3) & ~3u); i += 256) key[i] = 0xFFFFFFFFu;
+  __syncthreads();
+  const uint32_t nk = (total + 255) / 256;  // keys per thread (<= 8), uniform over the workgroup
+  switch (nk) {
+    case 0: break;
+    case 1: tile_rank<1>(key, total, rank, tab + u0[t]); break;
+    case 2: tile_rank<2>(key, total, rank, tab + u0[t]); break;
+    case 3: tile_rank<3>(key, total, rank, tab + u0[t]); break;
+    case 4: tile_rank<4>(key, total, rank, tab + u0[t]); break;
+    default: tile_rank<8>(key, total, rank, tab + u0[t]); break;
   }
   if (tid == 0) nu[t] = total | (isA ? 0x80000000u : 0u);
   __syncthreads();
