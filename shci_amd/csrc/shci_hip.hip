@@ -1625,25 +1625,37 @@ struct SegKeyParams {
   const uint32_t* __restrict__ dnpos;  // internal det -> position in beta order
   int bC, bA, bB;  // widths of the low key fields
   int tag_shift;
+  uint64_t eC, eA;      // first key index of the A and B regions
+  uint64_t dC, dA, dB;  // ceil(2^64 / P) of PC, PA, PB (P >= 2), for seg_decode's divisions
 };
 
-// Region, det and removal pattern of key index t.
+// ceil(2^64 / d) for d >= 2 (0 otherwise).
+inline uint64_t div_mul(uint32_t d) { return d >= 2 ? UINT64_MAX / d + 1 : 0; }
+
+// floor(t / d) for t < 2^33: the high half of t * ceil(2^64 / d) is t / d
+// plus less than 2^-31, which cannot reach the next integer since t / d
+// falls short of it by at least 1/d (d < 2^31).  A multiply instead of the
+// long integer division.
+__device__ __forceinline__ uint64_t div_by(uint64_t t, uint32_t d, uint64_t mul) {
+  return d >= 2 ? __umul64hi(t, mul) : t;
+}
+
+// Region, det and removal pattern of key index t (t < 2^31).
 __device__ __forceinline__ void seg_decode(const SegKeyParams& K, uint64_t t, uint32_t& type, uint64_t& i,
                                            uint32_t& k) {
-  const uint64_t eC = K.n * K.PC, eA = eC + K.n * K.PA;
-  if (t < eC) {
+  if (t < K.eC) {
     type = kSegC;
-    i = t / K.PC;
+    i = div_by(t, K.PC, K.dC);
     k = (uint32_t)(t - i * K.PC);
-  } else if (t < eA) {
+  } else if (t < K.eA) {
     type = kSegA;
-    const uint64_t u = t - eC;
-    i = u / K.PA;
+    const uint64_t u = t - K.eC;
+    i = div_by(u, K.PA, K.dA);
     k = (uint32_t)(u - i * K.PA);
   } else {
     type = kSegB;
-    const uint64_t u = t - eA;
-    i = u / K.PB;
+    const uint64_t u = t - K.eA;
+    i = div_by(u, K.PB, K.dB);
     k = (uint32_t)(u - i * K.PB);
   }
 }
@@ -1707,14 +1719,13 @@ __global__ void k_seg_useful(SegKeyParams K, const uint32_t* __restrict__ vals_s
   const uint32_t a = rstart[r], b = rstart[r + 1];
   uint32_t useful = 0;
   if (b - a >= 2) {
-    uint32_t ty, k, ty0, k0;
-    uint64_t i, i0;
+    uint32_t ty, k;
+    uint64_t i;
     seg_decode(K, vals_s[p], ty, i, k);
-    seg_decode(K, vals_s[a], ty0, i0, k0);
-    const uint64_t core = ty == kSegC ? 0ull : ty == kSegA ? drop_electrons(K.up[i0], k0, K.ra)
-                                                           : drop_electrons(K.dn[i0], k0, K.rb);
-    const uint32_t nrm = ty == kSegA ? K.ra : K.rb;
     const uint64_t ui = K.up[i], di = K.dn[i];
+    // the run's core: this member's string without its removal pattern k
+    const uint64_t core = ty == kSegC ? 0ull : ty == kSegA ? drop_electrons(ui, k, K.ra) : drop_electrons(di, k, K.rb);
+    const uint32_t nrm = ty == kSegA ? K.ra : K.rb;
     // four candidates per trip, so that their det loads are in flight together
     constexpr uint32_t Q = 4;
     for (uint32_t q0 = a; q0 < b && !useful; q0 += Q) {
@@ -3442,6 +3453,11 @@ SegKeyParams seg_key_params(shci_ctx* ctx) {
   K.bB = bits_for(rB);
   K.tag_shift = std::max({bits_for(ctx->r_alpha) + K.bC, bits_for(ctx->n_dgroups) + K.bA,
                           bits_for(ctx->n_ugroups) + K.bB});
+  K.eC = K.n * K.PC;
+  K.eA = K.eC + K.n * K.PA;
+  K.dC = div_mul(K.PC);
+  K.dA = div_mul(K.PA);
+  K.dB = div_mul(K.PB);
   return K;
 }
 
