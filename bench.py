@@ -326,6 +326,13 @@ def spmv_bench(w, args):
         ms, by = ham.matrix.time_spmv(fp32=store == "fp32", reps=args.reps)
         log(f"[bench] {store} product {ms:.3f} ms, {by / 1e9:.2f} GB algorithmic, {by / ms / 1e6:.0f} GB/s")
         out[store] = dict(ms=ms, algorithmic_bytes=by, GBps=by / ms / 1e6, frac=by / ms / 1e6 / HBM_PEAK_GBS)
+        if store == "fp32" and w.spec.n_states > 1:
+            # the block product of the multi-root Davidson: 4 FP32 vectors per pass over the values
+            bms, bby = ham.matrix.time_spmm(4, reps=args.reps)
+            log(f"[bench] fp32 4-vector block product {bms:.3f} ms ({bms / 4:.3f} ms per root, "
+                f"{ms / (bms / 4):.2f}x per root against the single product)")
+            out["fp32_block4"] = dict(ms=bms, ms_per_root=bms / 4, speedup_per_root=ms / (bms / 4),
+                                      algorithmic_bytes=bby, GBps=bby / bms / 1e6)
         ham.clear()
     out["value"] = out["fp32"]["GBps"]
     out["fp32_over_fp64"] = out["fp32"]["ms"] / out["fp64"]["ms"]
