@@ -2663,11 +2663,78 @@ __device__ __forceinline__ void tile_class_run(uint32_t n, uint32_t k0, uint32_t
   }
 }
 
+// Pair sums of one segment of M members (M <= 8) held by one lane: slot
+// (d - 1) * M + j of the segment's rotation rows is the pair (j, j + d mod M)
+// (d <= (M - 1) / 2), and for an even M the half row's slot
+// ((M - 1) / 2) * M + j the pair (j, j + M / 2), j < M / 2.
+template <int M, typename XT>
+__device__ __forceinline__ void small_pairs(const XT (&v)[28], const XT (&x)[8], XT (&y)[8]) {
+#pragma unroll
+  for (int d = 1; d <= (M - 1) / 2; d++)
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+      const int k = (j + d) % M;
+      const XT w = v[(d - 1) * M + j];
+      y[j] = fma(w, x[k], y[j]);
+      y[k] = fma(w, x[j], y[k]);
+    }
+  if constexpr (M % 2 == 0) {
+#pragma unroll
+    for (int j = 0; j < M / 2; j++) {
+      const XT w = v[((M - 1) / 2) * M + j];
+      y[j] = fma(w, x[j + M / 2], y[j]);
+      y[j + M / 2] = fma(w, x[j], y[j + M / 2]);
+    }
+  }
+}
+
+// The class runs of 5-8, 3-4 and 2 members of a tile (LDS entries k3 ..
+// k3 + n, three runs in order, segments largest first within a run): one
+// lane per segment, its slots loaded at once, its members' x from the
+// tile's LDS x, its sums added to the LDS y.
+template <typename VT, typename XT>
+__device__ __forceinline__ void tile_small_runs(const uint32_t (&nseg)[6], const uint32_t (&k0)[6],
+                                                const uint32_t (&m0)[6], const uint64_t (&voff0)[6],
+                                                const VT* __restrict__ val, TileLds<XT>& L, const TileBody& B) {
+  const uint32_t kb = k0[3], ke = k0[5] + nseg[5];
+  for (uint32_t e = kb + threadIdx.x; e < ke; e += 256) {
+    const int c = e >= k0[5] ? 5 : e >= k0[4] ? 4 : 3;
+    const uint32_t pk = B.sp[e], m = B.sm[e];
+    const uint32_t moff = m0[c] + (pk & 0x7FFu);
+    const VT* __restrict__ base = val + voff0[c] + (pk >> 11);
+    const uint32_t ns = m * (m - 1) / 2;
+    XT v[28];
+#pragma unroll
+    for (int t = 0; t < 28; t++) v[t] = (uint32_t)t < ns ? (XT)base[t] : (XT)0;
+    uint32_t l[8];
+    XT x[8], y[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      l[j] = (uint32_t)j < m ? B.sl[moff + j] : 0u;
+      x[j] = (uint32_t)j < m ? L.xs[l[j]] : (XT)0;
+      y[j] = (XT)0;
+    }
+    switch (m) {
+      case 8: small_pairs<8, XT>(v, x, y); break;
+      case 7: small_pairs<7, XT>(v, x, y); break;
+      case 6: small_pairs<6, XT>(v, x, y); break;
+      case 5: small_pairs<5, XT>(v, x, y); break;
+      case 4: small_pairs<4, XT>(v, x, y); break;
+      case 3: small_pairs<3, XT>(v, x, y); break;
+      default: small_pairs<2, XT>(v, x, y); break;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if ((uint32_t)j < m) atomicAdd(&L.ys[l[j]], y[j]);
+  }
+}
+
 // Off-diagonal part of the segments with m <= 64, one workgroup per tile: the
 // tile's record body copied into LDS with 16-byte loads (its header read
 // through scalar loads), its x gathered (one load per distinct det), its class
-// runs with lane groups of 64, 32, 16, 8, 4, 2 lanes, member sums into an LDS
-// y, then one global atomic per distinct det.  VT stored value type, XT
+// runs of 33-64, 17-32 and 9-16 members with lane groups of 64, 32 and 16
+// lanes and those of at most 8 members with one lane per segment, member sums
+// into an LDS y, then one global atomic per distinct det.  VT stored value type, XT
 // vector type.
 template <typename VT, typename XT>
 __device__ __forceinline__ void seg_spmv_tile(const uint32_t* __restrict__ roff, const uint4* __restrict__ rec,
@@ -2713,9 +2780,7 @@ __device__ __forceinline__ void seg_spmv_tile(const uint32_t* __restrict__ roff,
   tile_class_run<64, MX ? 1 : 2, MX ? 16 : 8, VT, XT>(nseg[0], k0[0], m0[0], voff0[0], val, L, B);
   tile_class_run<32, MX ? 1 : 2, MX ? 16 : 8, VT, XT>(nseg[1], k0[1], m0[1], voff0[1], val, L, B);
   tile_class_run<16, MX ? 1 : 2, 8, VT, XT>(nseg[2], k0[2], m0[2], voff0[2], val, L, B);
-  tile_class_run<8, 4, 4, VT, XT>(nseg[3], k0[3], m0[3], voff0[3], val, L, B);
-  tile_class_run<4, 4, 2, VT, XT>(nseg[4], k0[4], m0[4], voff0[4], val, L, B);
-  tile_class_run<2, 4, 1, VT, XT>(nseg[5], k0[5], m0[5], voff0[5], val, L, B);
+  tile_small_runs<VT, XT>(nseg, k0, m0, voff0, val, L, B);
   __syncthreads();
   for (uint32_t u = threadIdx.x; u < U; u += 256) atomicAdd(&y[tb[u]], L.ys[u]);
 }
