@@ -2264,10 +2264,44 @@ __global__ __launch_bounds__(256) void k_seg_fill_small(SegFillParams F) {
   seg_count(F, kept);
 }
 
+// c2 of orbital indices and of their pair indices (below 2^12): 32-bit form.
+__device__ __forceinline__ uint32_t c2s(uint32_t a, uint32_t b) {
+  const uint32_t hi = max(a, b), lo = min(a, b);
+  return hi * (hi + 1u) / 2u + lo;
+}
+
+// A member of a mixed-core (C) segment over the segment's core strings cu, cd
+// (alpha and beta, one electron fewer each): the alpha orbital a and beta
+// orbital b it adds, and the parity r of the core orbitals below a plus those
+// below b, packed as a | b << 7 | r << 14.  For two members j, l with a_j !=
+// a_l and b_j != b_l, double_mixed's excitation is a_j -> a_l, b_j -> b_l;
+// its below() counts are the core counts below those orbitals, so its sign is
+// (-1)^(r_j + r_l) and its integral (a_j a_l | b_j b_l).  When every member
+// adds the same alpha (or beta) orbital, the AND of the members' strings
+// holds it: a (or b) is then 64 for all members and no pair is owned.
+__device__ __forceinline__ uint32_t c_member_word(uint64_t u, uint64_t d, uint64_t cu, uint64_t cd) {
+  const uint64_t xu = u & ~cu, xd = d & ~cd;
+  const uint32_t a = xu ? (uint32_t)ctz64(xu) : 64u, b = xd ? (uint32_t)ctz64(xd) : 64u;
+  const uint32_t r = (uint32_t)(__popcll(cu & ((1ull << (a & 63u)) - 1ull)) +
+                                __popcll(cd & ((1ull << (b & 63u)) - 1ull))) & 1u;
+  return a | b << 7 | r << 14;
+}
+
+// The value index and sign of the C pair (j, l) from their member words, and
+// whether the segment owns it (both spins change).
+__device__ __forceinline__ bool c_pair(uint32_t wj, uint32_t wl, uint32_t& idx, double& sgn) {
+  const uint32_t x = wj ^ wl;
+  idx = c2s(c2s(wj & 0x7Fu, wl & 0x7Fu), c2s((wj >> 7) & 0x7Fu, (wl >> 7) & 0x7Fu));
+  sgn = (x >> 14) & 1u ? -1.0 : 1.0;
+  return (x & 0x7Fu) != 0u && (x & (0x7Fu << 7)) != 0u;
+}
+
 // Mixed-core (C) segments at build time: every owned pair is an opposite-spin
 // double whose value is sign * (pu qu | pd qd) (chem_system.cc:599-607, as in
-// double_mixed).  Four steps per iteration: the partner dets, ownership tests,
-// signs and integral indices first, then four unconditional integral loads.
+// double_mixed), computed from the two members' words (c_member_word; one
+// 32-bit lane exchange per slot).  Four steps per iteration: the partner
+// words, ownership tests, signs and integral indices first, then four
+// unconditional integral loads.
 template <int S>
 __global__ __launch_bounds__(256) void k_seg_fill_c(SegFillParams F) {
   constexpr int U = 4;
@@ -2278,38 +2312,36 @@ __global__ __launch_bounds__(256) void k_seg_fill_c(SegFillParams F) {
   if (s < F.s1) D = F.desc[s];
   const uint32_t m = D.m, mh = m >> 1;
   const bool valid = j < m;
-  uint64_t uj = 0, dj = 0;
+  // the members' strings; the AND over a segment's lanes is its core
+  uint64_t uj = ~0ull, dj = ~0ull;
   if (valid) {
     const uint32_t det = F.mem[D.moff + j];
     uj = F.up[det];
     dj = F.dn[det];
   }
+  uint64_t cu = uj, cd = dj;
+#pragma unroll
+  for (int o = 1; o < S; o <<= 1) {
+    cu &= shfl64(cu, (int)(lane_id() ^ (uint32_t)o));
+    cd &= shfl64(cd, (int)(lane_id() ^ (uint32_t)o));
+  }
+  const uint32_t wj = c_member_word(uj, dj, cu, cd);
   // Steps of the wave's largest segment (segments of a range are in group order).
   uint32_t steps = mh;
   for (int o = 32; o > 0; o >>= 1) steps = max(steps, (uint32_t)__shfl_xor((int)steps, o, kWave));
   uint32_t kept = 0, p = j;
   for (uint32_t d0 = 1; d0 <= steps; d0 += U) {
-    uint64_t idx[U];
+    uint32_t idx[U];
     double sgn[U];
     bool live[U], own[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const uint32_t d = d0 + u;
       if (valid) p = p + 1 == m ? 0 : p + 1;
-      const uint64_t ul = shfl64(uj, gbase + (int)p), dl = shfl64(dj, gbase + (int)p);
+      const uint32_t wl = (uint32_t)__shfl((int)wj, gbase + (int)p, kWave);
       live[u] = valid && d <= mh && (2 * d != m || j < mh);
-      own[u] = live[u] && uj != ul && dj != dl;
-      idx[u] = 0;
-      sgn[u] = 1.0;
-      if (own[u]) {
-        const uint64_t lu = uj & ~ul, ru = ul & ~uj, ld = dj & ~dl, rd = dl & ~dj;
-        const int pu = ctz64(lu), qu = ctz64(ru), pd = ctz64(ld), qd = ctz64(rd);
-        const int cu = below(uj, pu) + below(ul, qu);
-        const int cd = below(dj, pd) + below(dl, qd);
-        const int pf = ((cu & 1) ? -1 : 1) * ((cd & 1) ? -1 : 1);
-        idx[u] = c2(c2(pu, qu), c2(pd, qd));
-        sgn[u] = (double)pf;
-      }
+      own[u] = c_pair(wj, wl, idx[u], sgn[u]) && live[u];
+      if (!own[u]) idx[u] = 0;
     }
     double e[U];
 #pragma unroll
@@ -2365,9 +2397,9 @@ __global__ __launch_bounds__(64) void k_seg_fill_large(SegFillParams F) {
 }
 
 // Mixed-core (C) segments of m > 64 at build time: one wavefront per segment,
-// member strings in LDS, each lane walking its slots (d, j), (d, j + 64), ...
-// four at a time so that four integral loads are in flight.  Values as in
-// k_seg_fill_c.
+// member words (c_member_word) in LDS, each lane walking its slots (d, j),
+// (d, j + 64), ... four at a time so that four integral loads are in flight.
+// Values as in k_seg_fill_c.  Dynamic LDS: 4 bytes per member.
 __global__ __launch_bounds__(64) void k_seg_fill_large_c(SegFillParams F) {
   extern __shared__ __align__(16) unsigned char seg_lds[];
   constexpr int U = 4;
@@ -2375,18 +2407,29 @@ __global__ __launch_bounds__(64) void k_seg_fill_large_c(SegFillParams F) {
   if (s >= F.s1) return;
   const SegDesc D = F.desc[s];
   const uint32_t m = D.m, mh = m >> 1, t = threadIdx.x;
-  uint64_t* su = reinterpret_cast<uint64_t*>(seg_lds);
-  uint64_t* sd = su + m;
+  uint32_t* sw = reinterpret_cast<uint32_t*>(seg_lds);
+  // the core: the AND of the members' strings
+  uint64_t cu = ~0ull, cd = ~0ull;
   for (uint32_t j = t; j < m; j += kWave) {
     const uint32_t det = F.mem[D.moff + j];
-    su[j] = F.up[det];
-    sd[j] = F.dn[det];
+    cu &= F.up[det];
+    cd &= F.dn[det];
+  }
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    cu &= shfl64(cu, (int)(lane_id() ^ (uint32_t)o));
+    cd &= shfl64(cd, (int)(lane_id() ^ (uint32_t)o));
+  }
+  for (uint32_t j = t; j < m; j += kWave) {
+    const uint32_t det = F.mem[D.moff + j];
+    sw[j] = c_member_word(F.up[det], F.dn[det], cu, cd);
   }
   __syncthreads();
   uint32_t kept = 0;
   uint32_t d = 1, j = t, lim = m;  // slots of step d (m > 64, so step 1 has m); j < 64 < lim
   while (d <= mh) {
-    uint64_t idx[U], slot[U];
+    uint32_t idx[U];
+    uint64_t slot[U];
     double sgn[U];
     bool live[U], own[U];
 #pragma unroll
@@ -2399,18 +2442,9 @@ __global__ __launch_bounds__(64) void k_seg_fill_large_c(SegFillParams F) {
       if (live[u]) {
         uint32_t p = j + d;
         if (p >= m) p -= m;
-        const uint64_t uj = su[j], dj = sd[j], ul = su[p], dl = sd[p];
         slot[u] = D.voff + (uint64_t)(d - 1) * m + j;
-        own[u] = uj != ul && dj != dl;
-        if (own[u]) {
-          const uint64_t lu = uj & ~ul, ru = ul & ~uj, ld = dj & ~dl, rd = dl & ~dj;
-          const int pu = ctz64(lu), qu = ctz64(ru), pd = ctz64(ld), qd = ctz64(rd);
-          const int cu = below(uj, pu) + below(ul, qu);
-          const int cd = below(dj, pd) + below(dl, qd);
-          const int pf = ((cu & 1) ? -1 : 1) * ((cd & 1) ? -1 : 1);
-          idx[u] = c2(c2(pu, qu), c2(pd, qd));
-          sgn[u] = (double)pf;
-        }
+        own[u] = c_pair(sw[j], sw[p], idx[u], sgn[u]);
+        if (!own[u]) idx[u] = 0;
         // next slot of this lane: 64 further in the rotation order
         j += kWave;
         while (d <= mh && j >= lim) {
@@ -3793,7 +3827,7 @@ int seg_fill(shci_ctx* ctx, int update) {
     if (cnt == 0) continue;
     if (c < 3) {
       if (!update && r % 3 == (int)kSegC)
-        k_seg_fill_large_c<<<cnt, kWave, ctx->seg_maxm[r] * 16, st>>>(F);
+        k_seg_fill_large_c<<<cnt, kWave, ctx->seg_maxm[r] * 4, st>>>(F);
       else
         k_seg_fill_large<<<cnt, kWave, ctx->seg_maxm[r] * 16, st>>>(F);
       continue;
