@@ -114,10 +114,14 @@ def test_spmv_fp64_fp32(system, layout):
     np.testing.assert_allclose(y64, y_ref, rtol=1e-12, atol=1e-11)
     y32 = ham.matrix.mul(x, fp32=True)
     np.testing.assert_allclose(y32, y_ref, rtol=2e-5, atol=2e-5 * np.abs(y_ref).max())
-    X = np.stack([x, 0.5 * x, rng.uniform(-1, 1, len(x))])
-    Y = ham.matrix.mul_block(X, fp32=True)
-    for v in range(3):
-        np.testing.assert_allclose(Y[v], ham.matrix.mul(X[v], fp32=True), rtol=1e-5, atol=1e-5)
+    # the FP32 block product (k = 3 and 4 vectors per pass) against the
+    # oracle's product of each vector
+    for k in (3, 4):
+        X = np.stack([x, 0.5 * x] + [rng.uniform(-1, 1, len(x)) for _ in range(k - 2)])
+        Y = ham.matrix.mul_block(X, fp32=True)
+        for v in range(k):
+            yo = y_ref if v == 0 else 0.5 * y_ref if v == 1 else s["orc"].mul(X[v])
+            assert np.linalg.norm(Y[v] - yo) <= 1e-5 * np.linalg.norm(yo), (k, v)
 
 
 @pytest.mark.gpu
