@@ -1278,6 +1278,7 @@ struct RdmParams {
   const double* __restrict__ c;  // n_states x n, internal order
   double* __restrict__ rdm1;
   double* __restrict__ rdm2;
+  int pair1;  // the pairs also write the 1-RDM (its diagonal and singles)
 };
 
 __device__ __forceinline__ double cc(const RdmParams& R, uint64_t i, uint64_t j) {
@@ -1313,14 +1314,39 @@ __device__ __forceinline__ double pf_of(uint64_t l, uint64_t rr) {
   return (c & 1) ? -1.0 : 1.0;
 }
 
-// rdm.cc:435-642 with connected = (uc, dc) at j and this = (ut, dt) at i.
+// 1-RDM terms of a single excitation between det i (this) and det j
+// (connected), both orientations (rdm.cc:105-124); nothing for other pairs.
+__device__ __forceinline__ void rdm1_single(const RdmParams& R, uint64_t ut, uint64_t dt, uint64_t i, uint64_t uc,
+                                            uint64_t dc, uint64_t j) {
+  uint64_t st, sc;
+  if (dt == dc && __popcll(ut ^ uc) == 2) {
+    st = ut;
+    sc = uc;
+  } else if (ut == uc && __popcll(dt ^ dc) == 2) {
+    st = dt;
+    sc = dc;
+  } else {
+    return;
+  }
+  const int n = R.n_orb;
+  const int a1 = ctz64(st & ~sc), a2 = ctz64(sc & ~st);
+  const double v = pf_of(sc, st) * cc(R, i, j);
+  atomicAdd(&R.rdm1[a1 * n + a2], v);
+  atomicAdd(&R.rdm1[a2 * n + a1], v);
+}
+
+// One stored pair (rdm.cc:435-642 with connected = (uc, dc) at j and this =
+// (ut, dt) at i; i = j for the diagonal).  The reference's 2-RDM runs over
+// the Hamiltonian's connections and its 1-RDM over the det list; the pairs
+// write the 1-RDM when every single of the list is among them (R.pair1),
+// and k_rdm1 does it otherwise.
 __device__ void rdm_pair(const RdmParams& R, uint64_t ut, uint64_t dt, uint64_t i, uint64_t uc, uint64_t dc,
                          uint64_t j) {
   const double cij = cc(R, i, j);
-  const int n = R.n_orb;
-  if (ut == uc) {
-    if (dt == dc) {
-      // 1-RDM diagonal: c_i^2 for each occupied orbital of either spin.
+  if (R.pair1) {
+    if (ut == uc && dt == dc) {
+      // 1-RDM diagonal: c_i^2 for each occupied orbital of either spin
+      const int n = R.n_orb;
       for (uint64_t m = ut; m; m &= m - 1) {
         const int p = ctz64(m);
         atomicAdd(&R.rdm1[p * n + p], cij);
@@ -1329,6 +1355,12 @@ __device__ void rdm_pair(const RdmParams& R, uint64_t ut, uint64_t dt, uint64_t 
         const int p = ctz64(m);
         atomicAdd(&R.rdm1[p * n + p], cij);
       }
+    } else {
+      rdm1_single(R, ut, dt, i, uc, dc, j);
+    }
+  }
+  if (ut == uc) {
+    if (dt == dc) {
       for (uint64_t a = ut; a; a &= a - 1)
         for (uint64_t b = a & (a - 1); b; b &= b - 1) {
           const int s = ctz64(a), r = ctz64(b);
@@ -1357,9 +1389,6 @@ __device__ void rdm_pair(const RdmParams& R, uint64_t ut, uint64_t dt, uint64_t 
     if (nd == 1) {
       const int b1 = ctz64(dt & ~dc), b2 = ctz64(dc & ~dt);  // from (this), to (connected)
       const double pf = pf_of(dc, dt);
-      // 1-RDM, both orientations (rdm.cc:105-124).
-      atomicAdd(&R.rdm1[b1 * n + b2], pf * cij);
-      atomicAdd(&R.rdm1[b2 * n + b1], pf * cij);
       for (uint64_t m = dt; m; m &= m - 1) {
         const int p = ctz64(m);
         if (p == b1 || p == b2) continue;
@@ -1403,8 +1432,6 @@ __device__ void rdm_pair(const RdmParams& R, uint64_t ut, uint64_t dt, uint64_t 
     const int a1 = ctz64(ut & ~uc), a2 = ctz64(uc & ~ut);
     if (dt == dc) {
       const double pf = pf_of(uc, ut);
-      atomicAdd(&R.rdm1[a1 * n + a2], pf * cij);
-      atomicAdd(&R.rdm1[a2 * n + a1], pf * cij);
       for (uint64_t m = ut; m; m &= m - 1) {
         const int p = ctz64(m);
         if (p == a1 || p == a2) continue;
@@ -3230,7 +3257,12 @@ __global__ __launch_bounds__(256) void k_seg_rdm(RdmParams R, const SegDesc* __r
     }
     const uint64_t ua = R.up[a], da = R.dn[a], ub = R.up[b], db = R.dn[b];
     if (!seg_owns(ty, cs, ty == kSegA ? ra : rb, ua, da, ub, db)) continue;
-    if (!keep_zeros && !(v64 ? v64[D.voff + k] != 0.0 : v32[D.voff + k] != 0.0f)) continue;
+    if (!keep_zeros && !(v64 ? v64[D.voff + k] != 0.0 : v32[D.voff + k] != 0.0f)) {
+      // a cut pair (|H| below 1e-12): not a connection, but a single of
+      // the det list still enters the 1-RDM
+      rdm1_single(R, ua, da, a, ub, db, b);
+      continue;
+    }
     rdm_pair(R, ua, da, a, ub, db, b);
   }
 }
@@ -3239,6 +3271,7 @@ __global__ void k_rdm_diag(RdmParams R) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < R.n) rdm_pair(R, R.up[i], R.dn[i], i, R.up[i], R.dn[i], i);
 }
+
 
 // ---------------------------------------------------------------------------
 // Host-side helpers
@@ -4575,6 +4608,62 @@ __device__ __forceinline__ uint64_t det_hash(uint64_t u, uint64_t d) {
   x *= 0x94D049BB133111EBull;
   x ^= x >> 31;
   return x;
+}
+
+// The dets as an open-addressing set of their indices (kDetNone when free;
+// at most half full), probed on det_hash.
+constexpr uint32_t kDetNone = 0xFFFFFFFFu;
+
+__global__ void k_det_index_insert(const uint64_t* __restrict__ up, const uint64_t* __restrict__ dn, uint64_t n,
+                                   uint32_t* __restrict__ tab, uint64_t mask) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t h = det_hash(up[i], dn[i]) & mask;
+  while (atomicCAS(&tab[h], kDetNone, (uint32_t)i) != kDetNone) h = (h + 1) & mask;
+}
+
+__device__ __forceinline__ uint32_t det_index_find(const uint32_t* __restrict__ tab, uint64_t mask,
+                                                   const uint64_t* __restrict__ up, const uint64_t* __restrict__ dn,
+                                                   uint64_t u, uint64_t d) {
+  uint64_t h = det_hash(u, d) & mask;
+  while (true) {
+    const uint32_t k = tab[h];
+    if (k == kDetNone || (up[k] == u && dn[k] == d)) return k;
+    h = (h + 1) & mask;
+  }
+}
+
+// 1-RDM (rdm.cc:12-131, get_1rdm) of a CSR build that drops cut pairs: for
+// each det i and each electron p of either spin, every orbital r of its spin
+// (r = p, or r empty in det i) whose det i with p -> r is in the list, as
+// det j, adds diff(i, j)'s permutation factor times sum_s c_s[i] c_s[j] to
+// element (p, r).  The singles are looked up in the det list, as the
+// reference does, whether or not the Hamiltonian stores the pair (|H_ij|
+// below 1e-12 is cut there).  The reference's orbital-symmetry filter holds
+// by itself for a selected list: its dets share one spatial symmetry, which
+// a single p -> r keeps only when p and r share an irrep.  One thread per
+// det.
+__global__ void k_rdm1(RdmParams R, const uint32_t* __restrict__ tab, uint64_t mask) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R.n) return;
+  const int n = R.n_orb;
+  const uint64_t u = R.up[i], d = R.dn[i];
+  const double cii = cc(R, i, i);
+  for (int spin = 0; spin < 2; spin++) {
+    const uint64_t sp = spin == 0 ? u : d;
+    for (uint64_t m = sp; m; m &= m - 1) {
+      const int p = ctz64(m);
+      atomicAdd(&R.rdm1[p * n + p], cii);
+      for (int r = 0; r < n; r++) {
+        if ((sp >> r) & 1ull) continue;
+        const uint64_t ns = (sp & ~(1ull << p)) | (1ull << r);
+        const uint32_t j = spin == 0 ? det_index_find(tab, mask, R.up, R.dn, ns, d)
+                                     : det_index_find(tab, mask, R.up, R.dn, u, ns);
+        if (j == kDetNone) continue;
+        atomicAdd(&R.rdm1[p * n + r], pf_of(sp, ns) * cc(R, i, j));
+      }
+    }
+  }
 }
 
 // S(p, r) of the singles queue (chem_system.cc:237-259): h_pr plus the
@@ -5927,11 +6016,13 @@ static int rdm_full(shci_ctx* ctx, uint32_t n_states, const double* coefs, doubl
   const uint64_t n2 = no * no * (no * no + 1) / 2;
   const hipStream_t st = ctx->stream;
   DevBuf cb, ci, r1, r2;
+  DevBuf ht;  // the dets' index set (1-RDM lookups)
   auto cleanup = [&]() {
     release(ctx, cb);
     release(ctx, ci);
     release(ctx, r1);
     release(ctx, r2);
+    release(ctx, ht);
   };
   int rc = SHCI_OK;
   if ((rc = ensure(ctx, cb, (size_t)n_states * n * 8)) || (rc = ensure(ctx, ci, (size_t)n_states * n * 8)) ||
@@ -5959,6 +6050,26 @@ static int rdm_full(shci_ctx* ctx, uint32_t n_states, const double* coefs, doubl
   R.c = P<double>(ci);
   R.rdm1 = P<double>(r1);
   R.rdm2 = P<double>(r2);
+  // The 2-RDM over the stored pairs.  The 1-RDM over the det list: every
+  // single of the list is a stored pair of a CSR build that keeps cut pairs,
+  // and an owned slot of the segment layout (cut or not); a CSR build that
+  // drops cut pairs looks the singles up instead (k_rdm1).
+  const bool keep_zeros = (ctx->build_flags & SHCI_KEEP_ZEROS) != 0;
+  R.pair1 = (ctx->segl || keep_zeros) ? 1 : 0;
+  if (!R.pair1) {
+    uint64_t H = 1024;
+    while (H < 2 * n) H <<= 1;
+    if ((rc = ensure(ctx, ht, H * 4)) != SHCI_OK) {
+      cleanup();
+      return rc;
+    }
+    if (hipMemsetAsync(ht.p, 0xFF, H * 4, st) != hipSuccess) {
+      cleanup();
+      return fail(ctx, SHCI_ERR_HIP, "rdm index set");
+    }
+    k_det_index_insert<<<div_up(n, 256), 256, 0, st>>>(R.up, R.dn, n, P<uint32_t>(ht), H - 1);
+    k_rdm1<<<div_up(n, 256), 256, 0, st>>>(R, P<uint32_t>(ht), H - 1);
+  }
   if (ctx->segl) {
     k_rdm_diag<<<div_up(n, 256), 256, 0, st>>>(R);
     if (ctx->n_seg)

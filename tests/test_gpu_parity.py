@@ -65,6 +65,10 @@ SYSTEMS = [
     ("n14_1_2", 14, 1, 2, 13, [1e-3, 1e-5]),
     # full space: mixed cores of up to 81 members (one-wavefront segments)
     ("fci_n11_2_2", 11, 2, 2, 17, None),
+    # full space at the largest orbital count (one uint64 per spin, orbital
+    # 63 occupied): one mixed core of all 4,096 dets, the largest segment
+    # the layout allows
+    ("fci_n64_1_1", 64, 1, 1, 19, None),
 ]
 
 
