@@ -34,13 +34,15 @@ def _canon_upper(rp, cols, vals, n):
     return key[order], vals[order]
 
 
-def _full_space(n, n_up, n_dn, h1, eri):
-    """Every determinant (HF first) with oracle Davidson coefficients."""
+def _full_space(n, n_up, n_dn, h1, eri, orbs=None):
+    """Every determinant over the orbitals ``orbs`` (all n by default; the
+    lowest ones occupied first) with oracle Davidson coefficients."""
     import itertools
 
-    ups = [sum(1 << k for k in c) for c in itertools.combinations(range(n), n_up)]
-    dns = [sum(1 << k for k in c) for c in itertools.combinations(range(n), n_dn)]
-    hf = ((1 << n_up) - 1, (1 << n_dn) - 1)
+    orbs = list(range(n)) if orbs is None else sorted(orbs)
+    ups = [sum(1 << k for k in c) for c in itertools.combinations(orbs, n_up)]
+    dns = [sum(1 << k for k in c) for c in itertools.combinations(orbs, n_dn)]
+    hf = (sum(1 << k for k in orbs[:n_up]), sum(1 << k for k in orbs[:n_dn]))
     dets = [hf] + [(u, d) for u in ups for d in dns if (u, d) != hf]
     up = np.array([d[0] for d in dets], dtype=np.uint64)
     dn = np.array([d[1] for d in dets], dtype=np.uint64)
@@ -69,6 +71,10 @@ SYSTEMS = [
     # 63 occupied): one mixed core of all 4,096 dets, the largest segment
     # the layout allows
     ("fci_n64_1_1", 64, 1, 1, 19, None),
+    # every 3+3 det over the top eight of 64 orbitals (56-63): 3,136 dets,
+    # orbital 63 in 21 of the 56 strings of each spin, two-electron A and B
+    # cores
+    ("sub_n64_3_3", 64, 3, 3, 29, ("orbs", [56, 57, 58, 59, 60, 61, 62, 63])),
 ]
 
 
@@ -78,6 +84,8 @@ def system(request):
     _, h1, eri, _ = si.synthetic(n, seed=seed)
     if eps is None:
         up, dn, orc, coefs = _full_space(n, nu, nd, h1, eri)
+    elif isinstance(eps, tuple):
+        up, dn, orc, coefs = _full_space(n, nu, nd, h1, eri, orbs=eps[1])
     else:
         up, dn, orc, coefs = _dets_from_selection(n, nu, nd, h1, eri, eps)
     return dict(label=label, n=n, nu=nu, nd=nd, h1=h1, eri=eri, up=up, dn=dn, orc=orc, coefs=coefs)
