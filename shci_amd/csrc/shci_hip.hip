@@ -4417,6 +4417,11 @@ struct Davidson {
       const double lam = lowest[i_state_precond];
       k_precond<T><<<g, 256, 0, st>>>(v[ic], hw[i_state_precond], w[i_state_precond], P<double>(ctx->diag), lam,
                                       shift, n);
+      // time_sym: the correction back onto the spin-flip-even vectors.  The
+      // diagonal of a det and of its reverse can differ in the last bit, and
+      // an odd component would meet the projected products' null space:
+      // eigenvalue 0, below every root of an all-positive spectrum.
+      if (ctx->ts) k_ts_sym<T><<<g, 256, 0, st>>>(v[ic], P<uint32_t>(ctx->ts_rev), n);
       // Modified Gram-Schmidt with device-resident scalars (davidson.cc:116-123).
       for (size_t i = 0; i < ic; i++) {
         const T* b[1] = {v[i]};

@@ -12,7 +12,23 @@ from shci_amd import integrals as si
 
 
 def _ts_dets(n, ne, h1, eri, eps_list):
-    """Host selection, then the canonical form (up <= dn), first occurrence kept."""
+    """Host selection (or, for ("orbs", list), every det over those orbitals,
+    the lowest ones occupied first), then the canonical form (up <= dn),
+    first occurrence kept."""
+    if isinstance(eps_list, tuple):
+        import itertools
+
+        orbs = sorted(eps_list[1])
+        strs = [sum(1 << k for k in c) for c in itertools.combinations(orbs, ne)]
+        hf = sum(1 << k for k in orbs[:ne])
+        dets = [(hf, hf)] + [(u, d) for u in strs for d in strs]
+        seen, out = set(), []
+        for u, d in dets:
+            c = (u, d) if u <= d else (d, u)
+            if c not in seen:
+                seen.add(c)
+                out.append(c)
+        return np.array([c[0] for c in out], dtype=np.uint64), np.array([c[1] for c in out], dtype=np.uint64)
     from shci_amd.host import HostSystem
 
     host = HostSystem.from_arrays(n, ne, ne, h1, eri, 0.0)
@@ -42,7 +58,9 @@ def _dense_ts(orc, up, dn):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("layout", ["csr", "segments"])
-@pytest.mark.parametrize("n,ne,seed,eps", [(12, 3, 7, [1e-2, 3e-3]), (14, 4, 11, [2e-2, 8e-3])])
+@pytest.mark.parametrize("n,ne,seed,eps", [(12, 3, 7, [1e-2, 3e-3]), (14, 4, 11, [2e-2, 8e-3]),
+                                            # every 3+3 det over the top seven of 64 orbitals
+                                            (64, 3, 29, ("orbs", [57, 58, 59, 60, 61, 62, 63]))])
 def test_gpu_time_sym_vs_oracle(layout, n, ne, seed, eps):
     from shci_amd.hamiltonian import Davidson, Hamiltonian
 
