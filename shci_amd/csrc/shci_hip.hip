@@ -4615,25 +4615,31 @@ __device__ __forceinline__ uint64_t det_hash(uint64_t u, uint64_t d) {
   return x;
 }
 
-// The dets as an open-addressing set of their indices (kDetNone when free;
-// at most half full), probed on det_hash.
-constexpr uint32_t kDetNone = 0xFFFFFFFFu;
+// The dets as an open-addressing set (at most a quarter full) on det_hash:
+// a slot holds the high 32 bits of the det's hash above its index, ~0 when
+// free, so that a probe loads the det's strings only when the hashes agree.
+constexpr uint64_t kDetNone = ~0ull;
 
 __global__ void k_det_index_insert(const uint64_t* __restrict__ up, const uint64_t* __restrict__ dn, uint64_t n,
-                                   uint32_t* __restrict__ tab, uint64_t mask) {
+                                   unsigned long long* __restrict__ tab, uint64_t mask) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint64_t h = det_hash(up[i], dn[i]) & mask;
-  while (atomicCAS(&tab[h], kDetNone, (uint32_t)i) != kDetNone) h = (h + 1) & mask;
+  const uint64_t hv = det_hash(up[i], dn[i]);
+  const unsigned long long e = (hv & 0xFFFFFFFF00000000ull) | i;
+  uint64_t h = hv & mask;
+  while (atomicCAS(&tab[h], (unsigned long long)kDetNone, e) != (unsigned long long)kDetNone) h = (h + 1) & mask;
 }
 
-__device__ __forceinline__ uint32_t det_index_find(const uint32_t* __restrict__ tab, uint64_t mask,
+__device__ __forceinline__ uint32_t det_index_find(const unsigned long long* __restrict__ tab, uint64_t mask,
                                                    const uint64_t* __restrict__ up, const uint64_t* __restrict__ dn,
                                                    uint64_t u, uint64_t d) {
-  uint64_t h = det_hash(u, d) & mask;
+  const uint64_t hv = det_hash(u, d);
+  uint64_t h = hv & mask;
   while (true) {
-    const uint32_t k = tab[h];
-    if (k == kDetNone || (up[k] == u && dn[k] == d)) return k;
+    const uint64_t e = tab[h];
+    if (e == kDetNone) return 0xFFFFFFFFu;
+    const uint32_t k = (uint32_t)e;
+    if ((e >> 32) == (hv >> 32) && up[k] == u && dn[k] == d) return k;
     h = (h + 1) & mask;
   }
 }
@@ -4647,28 +4653,40 @@ __device__ __forceinline__ uint32_t det_index_find(const uint32_t* __restrict__ 
 // below 1e-12 is cut there).  The reference's orbital-symmetry filter holds
 // by itself for a selected list: its dets share one spatial symmetry, which
 // a single p -> r keeps only when p and r share an irrep.  One thread per
-// det.
-__global__ void k_rdm1(RdmParams R, const uint32_t* __restrict__ tab, uint64_t mask) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= R.n) return;
+// (det, electron); each workgroup sums into an LDS copy of the 1-RDM
+// (n_orb^2 doubles of dynamic LDS) and adds it to the result at the end.
+__global__ __launch_bounds__(256) void k_rdm1(RdmParams R, const unsigned long long* __restrict__ tab, uint64_t mask,
+                                              uint32_t ne) {
+  extern __shared__ __align__(16) unsigned char rdm1_lds[];
+  double* acc = reinterpret_cast<double*>(rdm1_lds);
   const int n = R.n_orb;
-  const uint64_t u = R.up[i], d = R.dn[i];
-  const double cii = cc(R, i, i);
-  for (int spin = 0; spin < 2; spin++) {
-    const uint64_t sp = spin == 0 ? u : d;
-    for (uint64_t m = sp; m; m &= m - 1) {
-      const int p = ctz64(m);
-      atomicAdd(&R.rdm1[p * n + p], cii);
-      for (int r = 0; r < n; r++) {
-        if ((sp >> r) & 1ull) continue;
-        const uint64_t ns = (sp & ~(1ull << p)) | (1ull << r);
-        const uint32_t j = spin == 0 ? det_index_find(tab, mask, R.up, R.dn, ns, d)
-                                     : det_index_find(tab, mask, R.up, R.dn, u, ns);
-        if (j == kDetNone) continue;
-        atomicAdd(&R.rdm1[p * n + r], pf_of(sp, ns) * cc(R, i, j));
-      }
+  const uint32_t nn = (uint32_t)(n * n);
+  for (uint32_t k = threadIdx.x; k < nn; k += blockDim.x) acc[k] = 0.0;
+  __syncthreads();
+  const uint64_t total = R.n * ne;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = t / ne;
+    const uint32_t e = (uint32_t)(t - i * ne);
+    const uint64_t u = R.up[i], d = R.dn[i];
+    const uint32_t nu = (uint32_t)__popcll(u);
+    const bool alpha = e < nu;
+    const uint64_t sp = alpha ? u : d;
+    // the electron's orbital: the (e or e - nu)-th set bit of its string
+    uint64_t m = sp;
+    for (uint32_t q = alpha ? e : e - nu; q > 0; q--) m &= m - 1;
+    const int p = ctz64(m);
+    atomicAdd(&acc[p * n + p], cc(R, i, i));
+    for (int r = 0; r < n; r++) {
+      if ((sp >> r) & 1ull) continue;
+      const uint64_t ns = (sp & ~(1ull << p)) | (1ull << r);
+      const uint32_t j = alpha ? det_index_find(tab, mask, R.up, R.dn, ns, d) : det_index_find(tab, mask, R.up, R.dn, u, ns);
+      if (j == 0xFFFFFFFFu) continue;
+      atomicAdd(&acc[p * n + r], pf_of(sp, ns) * cc(R, i, j));
     }
   }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < nn; k += blockDim.x)
+    if (acc[k] != 0.0) atomicAdd(&R.rdm1[k], acc[k]);
 }
 
 // S(p, r) of the singles queue (chem_system.cc:237-259): h_pr plus the
@@ -6063,17 +6081,25 @@ static int rdm_full(shci_ctx* ctx, uint32_t n_states, const double* coefs, doubl
   R.pair1 = (ctx->segl || keep_zeros) ? 1 : 0;
   if (!R.pair1) {
     uint64_t H = 1024;
-    while (H < 2 * n) H <<= 1;
-    if ((rc = ensure(ctx, ht, H * 4)) != SHCI_OK) {
+    while (H < 4 * n) H <<= 1;
+    if ((rc = ensure(ctx, ht, H * 8)) != SHCI_OK) {
       cleanup();
       return rc;
     }
-    if (hipMemsetAsync(ht.p, 0xFF, H * 4, st) != hipSuccess) {
+    if (hipMemsetAsync(ht.p, 0xFF, H * 8, st) != hipSuccess) {
       cleanup();
       return fail(ctx, SHCI_ERR_HIP, "rdm index set");
     }
-    k_det_index_insert<<<div_up(n, 256), 256, 0, st>>>(R.up, R.dn, n, P<uint32_t>(ht), H - 1);
-    k_rdm1<<<div_up(n, 256), 256, 0, st>>>(R, P<uint32_t>(ht), H - 1);
+    uint64_t u0 = 0, d0 = 0;
+    if (hipMemcpy(&u0, ctx->up_s.p, 8, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&d0, ctx->dn_s.p, 8, hipMemcpyDeviceToHost) != hipSuccess) {
+      cleanup();
+      return fail(ctx, SHCI_ERR_HIP, "rdm electron count");
+    }
+    const uint32_t ne = (uint32_t)(__builtin_popcountll(u0) + __builtin_popcountll(d0));
+    k_det_index_insert<<<div_up(n, 256), 256, 0, st>>>(R.up, R.dn, n, P<unsigned long long>(ht), H - 1);
+    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(div_up(n * ne, 256), 2048));
+    k_rdm1<<<blocks, 256, no * no * 8, st>>>(R, P<unsigned long long>(ht), H - 1, ne);
   }
   if (ctx->segl) {
     k_rdm_diag<<<div_up(n, 256), 256, 0, st>>>(R);
