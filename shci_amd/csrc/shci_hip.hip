@@ -1335,56 +1335,16 @@ __device__ __forceinline__ void rdm1_single(const RdmParams& R, uint64_t ut, uin
   atomicAdd(&R.rdm1[a2 * n + a1], v);
 }
 
-// One stored pair (rdm.cc:435-642 with connected = (uc, dc) at j and this =
-// (ut, dt) at i; i = j for the diagonal).  The reference's 2-RDM runs over
-// the Hamiltonian's connections and its 1-RDM over the det list; the pairs
-// write the 1-RDM when every single of the list is among them (R.pair1),
-// and k_rdm1 does it otherwise.
+// One stored pair i != j (rdm.cc:435-642 with connected = (uc, dc) at j and
+// this = (ut, dt) at i; the diagonal pairs are k_rdm_diag's).  The
+// reference's 2-RDM runs over the Hamiltonian's connections and its 1-RDM
+// over the det list; the pairs write the 1-RDM when every single of the list
+// is among them (R.pair1), and k_rdm1 does it otherwise.
 __device__ void rdm_pair(const RdmParams& R, uint64_t ut, uint64_t dt, uint64_t i, uint64_t uc, uint64_t dc,
                          uint64_t j) {
   const double cij = cc(R, i, j);
-  if (R.pair1) {
-    if (ut == uc && dt == dc) {
-      // 1-RDM diagonal: c_i^2 for each occupied orbital of either spin
-      const int n = R.n_orb;
-      for (uint64_t m = ut; m; m &= m - 1) {
-        const int p = ctz64(m);
-        atomicAdd(&R.rdm1[p * n + p], cij);
-      }
-      for (uint64_t m = dt; m; m &= m - 1) {
-        const int p = ctz64(m);
-        atomicAdd(&R.rdm1[p * n + p], cij);
-      }
-    } else {
-      rdm1_single(R, ut, dt, i, uc, dc, j);
-    }
-  }
+  if (R.pair1) rdm1_single(R, ut, dt, i, uc, dc, j);
   if (ut == uc) {
-    if (dt == dc) {
-      for (uint64_t a = ut; a; a &= a - 1)
-        for (uint64_t b = a & (a - 1); b; b &= b - 1) {
-          const int s = ctz64(a), r = ctz64(b);
-          w2(R, s, r, r, s, 1.0, cij);
-          w2(R, s, r, s, r, -1.0, cij);
-          w2(R, r, s, s, r, 1.0, cij);
-          w2(R, r, s, r, s, -1.0, cij);
-        }
-      for (uint64_t a = dt; a; a &= a - 1)
-        for (uint64_t b = a & (a - 1); b; b &= b - 1) {
-          const int s = ctz64(a), r = ctz64(b);
-          w2(R, s, r, r, s, 1.0, cij);
-          w2(R, s, r, s, r, -1.0, cij);
-          w2(R, r, s, s, r, 1.0, cij);
-          w2(R, r, s, r, s, -1.0, cij);
-        }
-      for (uint64_t a = ut; a; a &= a - 1)
-        for (uint64_t b = dt; b; b &= b - 1) {
-          const int s = ctz64(a), r = ctz64(b);
-          w2(R, s, r, r, s, 1.0, cij);
-          w2(R, r, s, s, r, 1.0, cij);
-        }
-      return;
-    }
     const int nd = __popcll(dc ^ dt) / 2;
     if (nd == 1) {
       const int b1 = ctz64(dt & ~dc), b2 = ctz64(dc & ~dt);  // from (this), to (connected)
@@ -1483,7 +1443,6 @@ __global__ __launch_bounds__(256) void k_rdm(RdmParams R) {
   const uint64_t r = (uint64_t)xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   if (r >= R.n) return;
   const uint64_t ut = R.up[r], dt = R.dn[r];
-  if (lane_id() == 0) rdm_pair(R, ut, dt, r, ut, dt, r);
   for (int64_t k = R.rp[r] + lane_id(); k < R.re[r]; k += kWave) {
     const uint32_t j = R.col[k];
     if (j <= r) continue;
@@ -3267,9 +3226,68 @@ __global__ __launch_bounds__(256) void k_seg_rdm(RdmParams R, const SegDesc* __r
   }
 }
 
-__global__ void k_rdm_diag(RdmParams R) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < R.n) rdm_pair(R, R.up[i], R.dn[i], i, R.up[i], R.dn[i], i);
+// The diagonal pairs (i = j) of the RDMs, one thread per det: what
+// rdm_pair's i = j branch writes, summed per workgroup in LDS and then added
+// to the results once.  Its 2-RDM elements are of two kinds, (ss, rr) and
+// (sr, rs) for occupied s, r (packed a = s n + s, b = r n + r, or a = s n +
+// r, b = r n + s, stored when a >= b), kept in two triangular arrays C and X
+// indexed by (hi, lo), hi >= lo; the 1-RDM diagonal when the pairs write the
+// 1-RDM.  Dynamic LDS: (n (n + 1) + n) doubles.
+__global__ __launch_bounds__(256) void k_rdm_diag(RdmParams R) {
+  extern __shared__ __align__(16) unsigned char rdiag_lds[];
+  const int n = R.n_orb;
+  const uint32_t nt = (uint32_t)(n * (n + 1) / 2);
+  double* C = reinterpret_cast<double*>(rdiag_lds);
+  double* X = C + nt;
+  double* D1 = X + nt;
+  for (uint32_t k = threadIdx.x; k < 2 * nt + (uint32_t)n; k += blockDim.x) C[k] = 0.0;
+  __syncthreads();
+  auto tri = [](int hi, int lo) { return (uint32_t)(hi * (hi + 1) / 2 + lo); };
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < R.n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t u = R.up[i], d = R.dn[i];
+    const double cii = cc(R, i, i);
+    if (R.pair1) {
+      for (uint64_t m = u; m; m &= m - 1) atomicAdd(&D1[ctz64(m)], cii);
+      for (uint64_t m = d; m; m &= m - 1) atomicAdd(&D1[ctz64(m)], cii);
+    }
+    // same-spin pairs s < r: w2(r, s, s, r, +1) and w2(r, s, r, s, -1) are
+    // the stored ones
+    for (int spin = 0; spin < 2; spin++) {
+      const uint64_t sp = spin == 0 ? u : d;
+      for (uint64_t a = sp; a; a &= a - 1)
+        for (uint64_t b = a & (a - 1); b; b &= b - 1) {
+          const int lo = ctz64(a), hi = ctz64(b);
+          atomicAdd(&C[tri(hi, lo)], cii);
+          atomicAdd(&X[tri(hi, lo)], -cii);
+        }
+    }
+    // opposite-spin pairs (s alpha, r beta): w2(s, r, r, s) stored when s >=
+    // r, w2(r, s, s, r) when r >= s
+    for (uint64_t a = u; a; a &= a - 1)
+      for (uint64_t b = d; b; b &= b - 1) {
+        const int sa = ctz64(a), rb = ctz64(b);
+        if (sa >= rb) atomicAdd(&C[tri(sa, rb)], cii);
+        if (rb >= sa) atomicAdd(&C[tri(rb, sa)], cii);
+      }
+  }
+  __syncthreads();
+  const uint64_t nn = (uint64_t)n;
+  for (uint32_t k = threadIdx.x; k < nt; k += blockDim.x) {
+    int hi = 0;
+    while ((uint32_t)((hi + 1) * (hi + 2) / 2) <= k) hi++;
+    const int lo = (int)(k - (uint32_t)(hi * (hi + 1) / 2));
+    if (C[k] != 0.0) {
+      const uint64_t a = (uint64_t)hi * nn + hi, b = (uint64_t)lo * nn + lo;
+      atomicAdd(&R.rdm2[(a * (a + 1)) / 2 + b], C[k]);
+    }
+    if (X[k] != 0.0) {
+      const uint64_t a = (uint64_t)hi * nn + lo, b = (uint64_t)lo * nn + hi;
+      atomicAdd(&R.rdm2[(a * (a + 1)) / 2 + b], X[k]);
+    }
+  }
+  if (R.pair1)
+    for (uint32_t k = threadIdx.x; k < (uint32_t)n; k += blockDim.x)
+      if (D1[k] != 0.0) atomicAdd(&R.rdm1[k * nn + k], D1[k]);
 }
 
 
@@ -6101,8 +6119,8 @@ static int rdm_full(shci_ctx* ctx, uint32_t n_states, const double* coefs, doubl
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(div_up(n * ne, 256), 2048));
     k_rdm1<<<blocks, 256, no * no * 8, st>>>(R, P<unsigned long long>(ht), H - 1, ne);
   }
+  k_rdm_diag<<<(uint32_t)std::min<uint64_t>(div_up(n, 256), 2048), 256, (no * (no + 1) + no) * 8, st>>>(R);
   if (ctx->segl) {
-    k_rdm_diag<<<div_up(n, 256), 256, 0, st>>>(R);
     if (ctx->n_seg)
       k_seg_rdm<<<div_up(ctx->n_seg, 4), 256, 0, st>>>(
           R, P<SegDesc>(ctx->seg_desc), P<uint32_t>(ctx->seg_mem), P<uint8_t>(ctx->seg_type),
