@@ -14,6 +14,17 @@ from shci_amd import integrals as si
 
 
 def _host_dets(n, nu, nd, h1, eri, eps_list):
+    """The host selection's dets at eps_list, or for ("orbs", list) every det
+    over those orbitals (the lowest ones occupied first)."""
+    if isinstance(eps_list, tuple):
+        import itertools
+
+        orbs = sorted(eps_list[1])
+        ups = [sum(1 << k for k in c) for c in itertools.combinations(orbs, nu)]
+        dns = [sum(1 << k for k in c) for c in itertools.combinations(orbs, nd)]
+        hf = (sum(1 << k for k in orbs[:nu]), sum(1 << k for k in orbs[:nd]))
+        dets = [hf] + [(u, d) for u in ups for d in dns if (u, d) != hf]
+        return (np.array([d[0] for d in dets], dtype=np.uint64), np.array([d[1] for d in dets], dtype=np.uint64))
     from shci_amd.host import HostSystem
 
     host = HostSystem.from_arrays(n, nu, nd, h1, eri, 0.0)
@@ -32,6 +43,8 @@ def _host_dets(n, nu, nd, h1, eri, eps_list):
     (20, 5, 5, 1001, [1e-2, 2e-3]),
     (33, 4, 3, 5, [3e-2, 1e-2]),
     (64, 2, 2, 9, [1e-2, 1e-3]),
+    # every 2+2 det over the top six of 64 orbitals
+    (64, 2, 2, 9, ("orbs", [58, 59, 60, 61, 62, 63])),
 ])
 def test_gpu_pt_dtm_vs_oracle(n, nu, nd, seed, eps_list):
     from shci_amd.hamiltonian import Hamiltonian
