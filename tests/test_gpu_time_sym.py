@@ -142,13 +142,15 @@ def test_gpu_time_sym_append_and_checks():
 
 
 @pytest.mark.gpu
-def test_gpu_select_time_sym_vs_oracle():
+@pytest.mark.parametrize("n,ne,seed,dets", [(14, 4, 11, [2e-2, 5e-3]),
+                                             # every canonical 2+2 det over the top six of 64 orbitals
+                                             (64, 2, 9, ("orbs", [58, 59, 60, 61, 62, 63]))])
+def test_gpu_select_time_sym_vs_oracle(n, ne, seed, dets):
     from shci_amd.hamiltonian import Hamiltonian
     from shci_amd.selection import Selector
 
-    n, ne = 14, 4
-    _, h1, eri, core = si.synthetic(n, seed=11)
-    up, dn = _ts_dets(n, ne, h1, eri, [2e-2, 5e-3])
+    _, h1, eri, core = si.synthetic(n, seed=seed)
+    up, dn = _ts_dets(n, ne, h1, eri, dets)
     rng = np.random.default_rng(5)
     c = rng.choice([-1.0, 1.0], (2, len(up))) * 10.0 ** rng.uniform(-5, 0, (2, len(up)))
     c[0, 0] = 0.95
