@@ -156,9 +156,11 @@ int shci_davidson(shci_ctx* ctx, uint32_t n_states, const double* init, double t
                   uint32_t max_calls, double* evals, double* evecs, int* converged, double* traj_e,
                   double* traj_t, int traj_cap, int* n_traj);
 
-/* RDM::get_1rdm (rdm.cc:12-131) and RDM::get_2rdm(const SparseMatrix&)
- * (rdm.cc:374-395) over the built connections, FP64 accumulation, states
- * summed (rdm.cc:227,652).  coefs: n_states x n_dets (caller order).
+/* RDM::get_1rdm (rdm.cc:12-131) over every single excitation of the det
+ * list, whether or not the matrix stores the pair (|H| < 1e-12 is cut
+ * there), and RDM::get_2rdm(const SparseMatrix&) (rdm.cc:374-395) over the
+ * built connections; FP64 accumulation, states summed (rdm.cc:227,652).
+ * coefs: n_states x n_dets (caller order).
  * rdm1: n x n; rdm2: packed n^2(n^2+1)/2 in the reference layout
  * (index a = p*n+s, b = q*n+r, a >= b, rdm.cc:644-657).  Either may be NULL.
  * This is the adjoint of E(h, v): dE/dh = rdm1, dE/d(ps|qr) = D_pqrs / 2. */
