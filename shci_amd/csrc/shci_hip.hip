@@ -3304,14 +3304,6 @@ int radix_sort_pairs(shci_ctx* ctx, const KT* kin, KT* kout, const uint32_t* vin
   return SHCI_OK;
 }
 
-int sort_keys_u64(shci_ctx* ctx, const uint64_t* kin, uint64_t* kout, uint64_t n, int end_bit) {
-  size_t tb = 0;
-  HIPC(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, kin, kout, (int)n, 0, end_bit, ctx->stream));
-  CHECK_RC(ensure(ctx, ctx->tmp, tb));
-  HIPC(hipcub::DeviceRadixSort::SortKeys(ctx->tmp.p, tb, kin, kout, (int)n, 0, end_bit, ctx->stream));
-  return SHCI_OK;
-}
-
 // Prefix sums: rocPRIM's lookback scan with the block shapes of its gfx942
 // tuning for 4- and 8-byte sums (256 threads x 24 or 15 items, reduce-then-
 // scan blocks); on the MI355X they beat the shapes it picks for gfx950 (the
@@ -4015,13 +4007,14 @@ int launch_seg_spmv(shci_ctx* ctx, double shift, const XT* x, XT* y, XT* xb_ext 
   }
   if constexpr (NV == 1) {
     // m <= 64: one workgroup per tile
-    if (ctx->n_tiles)
+    if (ctx->n_tiles) {
       if constexpr (std::is_same<VT, float>::value && std::is_same<XT, float>::value)
         k_seg_spmv_tiles_f32<<<ctx->n_tiles, 256, 0, st>>>(P<uint32_t>(ctx->tile_roff), P<uint4>(ctx->tile_rec),
                                                             P<uint32_t>(ctx->tile_tab), val, x, y, xb, yb);
       else
         k_seg_spmv_tiles<VT, XT><<<ctx->n_tiles, 256, 0, st>>>(P<uint32_t>(ctx->tile_roff), P<uint4>(ctx->tile_rec),
                                                                P<uint32_t>(ctx->tile_tab), val, x, y, xb, yb);
+    }
   }
   if (have_a) k_perm_add<NV, XT><<<div_up(n, 256), 256, 0, st>>>(dnord, yb, y, n);
   HIPC(hipGetLastError());
