@@ -371,6 +371,42 @@ def test_multiroot_davidson_vs_exact_roots(system, layout):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("layout", ["csr", "segments"])
+def test_mixed_multiroot_degenerate_pair(layout):
+    """Every 3+3 det over orbitals 0, 1, 30 and 59-63 of 64 (3,136 dets): the
+    second and third roots are degenerate to 2.5e-10, the states of two
+    5-det clusters with the same spectrum that couple to each other at
+    1.8e-9 Ha and to the rest at most 8.2e-5 Ha.  The start vectors (dets
+    0-2) lie outside the partner's cluster.  With FP32 products the
+    eigenvalue-change test (davidson.cc:174-186) is met by stagnation with
+    the second root ~5e-3 Ha high (a numpy restatement of the reference
+    Davidson with FP32 products stops there too); the mixed mode's residual
+    guard restarts from the Ritz vectors and reaches all three roots."""
+    from shci_amd.hamiltonian import Davidson, Hamiltonian
+
+    n_orb, orbs = 64, [0, 1, 30, 59, 60, 61, 62, 63]
+    _, h1, eri, _ = si.synthetic(n_orb, seed=29)
+    up, dn, orc, _ = _full_space(n_orb, 3, 3, h1, eri, orbs=orbs)
+    n = len(up)
+    w = _lowest_three(dict(label="degenerate_n64_3_3", up=up, dn=dn, orc=orc))
+    assert w[2] - w[1] < 1e-9  # the pair is degenerate
+    ham = Hamiltonian(n_orb, h1, eri, 0.0, store="both", layout=layout)
+    ham.update(up, dn)
+    print(f"degenerate pair {layout}: {n} dets, roots {w}", flush=True)
+    init = np.zeros((3, n))
+    for k in range(3):
+        init[k, k] = 1.0
+    d = Davidson(3)
+    d.diagonalize(ham.matrix, init, 1e-12, mode="fp64", max_calls=100)
+    np.testing.assert_allclose(d.lowest_eigenvalues, w, rtol=0, atol=1e-9)
+    dm = Davidson(3)
+    dm.diagonalize(ham.matrix, init, 1e-10, mode="mixed", max_calls=100)
+    print(f"  mixed {dm.lowest_eigenvalues} converged {dm.converged}", flush=True)
+    assert dm.converged
+    np.testing.assert_allclose(dm.lowest_eigenvalues, w, rtol=0, atol=1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("layout", ["csr", "segments"])
 def test_ritz_capture_fp64_energies(system, layout):
     """The captured Ritz vectors (shci_capture_ritz) and their FP64 energies
     (shci_ritz_energies): in FP64 mode each trajectory entry is a Rayleigh
