@@ -151,7 +151,11 @@ int shci_spmm(shci_ctx* ctx, const double* X, double* Y, uint32_t k, uint32_t fl
  * (solver.h:430-530); max_calls > 1 repeats that warm restart here until
  * converged.  traj_e/traj_t (optional, capacity traj_cap) receive the
  * ground-state estimate and elapsed seconds after every Rayleigh-Ritz step;
- * in MIXED mode the last entry is the FP64 re-evaluated energy. */
+ * in MIXED mode the last entry is the FP64 re-evaluated energy.  MIXED mode
+ * also accepts a call's convergence only if every Ritz vector's residual
+ * |Hc - Ec| (FP64 vectors) is at most max(10 sqrt(tol), 1e-4); otherwise
+ * the call counts as unconverged and is warm-restarted while calls remain
+ * (*converged = 0 if none remain). */
 int shci_davidson(shci_ctx* ctx, uint32_t n_states, const double* init, double tol, uint32_t mode,
                   uint32_t max_calls, double* evals, double* evecs, int* converged, double* traj_e,
                   double* traj_t, int traj_cap, int* n_traj);
