@@ -120,6 +120,63 @@ def test_gpu_time_sym_vs_oracle(layout, n, ne, seed, eps):
     np.testing.assert_allclose(r2, o2, rtol=0, atol=1e-12)
 
 
+def _b(*orbs):
+    return sum(1 << k for k in orbs)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("layout", ["csr", "segments"])
+@pytest.mark.parametrize("label,ups,dns", [
+    # one det that is its own reverse: H_ts = H
+    ("self", [_b(0, 1, 2)], [_b(0, 1, 2)]),
+    # one det and its reverse: a one-dimensional H_ts from a two-det list
+    ("pair", [_b(0, 1, 2)], [_b(0, 1, 5)]),
+    # the two together: the self det couples to the pair state with a sqrt2
+    ("self_and_pair", [_b(0, 1, 2), _b(0, 1, 2)], [_b(0, 1, 2), _b(0, 1, 5)]),
+])
+def test_gpu_time_sym_tiny_lists(layout, label, ups, dns):
+    """The smallest symmetrised lists against the oracle's dense H_ts: the
+    diagonal, the product, FP64 and mixed eigenvalues, and the RDMs of the
+    unpacked wavefunction."""
+    from shci_amd.hamiltonian import Davidson, Hamiltonian
+
+    n, ne = 12, 3
+    _, h1, eri, core = si.synthetic(n, seed=7)
+    up, dn = np.array(ups, dtype=np.uint64), np.array(dns, dtype=np.uint64)
+    m = len(up)
+    orc = oracle.System.from_arrays(n, ne, ne, h1, eri, core)
+    H = _dense_ts(orc, up, dn)
+    print(f"[ts tiny] {label} {layout}: {m} symmetrised dets", flush=True)
+    ham = Hamiltonian(n, h1, eri, core, store="both", layout=layout, time_sym=True)
+    ham.update(up, dn)
+    assert ham.matrix.count_n_rows() == m
+    np.testing.assert_allclose(ham.matrix.get_diag(), np.diag(H), rtol=0, atol=1e-12)
+    x = np.linspace(0.5, -1.0, m) + 0.1
+    np.testing.assert_allclose(ham.matrix.mul(x), H @ x, rtol=0, atol=1e-11)
+    w = np.linalg.eigvalsh(H)
+    init = np.zeros(m)
+    init[0] = 1.0
+    d = Davidson(1)
+    d.diagonalize(ham.matrix, init, 1e-11, max_calls=40)
+    assert abs(d.lowest_eigenvalues[0] - w[0]) < 1e-9, (d.lowest_eigenvalues[0], w[0])
+    dm = Davidson(1)
+    dm.diagonalize(ham.matrix, init, 1e-10, mode="mixed", max_calls=40)
+    assert abs(dm.lowest_eigenvalues[0] - w[0]) < 1e-6
+    c = np.atleast_2d(d.get_lowest_eigenvectors())[0]
+    assert abs(c @ c - 1.0) < 1e-8
+    r1, r2 = ham.rdm(c[None])
+    eu, ed, ec = [], [], []
+    for u, dd, ci in zip(up.tolist(), dn.tolist(), c.tolist()):
+        if u == dd:
+            eu.append(u); ed.append(dd); ec.append(ci)
+        else:
+            eu += [u, dd]; ed += [dd, u]; ec += [ci * 0.7071067811865475] * 2
+    orc.update(np.array(eu, dtype=np.uint64), np.array(ed, dtype=np.uint64))
+    o1, o2 = orc.rdm(np.array(ec)[None])
+    np.testing.assert_allclose(r1, o1, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(r2, o2, rtol=0, atol=1e-12)
+
+
 @pytest.mark.gpu
 def test_gpu_time_sym_append_and_checks():
     from shci_amd.hamiltonian import Hamiltonian, ShciError
