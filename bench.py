@@ -381,8 +381,8 @@ def main():
                          "directions on the --config workload (use --config 4), one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU row sharding is "
-              "design-only in this build.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); sharding the rows over several "
+              "GPUs is not implemented.", file=sys.stderr)
         sys.exit(2)
 
     from shci_amd import workloads

@@ -13,7 +13,7 @@
 //                         std::vector<double>& eri_packed, double& core);
 //   static bool time_sym(const S&);
 // (for the reference's ChemSystem: get_n_dets(), dets[i].up/.dn chunk 0 and
-// Integrals::get_1b/get_2b over the combine4 keys).
+// Integrals::get_1b/get_2b over the combine4 keys, src/chem/integrals.h:42-44).
 // Errors throw std::runtime_error like the reference.
 #pragma once
 

@@ -114,7 +114,7 @@ def grow_det_list(cfg: int, target: int | None = None, device: int = 0, verbose:
     w.det_hash = det_hash(w.up, w.dn)
     # No clear: the context keeps the growth's device blocks, and the next
     # build of this list (on the same context) reuses them instead of asking
-    # the driver for memory again.
+    # the HIP runtime for memory again.
     del ham
     if verbose:
         print(f"[workload cfg{cfg}] {len(w.up):,} dets in {time.time() - t_start:.1f}s", file=sys.stderr, flush=True)
